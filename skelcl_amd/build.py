@@ -1,0 +1,90 @@
+"""Build recipe of libskelcl_amd.so (hipcc, gfx950).
+
+  python -m skelcl_amd.build          # or skelcl_amd.build.build()
+
+Steps: embed the kernel templates (csrc/kernels/*.hip) as C strings, compile
+every csrc/*.cpp and csrc/native/*.hip with hipcc --offload-arch=gfx950 and
+link the shared library in-tree next to this file.
+"""
+from __future__ import annotations
+
+import concurrent.futures as cf
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+CSRC = os.path.join(HERE, "csrc")
+BUILD = os.path.join(ROOT, "_build", "skelcl_amd")
+LIB = os.path.join(HERE, "libskelcl_amd.so")
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ARCH = os.environ.get("SKELCL_ARCH", "gfx950")
+COMMON = [
+    "-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function",
+    f"--offload-arch={ARCH}",
+    # packed FP32 instructions stay off in every device code object
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-I", os.path.join(ROOT, "include"), "-I", CSRC,
+]
+
+TEMPLATES = ["prelude", "map", "reduce", "scan", "mapoverlap", "allpairs"]
+
+
+def _embed() -> str:
+    out = os.path.join(BUILD, "embedded_kernels.cpp")
+    parts = ["// generated by skelcl_amd/build.py from csrc/kernels/*.hip\n"]
+    for name in TEMPLATES:
+        with open(os.path.join(CSRC, "kernels", name + ".hip")) as f:
+            text = f.read()
+        assert ")SKLRAW\"" not in text
+        parts.append(f'extern "C" const char* skl_tmpl_{name} = R"SKLRAW({text})SKLRAW";\n')
+    data = "".join(parts)
+    old = None
+    if os.path.exists(out):
+        with open(out) as f:
+            old = f.read()
+    if old != data:
+        with open(out, "w") as f:
+            f.write(data)
+    return out
+
+
+def _sources() -> list[str]:
+    srcs = [os.path.join(CSRC, f) for f in sorted(os.listdir(CSRC)) if f.endswith(".cpp")]
+    nat = os.path.join(CSRC, "native")
+    srcs += [os.path.join(nat, f) for f in sorted(os.listdir(nat)) if f.endswith(".hip")]
+    return srcs
+
+
+def _compile(src: str) -> str:
+    obj = os.path.join(BUILD, os.path.basename(src) + ".o")
+    lang = ["-x", "hip"]
+    cmd = [HIPCC, *COMMON, *lang, "-c", src, "-o", obj]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"compile failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    return obj
+
+
+def build(verbose: bool = False) -> str:
+    os.makedirs(BUILD, exist_ok=True)
+    srcs = _sources() + [_embed()]
+    jobs = min(len(srcs), max(1, min(8, os.cpu_count() or 1)))
+    with cf.ThreadPoolExecutor(jobs) as ex:
+        objs = list(ex.map(_compile, srcs))
+    tmp = LIB + ".tmp"
+    cmd = [HIPCC, *COMMON, "-shared", "-o", tmp, *objs, "-lhiprtc", "-lpthread"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"link failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    os.replace(tmp, LIB)
+    if verbose:
+        print(f"built {LIB}")
+    return LIB
+
+
+if __name__ == "__main__":
+    build(verbose=True)
+    sys.exit(0)

@@ -1,0 +1,305 @@
+// ---------------------------------------------------------------------------
+// Prelude of every fused skeleton kernel (compiled at run time by hiprtc for
+// gfx950).  It gives user functions written for SkelCL the OpenCL C
+// vocabulary they use -- address-space qualifiers, work-item functions,
+// barrier, mad24, the vector types with .s0/.x members, T_matrix_t with
+// get/set (MatrixDef.h:834-867), Index/IndexPoint (MapDef.h:419-423,
+// 610-615), skelcl_get_device_id() (Program.cpp:229-250) -- as plain HIP.
+// This is source compatibility for user strings, not an OpenCL runtime.
+// ---------------------------------------------------------------------------
+#undef __kernel
+#undef __global
+#undef __local
+#undef __constant
+#undef __private
+#undef __read_only
+#undef __write_only
+#define __kernel extern "C" __global__
+#define __global
+#define __local
+#define __constant
+#define __private
+#define __read_only
+#define __write_only
+#define CLK_LOCAL_MEM_FENCE 1
+#define CLK_GLOBAL_MEM_FENCE 2
+#define skelcl_get_device_id() 0
+
+typedef unsigned char uchar;
+typedef unsigned short ushort;
+typedef unsigned int uint;
+typedef unsigned long ulong;
+
+__device__ __forceinline__ size_t get_global_id(uint d)
+{
+  return d == 0 ? (size_t)blockIdx.x * blockDim.x + threadIdx.x
+       : d == 1 ? (size_t)blockIdx.y * blockDim.y + threadIdx.y
+                : (size_t)blockIdx.z * blockDim.z + threadIdx.z;
+}
+__device__ __forceinline__ size_t get_local_id(uint d)
+{
+  return d == 0 ? threadIdx.x : d == 1 ? threadIdx.y : threadIdx.z;
+}
+__device__ __forceinline__ size_t get_group_id(uint d)
+{
+  return d == 0 ? blockIdx.x : d == 1 ? blockIdx.y : blockIdx.z;
+}
+__device__ __forceinline__ size_t get_local_size(uint d)
+{
+  return d == 0 ? blockDim.x : d == 1 ? blockDim.y : blockDim.z;
+}
+__device__ __forceinline__ size_t get_num_groups(uint d)
+{
+  return d == 0 ? gridDim.x : d == 1 ? gridDim.y : gridDim.z;
+}
+__device__ __forceinline__ size_t get_global_size(uint d)
+{
+  return get_num_groups(d) * get_local_size(d);
+}
+__device__ __forceinline__ uint get_work_dim() { return 1; }
+__device__ __forceinline__ void barrier(int) { __syncthreads(); }
+__device__ __forceinline__ void mem_fence(int) { __threadfence(); }
+__device__ __forceinline__ int mad24(int a, int b, int c) { return a * b + c; }
+__device__ __forceinline__ uint mad24(uint a, uint b, uint c) { return a * b + c; }
+__device__ __forceinline__ int mul24(int a, int b) { return a * b; }
+__device__ __forceinline__ uint mul24(uint a, uint b) { return a * b; }
+__device__ __forceinline__ float mad(float a, float b, float c) { return a * b + c; }
+__device__ __forceinline__ double mad(double a, double b, double c) { return a * b + c; }
+template <typename T>
+__device__ __forceinline__ T clamp(T x, T lo, T hi)
+{
+  return x < lo ? lo : (x > hi ? hi : x);
+}
+#define native_exp expf
+#define native_log logf
+#define native_sqrt sqrtf
+#define native_recip(x) (1.0f / (x))
+#define native_divide(a, b) ((a) / (b))
+#define half_exp expf
+
+// ---- OpenCL vector types (Types.h:43-152 host side: fields s0..sf) --------
+template <typename T, int N> struct skl_vec;
+template <typename T> struct alignas(2 * sizeof(T)) skl_vec<T, 2> {
+  union { T v[2]; struct { T s0, s1; }; struct { T x, y; }; };
+};
+template <typename T> struct alignas(4 * sizeof(T)) skl_vec<T, 4> {
+  union { T v[4]; struct { T s0, s1, s2, s3; }; struct { T x, y, z, w; }; };
+};
+template <typename T> struct alignas(8 * sizeof(T)) skl_vec<T, 8> {
+  union { T v[8]; struct { T s0, s1, s2, s3, s4, s5, s6, s7; }; };
+};
+template <typename T> struct alignas(16 * sizeof(T) > 128 ? 128 : 16 * sizeof(T)) skl_vec<T, 16> {
+  union { T v[16]; struct { T s0, s1, s2, s3, s4, s5, s6, s7, s8, s9, sa, sb, sc, sd, se, sf; }; };
+};
+template <typename S> struct skl_is_scalar { static constexpr bool value = __is_arithmetic(S); };
+template <bool B, typename T = void> struct skl_enable_if {};
+template <typename T> struct skl_enable_if<true, T> { typedef T type; };
+
+#define SKL_VEC_BINOP(OP)                                                      \
+  template <typename T, int N>                                                 \
+  __device__ __forceinline__ skl_vec<T, N> operator OP(const skl_vec<T, N>& a, \
+                                                       const skl_vec<T, N>& b) \
+  {                                                                            \
+    skl_vec<T, N> r;                                                           \
+    _Pragma("unroll") for (int i = 0; i < N; ++i) r.v[i] = a.v[i] OP b.v[i];   \
+    return r;                                                                  \
+  }                                                                            \
+  template <typename T, int N, typename S>                                     \
+  __device__ __forceinline__                                                   \
+      typename skl_enable_if<skl_is_scalar<S>::value, skl_vec<T, N>>::type     \
+      operator OP(const skl_vec<T, N>& a, S b)                                 \
+  {                                                                            \
+    skl_vec<T, N> r;                                                           \
+    _Pragma("unroll") for (int i = 0; i < N; ++i) r.v[i] = a.v[i] OP(T) b;     \
+    return r;                                                                  \
+  }                                                                            \
+  template <typename T, int N, typename S>                                     \
+  __device__ __forceinline__                                                   \
+      typename skl_enable_if<skl_is_scalar<S>::value, skl_vec<T, N>>::type     \
+      operator OP(S a, const skl_vec<T, N>& b)                                 \
+  {                                                                            \
+    skl_vec<T, N> r;                                                           \
+    _Pragma("unroll") for (int i = 0; i < N; ++i) r.v[i] = (T)a OP b.v[i];     \
+    return r;                                                                  \
+  }                                                                            \
+  template <typename T, int N, typename S>                                     \
+  __device__ __forceinline__ skl_vec<T, N>& operator OP##=(skl_vec<T, N>& a,   \
+                                                           const S& b)         \
+  {                                                                            \
+    a = a OP b;                                                                \
+    return a;                                                                  \
+  }
+SKL_VEC_BINOP(+)
+SKL_VEC_BINOP(-)
+SKL_VEC_BINOP(*)
+SKL_VEC_BINOP(/)
+#undef SKL_VEC_BINOP
+template <typename T, int N>
+__device__ __forceinline__ skl_vec<T, N> operator-(const skl_vec<T, N>& a)
+{
+  skl_vec<T, N> r;
+#pragma unroll
+  for (int i = 0; i < N; ++i) r.v[i] = -a.v[i];
+  return r;
+}
+
+#define SKL_VEC_TYPES(T, NAME)                                                 \
+  typedef skl_vec<T, 2> skl_##NAME##2;                                         \
+  typedef skl_vec<T, 4> skl_##NAME##4;                                         \
+  typedef skl_vec<T, 8> skl_##NAME##8;                                         \
+  typedef skl_vec<T, 16> skl_##NAME##16;
+SKL_VEC_TYPES(char, char)
+SKL_VEC_TYPES(unsigned char, uchar)
+SKL_VEC_TYPES(short, short)
+SKL_VEC_TYPES(unsigned short, ushort)
+SKL_VEC_TYPES(int, int)
+SKL_VEC_TYPES(unsigned int, uint)
+SKL_VEC_TYPES(long, long)
+SKL_VEC_TYPES(unsigned long, ulong)
+SKL_VEC_TYPES(float, float)
+SKL_VEC_TYPES(double, double)
+#undef SKL_VEC_TYPES
+// the OpenCL names (3-component types are 4-component, Types.h:96-147)
+#define char2 skl_char2
+#define char3 skl_char4
+#define char4 skl_char4
+#define char8 skl_char8
+#define char16 skl_char16
+#define uchar2 skl_uchar2
+#define uchar3 skl_uchar4
+#define uchar4 skl_uchar4
+#define uchar8 skl_uchar8
+#define uchar16 skl_uchar16
+#define short2 skl_short2
+#define short3 skl_short4
+#define short4 skl_short4
+#define short8 skl_short8
+#define short16 skl_short16
+#define ushort2 skl_ushort2
+#define ushort3 skl_ushort4
+#define ushort4 skl_ushort4
+#define ushort8 skl_ushort8
+#define ushort16 skl_ushort16
+#define int2 skl_int2
+#define int3 skl_int4
+#define int4 skl_int4
+#define int8 skl_int8
+#define int16 skl_int16
+#define uint2 skl_uint2
+#define uint3 skl_uint4
+#define uint4 skl_uint4
+#define uint8 skl_uint8
+#define uint16 skl_uint16
+#define long2 skl_long2
+#define long3 skl_long4
+#define long4 skl_long4
+#define long8 skl_long8
+#define long16 skl_long16
+#define ulong2 skl_ulong2
+#define ulong3 skl_ulong4
+#define ulong4 skl_ulong4
+#define ulong8 skl_ulong8
+#define ulong16 skl_ulong16
+#define float2 skl_float2
+#define float3 skl_float4
+#define float4 skl_float4
+#define float8 skl_float8
+#define float16 skl_float16
+#define double2 skl_double2
+#define double3 skl_double4
+#define double4 skl_double4
+#define double8 skl_double8
+#define double16 skl_double16
+
+// ---- Matrix arguments (MatrixDef.h:834-867): T_matrix_t + get/set --------
+#define SKL_MATRIX_T(T, NAME)                                                  \
+  typedef struct {                                                             \
+    T* data;                                                                   \
+    unsigned int col_count;                                                    \
+  } NAME##_matrix_t;
+SKL_MATRIX_T(char, char)
+SKL_MATRIX_T(unsigned char, uchar)
+SKL_MATRIX_T(short, short)
+SKL_MATRIX_T(unsigned short, ushort)
+SKL_MATRIX_T(int, int)
+SKL_MATRIX_T(unsigned int, uint)
+SKL_MATRIX_T(long, long)
+SKL_MATRIX_T(unsigned long, ulong)
+SKL_MATRIX_T(float, float)
+SKL_MATRIX_T(double, double)
+SKL_MATRIX_T(skl_float2, float2)
+SKL_MATRIX_T(skl_float4, float4)
+SKL_MATRIX_T(skl_int4, int4)
+SKL_MATRIX_T(skl_double4, double4)
+#undef SKL_MATRIX_T
+#define get(m, y, x) (m).data[(int)((y) * (m).col_count + (x))]
+#define set(m, y, x, v) (m).data[(int)((y) * (m).col_count + (x))] = (v)
+
+// ---- index types ----------------------------------------------------------
+typedef size_t Index;
+typedef struct {
+  size_t x;
+  size_t y;
+} IndexPoint;
+
+// ---- skeleton-side helpers ------------------------------------------------
+// Cross-lane moves of any trivially copyable type, 32 bits at a time.
+template <typename T>
+__device__ __forceinline__ T skl_shfl_down(const T& v, unsigned d)
+{
+  constexpr int W = (int)((sizeof(T) + 3) / 4);
+  int w[W];
+  T r;
+  __builtin_memcpy(w, &v, sizeof(T));
+#pragma unroll
+  for (int i = 0; i < W; ++i) w[i] = __shfl_down(w[i], d, 64);
+  __builtin_memcpy(&r, w, sizeof(T));
+  return r;
+}
+template <typename T>
+__device__ __forceinline__ T skl_shfl_up(const T& v, unsigned d)
+{
+  constexpr int W = (int)((sizeof(T) + 3) / 4);
+  int w[W];
+  T r;
+  __builtin_memcpy(w, &v, sizeof(T));
+#pragma unroll
+  for (int i = 0; i < W; ++i) w[i] = __shfl_up(w[i], d, 64);
+  __builtin_memcpy(&r, w, sizeof(T));
+  return r;
+}
+template <typename T>
+__device__ __forceinline__ T skl_shfl(const T& v, int src)
+{
+  constexpr int W = (int)((sizeof(T) + 3) / 4);
+  int w[W];
+  T r;
+  __builtin_memcpy(w, &v, sizeof(T));
+#pragma unroll
+  for (int i = 0; i < W; ++i) w[i] = __shfl(w[i], src, 64);
+  __builtin_memcpy(&r, w, sizeof(T));
+  return r;
+}
+
+// Packs of N elements moved with one wide load/store.
+template <typename T, int N> struct alignas(N * sizeof(T)) skl_pack { T v[N]; };
+template <typename T> struct skl_pack<T, 1> { T v[1]; };
+
+// Double-float pair used by the compensated float paths.
+struct skl_df {
+  float hi, lo;
+};
+__device__ __forceinline__ skl_df skl_df_add(const skl_df& a, const skl_df& b)
+{
+  // Knuth two-sum on the high parts; low parts and the rounding error are
+  // accumulated in the low word.
+  float s = a.hi + b.hi;
+  float bb = s - a.hi;
+  float e = (a.hi - (s - bb)) + (b.hi - bb);
+  skl_df r;
+  r.hi = s;
+  r.lo = (a.lo + b.lo) + e;
+  return r;
+}
+
+extern __shared__ __attribute__((aligned(16))) char skl_dyn_lds[];

@@ -1,0 +1,188 @@
+// ---------------------------------------------------------------------------
+// Reduce skeleton kernel: one launch, HBM streaming, no identity needed.
+//
+// The reference folds with 8192 strided work-items then one work-group
+// tree (ReduceKernel.cl:51-128); like it, this kernel assumes the user
+// function is associative and commutative and never uses the identity
+// (ReduceDef.h:247-275).  Here:
+//   1. each thread folds packs p, p+G, p+2G, ... (G = grid threads), issuing
+//      SKL_UNROLL 16-byte loads before folding them;
+//   2. a wave64 shuffle tree, then an LDS step across the waves, gives the
+//      work-group partial;
+//   3. the last work-group to finish (agent-scope ticket counter, re-armed
+//      to zero by that work-group) folds all partials and writes out[0].
+// Only threads that own at least one element take part: the valid threads
+// are a prefix of the grid, so every wave knows how many lanes are valid.
+//
+// Accumulator / carry hooks (generated): SKL_ACC_T, skl_lift(x),
+// SKL_ACC_COMB(a, b); SKL_CARRY_T, skl_to_carry(acc), SKL_CARRY_COMB(a, b),
+// skl_carry_out(c).  For user functions they are the element type and the
+// user function.  For float addition (precision DEFAULT) the accumulator is
+// a compensated double-float pair and the cross-block carry is FP64.
+// For the fused Zip->Reduce form SKL_FUSED_ZIP is defined and element i is
+// SKL_ZCALL(l[i], r[i]).
+// ---------------------------------------------------------------------------
+
+template <typename A, typename F>
+__device__ __forceinline__ A skl_wave_reduce(A v, int nvalid, F comb)
+{
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) {
+    A o = skl_shfl_down(v, off);
+    if (lane + off < nvalid) v = comb(v, o);
+  }
+  return v;  // lane 0 holds the fold of lanes [0, nvalid)
+}
+
+// Fold `v` over the first `nvalid` threads of the work-group; the result is
+// valid in thread 0.  `buf` holds SKL_WG/64 entries.
+template <typename A, typename F>
+__device__ __forceinline__ A skl_block_reduce(A v, int nvalid, A* buf, F comb)
+{
+  const int lane = threadIdx.x & 63;
+  const int wv = threadIdx.x >> 6;
+  int wvalid = nvalid - wv * 64;
+  wvalid = wvalid < 0 ? 0 : (wvalid > 64 ? 64 : wvalid);
+  v = skl_wave_reduce(v, wvalid, comb);
+  if (SKL_WG <= 64) return v;
+  if (lane == 0 && wvalid > 0) buf[wv] = v;
+  __syncthreads();
+  const int nw = (nvalid + 63) / 64;
+  if (wv == 0) {
+    A x = lane < nw ? buf[lane] : v;
+    x = skl_wave_reduce(x, nw, comb);
+    v = x;
+  }
+  return v;
+}
+
+#ifdef SKL_FUSED_ZIP
+#define SKL_REDUCE_PARAMS const SKL_TL* l_, const SKL_TR* r_
+#define SKL_ELEM(i) SKL_ZCALL(l_[i], r_[i])
+#else
+#define SKL_REDUCE_PARAMS const SKL_T* in_
+#define SKL_ELEM(i) in_[i]
+#endif
+
+extern "C" __global__ __launch_bounds__(SKL_WG) void skl_reduce(
+    SKL_REDUCE_PARAMS, SKL_T* out_, unsigned long long n, SKL_CARRY_T* partials,
+    unsigned long long* ticket SKL_EXTRA_PARAMS)
+{
+  SKL_EXTRA_SETUP
+  __shared__ SKL_ACC_T s_acc[SKL_WG / 64 > 0 ? SKL_WG / 64 : 1];
+  __shared__ SKL_CARRY_T s_car[SKL_WG / 64 > 0 ? SKL_WG / 64 : 1];
+  __shared__ int s_last;
+  auto comb = [&](const SKL_ACC_T& a, const SKL_ACC_T& b) { return SKL_ACC_COMB(a, b); };
+  auto ccomb = [&](const SKL_CARRY_T& a, const SKL_CARRY_T& b) { return SKL_CARRY_COMB(a, b); };
+
+  const unsigned long long nfull = n / SKL_V;             // full packs
+  const unsigned long long npk = (n + SKL_V - 1) / SKL_V; // incl. a partial one
+  const unsigned long long G = (unsigned long long)gridDim.x * SKL_WG;
+  const unsigned long long g = (unsigned long long)blockIdx.x * SKL_WG + threadIdx.x;
+
+  SKL_ACC_T acc;
+  __builtin_memset(&acc, 0, sizeof(acc));  // never folded in: defined bits only
+  if (g < npk) {
+    // first pack: its first element seeds the accumulator
+    if (g < nfull) {
+#ifdef SKL_FUSED_ZIP
+      typedef skl_pack<SKL_TL, SKL_V> pl_t;
+      typedef skl_pack<SKL_TR, SKL_V> pr_t;
+      pl_t a = reinterpret_cast<const pl_t*>(l_)[g];
+      pr_t b = reinterpret_cast<const pr_t*>(r_)[g];
+      acc = skl_lift(SKL_ZCALL(a.v[0], b.v[0]));
+#pragma unroll
+      for (int k = 1; k < SKL_V; ++k) acc = comb(acc, skl_lift(SKL_ZCALL(a.v[k], b.v[k])));
+#else
+      typedef skl_pack<SKL_T, SKL_V> pin_t;
+      pin_t a = reinterpret_cast<const pin_t*>(in_)[g];
+      acc = skl_lift(a.v[0]);
+#pragma unroll
+      for (int k = 1; k < SKL_V; ++k) acc = comb(acc, skl_lift(a.v[k]));
+#endif
+    } else {
+      const unsigned long long e0 = g * SKL_V;
+      acc = skl_lift(SKL_ELEM(e0));
+      for (unsigned long long e = e0 + 1; e < n; ++e) acc = comb(acc, skl_lift(SKL_ELEM(e)));
+    }
+    // remaining packs, SKL_UNROLL loads in flight per thread
+    for (unsigned long long p0 = g + G; p0 < npk; p0 += G * SKL_UNROLL) {
+#ifdef SKL_FUSED_ZIP
+      typedef skl_pack<SKL_TL, SKL_V> pl_t;
+      typedef skl_pack<SKL_TR, SKL_V> pr_t;
+      pl_t a[SKL_UNROLL];
+      pr_t b[SKL_UNROLL];
+#pragma unroll
+      for (int u = 0; u < SKL_UNROLL; ++u) {
+        unsigned long long p = p0 + (unsigned long long)u * G;
+        if (p < nfull) {
+          a[u] = reinterpret_cast<const pl_t*>(l_)[p];
+          b[u] = reinterpret_cast<const pr_t*>(r_)[p];
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < SKL_UNROLL; ++u) {
+        unsigned long long p = p0 + (unsigned long long)u * G;
+        if (p < nfull) {
+#pragma unroll
+          for (int k = 0; k < SKL_V; ++k)
+            acc = comb(acc, skl_lift(SKL_ZCALL(a[u].v[k], b[u].v[k])));
+        } else if (p < npk) {
+          for (unsigned long long e = p * SKL_V; e < n; ++e)
+            acc = comb(acc, skl_lift(SKL_ELEM(e)));
+        }
+      }
+#else
+      typedef skl_pack<SKL_T, SKL_V> pin_t;
+      pin_t a[SKL_UNROLL];
+#pragma unroll
+      for (int u = 0; u < SKL_UNROLL; ++u) {
+        unsigned long long p = p0 + (unsigned long long)u * G;
+        if (p < nfull) a[u] = reinterpret_cast<const pin_t*>(in_)[p];
+      }
+#pragma unroll
+      for (int u = 0; u < SKL_UNROLL; ++u) {
+        unsigned long long p = p0 + (unsigned long long)u * G;
+        if (p < nfull) {
+#pragma unroll
+          for (int k = 0; k < SKL_V; ++k) acc = comb(acc, skl_lift(a[u].v[k]));
+        } else if (p < npk) {
+          for (unsigned long long e = p * SKL_V; e < n; ++e) acc = comb(acc, skl_lift(in_[e]));
+        }
+      }
+#endif
+    }
+  }
+  // valid threads of this work-group: a prefix of it
+  const unsigned long long first = (unsigned long long)blockIdx.x * SKL_WG;
+  const unsigned long long nvt = npk < G ? npk : G;
+  const int nvalid = first >= nvt ? 0 : (int)((nvt - first) < SKL_WG ? (nvt - first) : SKL_WG);
+  SKL_ACC_T bacc = skl_block_reduce(acc, nvalid, s_acc, comb);
+
+  if (threadIdx.x == 0) {
+    partials[blockIdx.x] = skl_to_carry(bacc);
+    unsigned long long t =
+        __hip_atomic_fetch_add(ticket, 1ull, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    s_last = (t == (unsigned long long)gridDim.x - 1) ? 1 : 0;
+  }
+  __syncthreads();
+  if (!s_last) return;
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  // last work-group: fold the gridDim.x partials in the carry type
+  SKL_CARRY_T c;
+  const int np = (int)gridDim.x;
+  __builtin_memset(&c, 0, sizeof(c));
+  if ((int)threadIdx.x < np) {
+    c = partials[threadIdx.x];
+    for (int i = threadIdx.x + SKL_WG; i < np; i += SKL_WG) c = ccomb(c, partials[i]);
+  }
+  const int cvalid = np < SKL_WG ? np : SKL_WG;
+  SKL_CARRY_T total = skl_block_reduce(c, cvalid, s_car, ccomb);
+  if (threadIdx.x == 0) {
+    out_[0] = skl_carry_out(total);
+    // every work-group has counted itself: re-arm the counter for the next
+    // launch (also across graph replays)
+    __hip_atomic_store(ticket, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+}

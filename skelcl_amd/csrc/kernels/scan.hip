@@ -1,0 +1,234 @@
+// ---------------------------------------------------------------------------
+// Scan skeleton kernel: single-pass prefix scan with decoupled look-back.
+//
+// The reference runs ceil(log_wg n) Blelloch passes plus uniform-add passes
+// (ScanDef.h:108-259, ScanKernel.cl:74-250), moving ~4N elements through
+// HBM.  This kernel reads N and writes N once:
+//   * a work-group takes the next tile (ticket counter, so it only ever
+//     waits on tiles that were handed out earlier and are running);
+//   * tile = SKL_WG threads x SKL_J packs x SKL_V elements, loaded with
+//     coalesced 16-byte accesses: pack j of lane l sits at wave_base +
+//     (j*64 + l)*SKL_V; every lane scans its packs, then one wave64 scan per
+//     pack row and an LDS step across waves give the tile-local prefixes;
+//   * wave 0 publishes the tile aggregate, looks back over windows of 64
+//     predecessor tiles (one status word per 32-bit chunk of the carry,
+//     tagged with the launch epoch, read and written with agent-scope
+//     relaxed atomics) and publishes the inclusive prefix;
+//   * outputs = carry (+) local prefix, exclusive (the reference's form,
+//     ScanKernel.cl:142-153) or inclusive.
+// The user function must be associative; SKL_IDENTITY is its identity.
+//
+// Hooks (generated): SKL_ACC_T, skl_lift, SKL_ACC_COMB, SKL_ACC_ID,
+// SKL_CARRY_T, skl_to_carry, SKL_CARRY_COMB, SKL_CARRY_ID, SKL_PREFIX_T,
+// skl_make_prefix(carry, accprefix), SKL_EMIT(prefix, acc),
+// skl_emit0(prefix).  For float addition (precision DEFAULT) the tile-local
+// values are compensated double-float pairs and the carry is FP64.
+// ---------------------------------------------------------------------------
+
+#define SKL_SCAN_AGG 1u
+#define SKL_SCAN_INCL 2u
+#define SKL_CARRY_WORDS ((int)((sizeof(SKL_CARRY_T) + 3) / 4))
+
+__device__ __forceinline__ void skl_scan_publish(unsigned long long* lb,
+                                                 unsigned long long tile,
+                                                 unsigned state,
+                                                 const SKL_CARRY_T& v,
+                                                 unsigned epoch)
+{
+  unsigned w[SKL_CARRY_WORDS];
+  w[SKL_CARRY_WORDS - 1] = 0;
+  __builtin_memcpy(w, &v, sizeof(SKL_CARRY_T));
+  const unsigned long long tag = (unsigned long long)((epoch << 2) | state) << 32;
+#pragma unroll
+  for (int i = 0; i < SKL_CARRY_WORDS; ++i)
+    __hip_atomic_store(&lb[tile * SKL_CARRY_WORDS + i], tag | (unsigned long long)w[i],
+                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Returns the state (AGG/INCL) once every word of the tile carries the same
+// tag of this launch; 0 while the tile has not published.
+__device__ __forceinline__ unsigned skl_scan_read(unsigned long long* lb,
+                                                  unsigned long long tile,
+                                                  unsigned epoch, SKL_CARRY_T* v)
+{
+  unsigned w[SKL_CARRY_WORDS];
+  unsigned tag0 = 0;
+#pragma unroll
+  for (int i = 0; i < SKL_CARRY_WORDS; ++i) {
+    unsigned long long x = __hip_atomic_load(&lb[tile * SKL_CARRY_WORDS + i],
+                                             __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    unsigned tag = (unsigned)(x >> 32);
+    if (i == 0) tag0 = tag;
+    else if (tag != tag0) return 0;
+    w[i] = (unsigned)x;
+  }
+  if ((tag0 >> 2) != epoch) return 0;
+  __builtin_memcpy(v, w, sizeof(SKL_CARRY_T));
+  return tag0 & 3u;
+}
+
+extern "C" __global__ __launch_bounds__(SKL_WG) void skl_scan(
+    const SKL_T* in_, SKL_T* out_, unsigned long long n, int inclusive,
+    unsigned long long* lb, unsigned long long* ctl SKL_EXTRA_PARAMS)
+{
+  SKL_EXTRA_SETUP
+  constexpr int W = SKL_WG / 64;
+  constexpr unsigned long long WAVE_ELEMS = 64ull * SKL_J * SKL_V;
+  constexpr unsigned long long TILE = (unsigned long long)SKL_WG * SKL_J * SKL_V;
+  typedef skl_pack<SKL_T, SKL_V> pack_t;
+  __shared__ unsigned long long s_tile;
+  __shared__ unsigned s_epoch;
+  __shared__ SKL_ACC_T s_wagg[W];
+  __shared__ SKL_CARRY_T s_carry;
+  auto comb = [&](const SKL_ACC_T& a, const SKL_ACC_T& b) { return SKL_ACC_COMB(a, b); };
+  auto ccomb = [&](const SKL_CARRY_T& a, const SKL_CARRY_T& b) { return SKL_CARRY_COMB(a, b); };
+  const int lane = threadIdx.x & 63;
+  const int wv = threadIdx.x >> 6;
+
+  // ctl[0]: tile ticket, ctl[1]: launch epoch (tags the look-back words, so
+  // words left by earlier launches never match).  The work-group drawing the
+  // last ticket re-arms both for the next launch; every other work-group has
+  // already read the epoch by then (the read precedes its acq_rel ticket).
+  if (threadIdx.x == 0) {
+    unsigned ep = (unsigned)__hip_atomic_load(&ctl[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    unsigned long long t =
+        __hip_atomic_fetch_add(&ctl[0], 1ull, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    if (t == (unsigned long long)gridDim.x - 1) {
+      __hip_atomic_store(&ctl[0], 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_store(&ctl[1], (unsigned long long)((ep + 1u) & 0x3FFFFFFFu),
+                         __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    s_tile = t;
+    s_epoch = ep;
+  }
+  __syncthreads();
+  const unsigned long long tile = s_tile;
+  const unsigned epoch = s_epoch;
+  const unsigned long long tbase = tile * TILE;
+  const unsigned long long wbase = tbase + (unsigned long long)wv * WAVE_ELEMS;
+  const bool full = tbase + TILE <= n;
+
+  // ---- load + per-lane inclusive scan of each pack
+  SKL_ACC_T q[SKL_J][SKL_V];
+  if (full) {
+    pack_t pk[SKL_J];
+#pragma unroll
+    for (int j = 0; j < SKL_J; ++j)
+      pk[j] = reinterpret_cast<const pack_t*>(in_)[(wbase + ((unsigned long long)j * 64 + lane) * SKL_V) / SKL_V];
+#pragma unroll
+    for (int j = 0; j < SKL_J; ++j)
+#pragma unroll
+      for (int k = 0; k < SKL_V; ++k) q[j][k] = skl_lift(pk[j].v[k]);
+  } else {
+#pragma unroll
+    for (int j = 0; j < SKL_J; ++j) {
+      const unsigned long long e0 = wbase + ((unsigned long long)j * 64 + lane) * SKL_V;
+#pragma unroll
+      for (int k = 0; k < SKL_V; ++k) q[j][k] = (e0 + k < n) ? skl_lift(in_[e0 + k]) : SKL_ACC_ID;
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < SKL_J; ++j)
+#pragma unroll
+    for (int k = 1; k < SKL_V; ++k) q[j][k] = comb(q[j][k - 1], q[j][k]);
+
+  // ---- one wave scan per pack row
+  SKL_ACC_T lp[SKL_J];
+  SKL_ACC_T rowpre = SKL_ACC_ID;
+#pragma unroll
+  for (int j = 0; j < SKL_J; ++j) {
+    SKL_ACC_T inc = q[j][SKL_V - 1];
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+      SKL_ACC_T o = skl_shfl_up(inc, off);
+      if (lane >= off) inc = comb(o, inc);
+    }
+    SKL_ACC_T exc = skl_shfl_up(inc, 1);
+    if (lane == 0) exc = SKL_ACC_ID;
+    SKL_ACC_T rt = skl_shfl(inc, 63);
+    lp[j] = comb(rowpre, exc);
+    rowpre = comb(rowpre, rt);
+  }
+  if (lane == 0) s_wagg[wv] = rowpre;
+  __syncthreads();
+  SKL_ACC_T wpre = SKL_ACC_ID;
+  SKL_ACC_T bagg = SKL_ACC_ID;
+#pragma unroll
+  for (int w = 0; w < W; ++w) {
+    SKL_ACC_T a = s_wagg[w];
+    if (w < wv) wpre = comb(wpre, a);
+    bagg = comb(bagg, a);
+  }
+
+  // ---- decoupled look-back (wave 0)
+  if (wv == 0) {
+    const SKL_CARRY_T cagg = skl_to_carry(bagg);
+    if (tile == 0) {
+      if (lane == 0) {
+        skl_scan_publish(lb, tile, SKL_SCAN_INCL, cagg, epoch);
+        s_carry = SKL_CARRY_ID;
+      }
+    } else {
+      if (lane == 0) skl_scan_publish(lb, tile, SKL_SCAN_AGG, cagg, epoch);
+      SKL_CARRY_T excl = SKL_CARRY_ID;
+      bool have = false;
+      long long end = (long long)tile;
+      while (true) {
+        const long long idx = end - 64 + lane;
+        unsigned st;
+        SKL_CARRY_T v;
+        if (idx < 0) {
+          st = SKL_SCAN_INCL;
+          v = SKL_CARRY_ID;
+        } else {
+          do {
+            st = skl_scan_read(lb, (unsigned long long)idx, epoch, &v);
+          } while (st == 0);
+        }
+        const unsigned long long im = __ballot(st == SKL_SCAN_INCL);
+        const int h = im ? 63 - __builtin_clzll(im) : -1;
+        // ordered fold of lanes [h, 63]: lane i ends with v_i (+) ... (+) v_63
+        SKL_CARRY_T r = v;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+          SKL_CARRY_T o = skl_shfl_down(r, off);
+          if (lane + off < 64) r = ccomb(r, o);
+        }
+        const SKL_CARRY_T ws = skl_shfl(r, h < 0 ? 0 : h);
+        excl = have ? ccomb(ws, excl) : ws;
+        have = true;
+        if (h >= 0) break;
+        end -= 64;
+      }
+      if (lane == 0) {
+        skl_scan_publish(lb, tile, SKL_SCAN_INCL, ccomb(excl, cagg), epoch);
+        s_carry = excl;
+      }
+    }
+  }
+  __syncthreads();
+  const SKL_CARRY_T carry = s_carry;
+
+  // ---- outputs
+#pragma unroll
+  for (int j = 0; j < SKL_J; ++j) {
+    const SKL_PREFIX_T P = skl_make_prefix(carry, comb(wpre, lp[j]));
+    pack_t o;
+    if (inclusive) {
+#pragma unroll
+      for (int k = 0; k < SKL_V; ++k) o.v[k] = SKL_EMIT(P, q[j][k]);
+    } else {
+      o.v[0] = skl_emit0(P);
+#pragma unroll
+      for (int k = 1; k < SKL_V; ++k) o.v[k] = SKL_EMIT(P, q[j][k - 1]);
+    }
+    const unsigned long long e0 = wbase + ((unsigned long long)j * 64 + lane) * SKL_V;
+    if (full) {
+      reinterpret_cast<pack_t*>(out_)[e0 / SKL_V] = o;
+    } else {
+#pragma unroll
+      for (int k = 0; k < SKL_V; ++k)
+        if (e0 + k < n) out_[e0 + k] = o.v[k];
+    }
+  }
+}

@@ -1,0 +1,52 @@
+// Internal state of a skeleton object (behind the opaque skl_skeleton).
+#pragma once
+
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "fusion.h"
+#include "skelcl_skeletons.h"
+
+struct TypeInfo {
+  std::string name;
+  size_t size = 0;
+};
+
+struct skl_skeleton_s {
+  int kind = 0;
+  std::string src, func, src2, func2, identity, neutral;
+  std::string deriv_src, deriv_func, deriv_func2, options;
+  std::vector<TypeInfo> types;  // kind-specific order, see skelcl_skeletons.h
+  int overlap = 1, padding = SKL_PADDING_NEAREST;
+  int wg = 256;
+  int precision = SKL_PREC_DEFAULT;
+
+  skl::UserFunction uf, uf2;          // main (Reduce for AllPairs) / Zip
+  skl::UserFunction duf, duf2;        // derivative functions
+  bool has_deriv = false;
+  std::vector<skl::Param> extras;     // extra parameters, in call order
+  bool simple = false;                // needs one element per work-item
+  skl::UserOp op = skl::UserOp::Other, op2 = skl::UserOp::Other;
+
+  // generated programs: key -> (source, program)
+  struct Prog {
+    std::string source;
+    skl_program program = nullptr;
+  };
+  std::map<std::string, Prog> progs;
+
+  // launch shape (autotuned or defaults)
+  int V = 1;        // elements per 16-byte pack
+  int unroll = 4;   // packs in flight per thread (Map/Zip/Reduce)
+  int scan_j = 4;   // packs per lane (Scan tile = 256*J*V)
+  int mo_rb = 8;    // MapOverlap rows per thread
+  int max_blocks = 0;
+
+  // scratch: [0,8) ticket, [8,12) epoch, partials / look-back words at 256
+  void* scratch = nullptr;
+  size_t scratch_bytes = 0;
+  // the float (+) forms with compensated / FP64 combine
+  bool precise_add() const;
+};
