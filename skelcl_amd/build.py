@@ -85,6 +85,25 @@ def build(verbose: bool = False) -> str:
     return LIB
 
 
+
+
+def build_cpp_tests(verbose: bool = False) -> str:
+    """Compile tests/cpp (the reference's skeleton tests restated against the
+    C++ API) with g++ against libskelcl_amd.so; host code only."""
+    src = os.path.join(ROOT, "tests", "cpp", "skeleton_tests.cpp")
+    out = os.path.join(ROOT, "tests", "cpp", "skelcl_cpp_tests")
+    cmd = ["g++", "-std=c++17", "-O1", "-Wall", "-I", os.path.join(ROOT, "include"),
+           "-I", os.path.join(HERE, "include"), "-o", out, src, "-L", HERE, "-lskelcl_amd",
+           "-Wl,-rpath,$ORIGIN/../../skelcl_amd"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"C++ tests failed to compile:\n{r.stdout}\n{r.stderr}")
+    if verbose:
+        print(f"built {out}")
+    return out
+
+
 if __name__ == "__main__":
     build(verbose=True)
+    build_cpp_tests(verbose=True)
     sys.exit(0)

@@ -1,57 +1,239 @@
-// Extensions without a reference counterpart that are not yet available on
-// this build report SKL_ERR_UNSUPPORTED.
+// Extensions of the skeleton layer that have no reference counterpart:
+// precision forms of the dense AllPairs contraction, adjoints, batched
+// segmented Reduce/Scan and batched small products.
+#include <hip/hip_runtime.h>
+
+#include <mutex>
+
 #include "common.h"
+#include "skeleton.h"
 #include "skelcl_skeletons.h"
 
 using namespace skl;
 
-extern "C" int skl_reduce_segmented_run(skl_skeleton, void*, const void*, const int64_t*, int,
-                                        skl_stream)
+extern "C" size_t skl_internal_matmul_workspace(size_t M, size_t N, size_t K, int precision);
+extern "C" int skl_internal_matmul(float* C, const float* A, const float* B, int M, int N,
+                                   int K, int ta, int tb, int precision, void* ws,
+                                   hipStream_t s);
+extern "C" int skl_internal_stencil_transpose(float* din, const float* dout, const float* w,
+                                              int rows, int cols, int R, int nearest,
+                                              hipStream_t s);
+extern "C" int skl_internal_segmented_sum(float* out, const float* in, const int64_t* offsets,
+                                          int nseg, void* ws, hipStream_t s);
+extern "C" int skl_internal_segmented_scan(float* out, const float* in,
+                                           const int64_t* offsets, int nseg, int inclusive,
+                                           void* ws, size_t ws_bytes, hipStream_t s);
+extern "C" size_t skl_internal_segmented_scan_workspace(int nseg);
+extern "C" int skl_internal_broadcast(void* dst, const void* src, size_t elem, size_t count,
+                                      hipStream_t s);
+extern "C" size_t skl_internal_batched_workspace(int count);
+extern "C" int skl_internal_batched_gemm_ws(const skl_gemm_desc* descs, int count, int precision,
+                                            void* ws, hipStream_t s);
+
+// skeleton helpers implemented in skeletons.cpp
+int skl_scan_run_reverse(skl_skeleton s, void* out, const void* in, size_t n, int inclusive,
+                         skl_stream stream);
+int skl_run_backward_map(skl_skeleton s, void* d_in, const void* in, const void* d_out,
+                         size_t n, const skl_arg* args, int nargs, skl_stream stream);
+int skl_run_backward_zip(skl_skeleton s, void* d_left, void* d_right, const void* left,
+                         const void* right, const void* d_out, size_t n, const skl_arg* args,
+                         int nargs, skl_stream stream);
+
+namespace {
+// Workspace of the MFMA forms: grows, never shrinks; calls on different
+// streams at the same time race on it.
+std::mutex g_ws_mutex;
+void* g_ws = nullptr;
+size_t g_ws_bytes = 0;
+
+int workspace(size_t bytes, void** out)
 {
-  return fail(SKL_ERR_UNSUPPORTED, "segmented Reduce is not available");
+  std::lock_guard<std::mutex> lk(g_ws_mutex);
+  if (g_ws_bytes < bytes) {
+    if (g_ws) {
+      SKL_HIP_TRY(hipDeviceSynchronize());
+      SKL_HIP_TRY(hipFree(g_ws));
+      g_ws = nullptr;
+      g_ws_bytes = 0;
+    }
+    SKL_HIP_TRY(hipMalloc(&g_ws, bytes));
+    g_ws_bytes = bytes;
+  }
+  *out = g_ws;
+  return SKL_OK;
 }
-extern "C" int skl_scan_segmented_run(skl_skeleton, void*, const void*, const int64_t*, int, int,
-                                      skl_stream)
+
+// Per-skeleton scratch of the segmented forms, zeroed when allocated (the
+// look-back words are tagged with an epoch kept in it).
+int seg_scratch(skl_skeleton_s* s, size_t bytes)
 {
-  return fail(SKL_ERR_UNSUPPORTED, "segmented Scan is not available");
+  if (s->seg_scratch_bytes >= bytes) return SKL_OK;
+  if (s->seg_scratch) {
+    SKL_HIP_TRY(hipDeviceSynchronize());
+    SKL_HIP_TRY(hipFree(s->seg_scratch));
+    s->seg_scratch = nullptr;
+    s->seg_scratch_bytes = 0;
+  }
+  SKL_HIP_TRY(hipMalloc(&s->seg_scratch, bytes));
+  SKL_HIP_TRY(hipMemset(s->seg_scratch, 0, bytes));
+  SKL_HIP_TRY(hipDeviceSynchronize());
+  s->seg_scratch_bytes = bytes;
+  return SKL_OK;
 }
-extern "C" int skl_map_backward(skl_skeleton, void*, const void*, const void*, size_t,
-                                const skl_arg*, int, skl_stream)
+
+bool is_float_addmul(const skl_skeleton_s* s)
 {
-  return fail(SKL_ERR_UNSUPPORTED, "Map backward is not available");
+  return s->kind == SKL_ALLPAIRS && s->op == UserOp::Add && s->op2 == UserOp::Mul &&
+         s->extras.empty() && s->types[0].name == "float" && s->types[1].name == "float" &&
+         s->types[2].name == "float";
 }
-extern "C" int skl_zip_backward(skl_skeleton, void*, void*, const void*, const void*,
-                                const void*, size_t, const skl_arg*, int, skl_stream)
+}  // namespace
+
+extern "C" int skl_matmul_f32(float* C, const float* A, const float* B, size_t M, size_t N,
+                              size_t K, int trans_a, int trans_b, int precision,
+                              skl_stream stream)
 {
-  return fail(SKL_ERR_UNSUPPORTED, "Zip backward is not available");
+  SKL_TRY(ensure_initialized());
+  if (M == 0 || N == 0) return SKL_OK;
+  if (!C || !A || !B || K == 0) return fail(SKL_ERR_INVALID, "matmul: NULL buffer or K == 0");
+  if (M > 0x7FFFFF80 || N > 0x7FFFFF80 || K > 0x7FFFFF80)
+    return fail(SKL_ERR_INVALID, "matmul: dimension too large");
+  if (precision < SKL_PREC_DEFAULT || precision > SKL_PREC_FP8)
+    return fail(SKL_ERR_INVALID, "matmul: unknown precision %d", precision);
+  if (precision == SKL_PREC_DEFAULT) precision = SKL_PREC_FP32;
+  void* ws = nullptr;
+  size_t wsb = skl_internal_matmul_workspace(M, N, K, precision);
+  if (wsb) SKL_TRY(workspace(wsb, &ws));
+  int st = skl_internal_matmul(C, A, B, (int)M, (int)N, (int)K, trans_a ? 1 : 0,
+                               trans_b ? 1 : 0, precision, ws, resolve_stream(stream));
+  if (st != SKL_OK) return fail(st, "matmul launch failed: %s", hipGetErrorString(hipGetLastError()));
+  return SKL_OK;
 }
-extern "C" int skl_reduce_backward(skl_skeleton, void*, const void*, size_t, skl_stream)
+
+extern "C" int skl_allpairs_backward(skl_skeleton s, void* d_left, void* d_right,
+                                     const void* left, const void* right, const void* d_out,
+                                     size_t rows, size_t cols, size_t dim, skl_stream stream)
 {
-  return fail(SKL_ERR_UNSUPPORTED, "Reduce backward is not available");
+  SKL_TRY(ensure_initialized());
+  if (!s) return fail(SKL_ERR_INVALID, "skeleton is NULL");
+  if (!is_float_addmul(s))
+    return fail(SKL_ERR_UNSUPPORTED,
+                "AllPairs backward needs the (+,*) contraction on float (a matmul)");
+  if (!d_out) return fail(SKL_ERR_INVALID, "d_out is NULL");
+  int prec = s->precision == SKL_PREC_DEFAULT ? SKL_PREC_FP32 : s->precision;
+  // C = A.B with A rows x dim, B dim x cols
+  if (d_left)  // dA = dC . B^T : (rows x cols) . (cols x dim)
+    SKL_TRY(skl_matmul_f32((float*)d_left, (const float*)d_out, (const float*)right, rows, dim,
+                           cols, 0, 1, prec, stream));
+  if (d_right)  // dB = A^T . dC : (dim x rows) . (rows x cols)
+    SKL_TRY(skl_matmul_f32((float*)d_right, (const float*)left, (const float*)d_out, dim, cols,
+                           rows, 1, 0, prec, stream));
+  return SKL_OK;
 }
-extern "C" int skl_scan_backward(skl_skeleton, void*, const void*, size_t, int, skl_stream)
+
+extern "C" int skl_reduce_backward(skl_skeleton s, void* d_in, const void* d_out, size_t n,
+                                   skl_stream stream)
 {
-  return fail(SKL_ERR_UNSUPPORTED, "Scan backward is not available");
+  SKL_TRY(ensure_initialized());
+  if (!s || s->kind != SKL_REDUCE) return fail(SKL_ERR_INVALID, "not a Reduce skeleton");
+  if (s->op != UserOp::Add || !s->extras.empty())
+    return fail(SKL_ERR_UNSUPPORTED, "Reduce backward is defined for Reduce(+)");
+  if (n == 0) return SKL_OK;
+  if (!d_in || !d_out) return fail(SKL_ERR_INVALID, "NULL buffer");
+  // d in_i = d out (broadcast): copy the scalar into place 0 then fill by a
+  // kernel that reads it on the device (no host round trip)
+  return skl_internal_broadcast(d_in, d_out, s->types[0].size, n, resolve_stream(stream));
 }
-extern "C" int skl_mapoverlap_stencil_backward(skl_skeleton, void*, const void*, const float*,
-                                               size_t, size_t, skl_stream)
+
+extern "C" int skl_scan_backward(skl_skeleton s, void* d_in, const void* d_out, size_t n,
+                                 int inclusive, skl_stream stream)
 {
-  return fail(SKL_ERR_UNSUPPORTED, "MapOverlap backward is not available");
+  SKL_TRY(ensure_initialized());
+  if (!s || s->kind != SKL_SCAN) return fail(SKL_ERR_INVALID, "not a Scan skeleton");
+  if (s->op != UserOp::Add || !s->extras.empty())
+    return fail(SKL_ERR_UNSUPPORTED, "Scan backward is defined for Scan(+)");
+  // y = scan(x): dx_i = sum_{j>i} dy_j (exclusive) / sum_{j>=i} dy_j
+  // (inclusive): the scan of dy read from the end
+  return skl_scan_run_reverse(s, d_in, d_out, n, inclusive, stream);
 }
-extern "C" int skl_allpairs_backward(skl_skeleton, void*, void*, const void*, const void*,
-                                     const void*, size_t, size_t, size_t, skl_stream)
+
+extern "C" int skl_map_backward(skl_skeleton s, void* d_in, const void* in, const void* d_out,
+                                size_t n, const skl_arg* args, int nargs, skl_stream stream)
 {
-  return fail(SKL_ERR_UNSUPPORTED, "AllPairs backward is not available");
+  return skl_run_backward_map(s, d_in, in, d_out, n, args, nargs, stream);
 }
-extern "C" int skl_matmul_f32(float*, const float*, const float*, size_t, size_t, size_t, int,
-                              int, int, skl_stream)
+
+extern "C" int skl_zip_backward(skl_skeleton s, void* d_left, void* d_right, const void* left,
+                                const void* right, const void* d_out, size_t n,
+                                const skl_arg* args, int nargs, skl_stream stream)
 {
-  return fail(SKL_ERR_UNSUPPORTED, "matmul precision forms are not available");
+  return skl_run_backward_zip(s, d_left, d_right, left, right, d_out, n, args, nargs, stream);
 }
-extern "C" int skl_matmul_batched_f32(const skl_gemm_desc*, int, int, skl_stream)
+
+extern "C" int skl_mapoverlap_stencil_backward(skl_skeleton s, void* d_in, const void* d_out,
+                                               const float* weights, size_t rows, size_t cols,
+                                               skl_stream stream)
 {
-  return fail(SKL_ERR_UNSUPPORTED, "batched matmul is not available");
+  SKL_TRY(ensure_initialized());
+  if (!s || s->kind != SKL_MAPOVERLAP) return fail(SKL_ERR_INVALID, "not a MapOverlap skeleton");
+  if (s->types[0].name != "float" || s->types[1].name != "float")
+    return fail(SKL_ERR_UNSUPPORTED, "stencil backward is defined for float MapOverlap");
+  if (s->padding == SKL_PADDING_NEUTRAL && s->neutral != "0" && s->neutral != "0.0f" &&
+      s->neutral != "0.0" && s->neutral != "0.f")
+    return fail(SKL_ERR_UNSUPPORTED, "stencil backward needs a zero neutral value");
+  if (rows == 0 || cols == 0) return SKL_OK;
+  if (!d_in || !d_out || !weights) return fail(SKL_ERR_INVALID, "NULL buffer");
+  if (rows > 0x7FFFFFFF || cols > 0x7FFFFFFF) return fail(SKL_ERR_INVALID, "matrix too large");
+  return skl_internal_stencil_transpose((float*)d_in, (const float*)d_out, weights, (int)rows,
+                                        (int)cols, s->overlap,
+                                        s->padding == SKL_PADDING_NEAREST ? 1 : 0,
+                                        resolve_stream(stream));
 }
+
+extern "C" int skl_reduce_segmented_run(skl_skeleton s, void* out, const void* in,
+                                        const int64_t* offsets, int nseg, skl_stream stream)
+{
+  SKL_TRY(ensure_initialized());
+  if (!s || s->kind != SKL_REDUCE) return fail(SKL_ERR_INVALID, "not a Reduce skeleton");
+  if (!(s->op == UserOp::Add && s->extras.empty() && s->types[0].name == "float"))
+    return fail(SKL_ERR_UNSUPPORTED, "segmented Reduce is available for float Reduce(+)");
+  if (nseg <= 0) return SKL_OK;
+  if (!out || !in || !offsets) return fail(SKL_ERR_INVALID, "NULL buffer");
+  SKL_TRY(seg_scratch(s, skl_internal_segmented_scan_workspace(nseg)));
+  return skl_internal_segmented_sum((float*)out, (const float*)in, offsets, nseg,
+                                    s->seg_scratch, resolve_stream(stream));
+}
+
+extern "C" int skl_scan_segmented_run(skl_skeleton s, void* out, const void* in,
+                                      const int64_t* offsets, int nseg, int inclusive,
+                                      skl_stream stream)
+{
+  SKL_TRY(ensure_initialized());
+  if (!s || s->kind != SKL_SCAN) return fail(SKL_ERR_INVALID, "not a Scan skeleton");
+  if (!(s->op == UserOp::Add && s->extras.empty() && s->types[0].name == "float"))
+    return fail(SKL_ERR_UNSUPPORTED, "segmented Scan is available for float Scan(+)");
+  if (nseg <= 0) return SKL_OK;
+  if (!out || !in || !offsets) return fail(SKL_ERR_INVALID, "NULL buffer");
+  size_t wsb = skl_internal_segmented_scan_workspace(nseg);
+  SKL_TRY(seg_scratch(s, wsb));
+  return skl_internal_segmented_scan((float*)out, (const float*)in, offsets, nseg, inclusive,
+                                     s->seg_scratch, wsb, resolve_stream(stream));
+}
+
+extern "C" int skl_matmul_batched_f32(const skl_gemm_desc* descs_device, int count,
+                                      int precision, skl_stream stream)
+{
+  SKL_TRY(ensure_initialized());
+  if (count <= 0) return SKL_OK;
+  if (!descs_device) return fail(SKL_ERR_INVALID, "descs is NULL");
+  if (precision != SKL_PREC_DEFAULT && precision != SKL_PREC_FP32 &&
+      precision != SKL_PREC_BF16X3)
+    return fail(SKL_ERR_UNSUPPORTED, "batched matmul supports FP32 and BF16X3");
+  void* ws = nullptr;
+  SKL_TRY(workspace(skl_internal_batched_workspace(count), &ws));
+  return skl_internal_batched_gemm_ws(descs_device, count, precision, ws, resolve_stream(stream));
+}
+
 extern "C" int skl_autotune(skl_skeleton, size_t, int)
 {
   return fail(SKL_ERR_UNSUPPORTED, "autotuning is not available");

@@ -106,7 +106,15 @@ extern "C" __global__ __launch_bounds__(SKL_WG) void skl_scan(
   const unsigned epoch = s_epoch;
   const unsigned long long tbase = tile * TILE;
   const unsigned long long wbase = tbase + (unsigned long long)wv * WAVE_ELEMS;
+#ifdef SKL_SCAN_REVERSE
+  // scan read from the end (the adjoint of a scan): element e of the scan
+  // is element n-1-e of the arrays; element-wise accesses only
+  const bool full = false;
+#define SKL_RIDX(e) (n - 1 - (e))
+#else
   const bool full = tbase + TILE <= n;
+#define SKL_RIDX(e) (e)
+#endif
 
   // ---- load + per-lane inclusive scan of each pack
   SKL_ACC_T q[SKL_J][SKL_V];
@@ -124,7 +132,8 @@ extern "C" __global__ __launch_bounds__(SKL_WG) void skl_scan(
     for (int j = 0; j < SKL_J; ++j) {
       const unsigned long long e0 = wbase + ((unsigned long long)j * 64 + lane) * SKL_V;
 #pragma unroll
-      for (int k = 0; k < SKL_V; ++k) q[j][k] = (e0 + k < n) ? skl_lift(in_[e0 + k]) : SKL_ACC_ID;
+      for (int k = 0; k < SKL_V; ++k)
+        q[j][k] = (e0 + k < n) ? skl_lift(in_[SKL_RIDX(e0 + k)]) : SKL_ACC_ID;
     }
   }
 #pragma unroll
@@ -228,7 +237,7 @@ extern "C" __global__ __launch_bounds__(SKL_WG) void skl_scan(
     } else {
 #pragma unroll
       for (int k = 0; k < SKL_V; ++k)
-        if (e0 + k < n) out_[e0 + k] = o.v[k];
+        if (e0 + k < n) out_[SKL_RIDX(e0 + k)] = o.v[k];
     }
   }
 }

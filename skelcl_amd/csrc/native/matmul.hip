@@ -1,0 +1,727 @@
+// Dense C = A.B on float (the AllPairs(Reduce(+), Zip(*)) contraction,
+// AllPairsKernel.cl:47-135) in three precision forms, built with hipcc for
+// gfx950:
+//
+//  FP32    VALU, LDS-tiled: 256 threads, 128x128 block tile, 8x8 outputs per
+//          thread, k-slices of 8; fmaf accumulation in k order per output.
+//  BF16X3  MFMA v_mfma_f32_16x16x32_bf16.  Each operand is split into three
+//          bf16 terms x = x0 + x1 + x2 (x1 = bf16(x - x0), x2 = bf16(x - x0 -
+//          x1)), stored K-contiguous (B transposed) and zero-padded to the
+//          tile grid by a split kernel; the GEMM kernel sums the six products
+//          with i+j <= 2 (x0y0 into one FP32 accumulator, the five correction
+//          products into a second one) and adds the two at the end.
+//  FP8     MFMA v_mfma_scale_f32_16x16x128_f8f6f4 on OCP e4m3 operands, each
+//          128x128 tile of A and of B^T quantised with its own scale
+//          absmax/448; every 128-deep k-block product is rescaled by
+//          sA*sB into the FP32 accumulator.
+//
+// Block tile 128x128, 256 threads = 4 waves as 2x2, each wave 64x64 = 4x4
+// MFMA tiles of 16x16.  blockIdx.x walks N, blockIdx.y walks M.
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "skelcl_skeletons.h"
+
+namespace {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef int i32x8 __attribute__((ext_vector_type(8)));
+typedef int i32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int BM = 128, BN = 128;
+
+// ---------------------------------------------------------------------------
+// bf16 round-to-nearest-even of a float (finite inputs; NaN stays NaN)
+__device__ __forceinline__ uint16_t bf16_rne(float x)
+{
+  uint32_t u = __float_as_uint(x);
+  if ((u & 0x7F800000u) == 0x7F800000u) return (uint16_t)((u >> 16) | ((u & 0xFFFF) ? 0x40 : 0));
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+__device__ __forceinline__ float bf16_to_f32(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+
+// Split into three bf16 terms, each the rounding of what is left.
+__device__ __forceinline__ void split3(float x, uint16_t& h0, uint16_t& h1, uint16_t& h2)
+{
+  h0 = bf16_rne(x);
+  float r1 = x - bf16_to_f32(h0);
+  h1 = bf16_rne(r1);
+  float r2 = r1 - bf16_to_f32(h1);
+  h2 = bf16_rne(r2);
+}
+
+// X is R x C row-major (ld = C) or, when trans, C x R row-major read as its
+// transpose.  Writes 3 planes of Rp x Cp bf16 (row-major, zero padded):
+// plane p at out + p * Rp * Cp.
+__global__ __launch_bounds__(256) void split3_kernel(const float* __restrict__ X, int R, int Cc,
+                                                     int trans, uint16_t* __restrict__ out,
+                                                     int Rp, int Cp)
+{
+  const size_t plane = (size_t)Rp * Cp;
+  const size_t total = plane;
+  for (size_t e = (size_t)blockIdx.x * 256 + threadIdx.x; e < total;
+       e += (size_t)gridDim.x * 256) {
+    const int r = (int)(e / Cp), c = (int)(e - (size_t)r * Cp);
+    float x = 0.0f;
+    if (r < R && c < Cc) x = trans ? X[(size_t)c * R + r] : X[(size_t)r * Cc + c];
+    uint16_t h0, h1, h2;
+    split3(x, h0, h1, h2);
+    out[e] = h0;
+    out[plane + e] = h1;
+    out[2 * plane + e] = h2;
+  }
+}
+
+// Transposing variant for a row-major K x N matrix B into N x K planes with
+// coalesced reads and writes through a 32x32 LDS tile.
+__global__ __launch_bounds__(256) void split3_transpose_kernel(const float* __restrict__ X,
+                                                               int R, int Cc,
+                                                               uint16_t* __restrict__ out,
+                                                               int Rp, int Cp)
+{
+  // X is Cc x R row-major (ld = R); the planes are R x Cc (Rp x Cp), i.e.
+  // out[r][c] = X[c][r].
+  __shared__ float t[32][33];
+  const size_t plane = (size_t)Rp * Cp;
+  const int r0 = blockIdx.x * 32, c0 = blockIdx.y * 32;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
+  for (int k = ty; k < 32; k += 8) {
+    const int c = c0 + k, r = r0 + tx;
+    t[k][tx] = (c < Cc && r < R) ? X[(size_t)c * R + r] : 0.0f;
+  }
+  __syncthreads();
+  for (int k = ty; k < 32; k += 8) {
+    const int r = r0 + k, c = c0 + tx;
+    if (r < Rp && c < Cp) {
+      uint16_t h0, h1, h2;
+      split3(t[tx][k], h0, h1, h2);
+      const size_t e = (size_t)r * Cp + c;
+      out[e] = h0;
+      out[plane + e] = h1;
+      out[2 * plane + e] = h2;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// bf16x3 GEMM.  Ap: 3 planes Mp x Kp, Bp: 3 planes Np x Kp (B^T), Kp % 32 == 0,
+// Mp % 128 == 0, Np % 128 == 0.
+constexpr int BK = 32;
+constexpr int LDS_STRIDE = BK + 8;  // bf16 per LDS row (padding vs bank conflicts)
+
+__global__ __launch_bounds__(256, 1) void gemm_bf16x3_kernel(
+    const uint16_t* __restrict__ Ap, const uint16_t* __restrict__ Bp, float* __restrict__ C,
+    int M, int N, int Mp, int Np, int Kp, int ldc)
+{
+  __shared__ __attribute__((aligned(16))) uint16_t As[3][BM][LDS_STRIDE];
+  __shared__ __attribute__((aligned(16))) uint16_t Bs[3][BN][LDS_STRIDE];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave >> 1, wn = wave & 1;
+  const int m0 = blockIdx.y * BM, n0 = blockIdx.x * BN;
+  const size_t pa = (size_t)Mp * Kp, pb = (size_t)Np * Kp;
+
+  f32x4 acc0[4][4], acc1[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      acc0[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+      acc1[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+    }
+
+  // global -> register staging: 3 planes x 128 rows x 4 chunks of 8 bf16
+  // = 1536 chunks per operand, 6 per thread
+  i32x4 ra[6], rb[6];
+  auto gload = [&](int k0) {
+#pragma unroll
+    for (int j = 0; j < 6; ++j) {
+      const int c = tid + 256 * j;
+      const int p = c >> 9, r = (c >> 2) & 127, kc = c & 3;
+      ra[j] = *reinterpret_cast<const i32x4*>(Ap + p * pa + (size_t)(m0 + r) * Kp + k0 + kc * 8);
+      rb[j] = *reinterpret_cast<const i32x4*>(Bp + p * pb + (size_t)(n0 + r) * Kp + k0 + kc * 8);
+    }
+  };
+  auto lstore = [&]() {
+#pragma unroll
+    for (int j = 0; j < 6; ++j) {
+      const int c = tid + 256 * j;
+      const int p = c >> 9, r = (c >> 2) & 127, kc = c & 3;
+      *reinterpret_cast<i32x4*>(&As[p][r][kc * 8]) = ra[j];
+      *reinterpret_cast<i32x4*>(&Bs[p][r][kc * 8]) = rb[j];
+    }
+  };
+
+  gload(0);
+  for (int k0 = 0; k0 < Kp; k0 += BK) {
+    lstore();
+    __syncthreads();
+    if (k0 + BK < Kp) gload(k0 + BK);  // next slice in flight during the MFMAs
+    const int fr = lane & 15, fk = (lane >> 4) * 8;
+    bf16x8 a[4][3], b[4][3];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int p = 0; p < 3; ++p) {
+        a[i][p] = *reinterpret_cast<const bf16x8*>(&As[p][wm * 64 + i * 16 + fr][fk]);
+        b[i][p] = *reinterpret_cast<const bf16x8*>(&Bs[p][wn * 64 + i * 16 + fr][fk]);
+      }
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        acc0[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i][0], b[j][0], acc0[i][j], 0, 0, 0);
+        acc1[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i][1], b[j][1], acc1[i][j], 0, 0, 0);
+        acc1[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i][0], b[j][2], acc1[i][j], 0, 0, 0);
+        acc1[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i][2], b[j][0], acc1[i][j], 0, 0, 0);
+        acc1[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i][0], b[j][1], acc1[i][j], 0, 0, 0);
+        acc1[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i][1], b[j][0], acc1[i][j], 0, 0, 0);
+      }
+    __syncthreads();
+  }
+  // C fragment of a 16x16 tile: lane holds column lane%16, rows 4*(lane/16)+0..3
+  const int cc = lane & 15, cr = (lane >> 4) * 4;
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int col = n0 + wn * 64 + j * 16 + cc;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = m0 + wm * 64 + i * 16 + cr + r;
+        if (row < M && col < N) C[(size_t)row * ldc + col] = acc0[i][j][r] + acc1[i][j][r];
+      }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// FP8 (OCP e4m3) with one scale per 128x128 tile.
+// Quantise X (R x C view, optional transpose as split3) into Rp x Cp bytes,
+// Rp % 128 == 0, Cp % 128 == 0; scales[(r/128) * (Cp/128) + c/128].
+// One work-group per 128x128 tile.
+__device__ __forceinline__ uint8_t fp8_e4m3(float x)
+{
+  // round-to-nearest-even into OCP e4m3 (bias 7, max 448, no inf)
+  const uint32_t u = __float_as_uint(x);
+  const uint32_t sign = (u >> 24) & 0x80u;
+  float a = fabsf(x);
+  if (!(a == a)) return (uint8_t)(sign | 0x7F);  // NaN
+  if (a >= 464.0f) return (uint8_t)(sign | 0x7E);  // saturate to 448
+  if (a < 0x1p-10f) return (uint8_t)sign;          // below half the smallest subnormal
+  int e;
+  frexpf(a, &e);                 // a = m * 2^e, m in [0.5, 1)
+  int ue = e - 1;                // a in [2^ue, 2^(ue+1))
+  if (ue < -6) ue = -6;          // subnormal range: spacing 2^-9
+  const float q = ldexpf(1.0f, ue - 3);  // spacing of this binade
+  float m = rintf(a / q);        // round half to even
+  float v = m * q;
+  if (v >= 464.0f) return (uint8_t)(sign | 0x7E);
+  // encode v
+  if (v == 0.0f) return (uint8_t)sign;
+  int ve;
+  float vm = frexpf(v, &ve);     // v = vm * 2^ve, vm in [0.5,1)
+  int ex = ve - 1 + 7;           // biased exponent
+  uint32_t mant;
+  if (ex <= 0) {                 // subnormal: v = mant * 2^-9
+    mant = (uint32_t)rintf(v * 512.0f);
+    return (uint8_t)(sign | mant);
+  }
+  mant = (uint32_t)rintf((vm * 2.0f - 1.0f) * 8.0f);
+  if (mant == 8) {
+    mant = 0;
+    ++ex;
+  }
+  if (ex > 15 || (ex == 15 && mant == 7)) return (uint8_t)(sign | 0x7E);
+  return (uint8_t)(sign | (ex << 3) | mant);
+}
+
+__global__ __launch_bounds__(256) void quant_fp8_kernel(const float* __restrict__ X, int R,
+                                                        int Cc, int trans,
+                                                        uint8_t* __restrict__ out,
+                                                        float* __restrict__ scales, int Rp,
+                                                        int Cp)
+{
+  const int tr = blockIdx.y, tc = blockIdx.x;
+  const int r0 = tr * 128, c0 = tc * 128;
+  __shared__ float red[256];
+  float amax = 0.0f;
+  for (int e = threadIdx.x; e < 128 * 128; e += 256) {
+    const int r = r0 + (e >> 7), c = c0 + (e & 127);
+    float x = 0.0f;
+    if (r < R && c < Cc) x = trans ? X[(size_t)c * R + r] : X[(size_t)r * Cc + c];
+    amax = fmaxf(amax, fabsf(x));
+  }
+  red[threadIdx.x] = amax;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) red[threadIdx.x] = fmaxf(red[threadIdx.x], red[threadIdx.x + s]);
+    __syncthreads();
+  }
+  const float scale = red[0] > 0.0f ? red[0] / 448.0f : 1.0f;
+  const float inv = 1.0f / scale;
+  if (threadIdx.x == 0) scales[tr * (Cp / 128) + tc] = scale;
+  for (int e = threadIdx.x; e < 128 * 128; e += 256) {
+    const int r = r0 + (e >> 7), c = c0 + (e & 127);
+    float x = 0.0f;
+    if (r < R && c < Cc) x = trans ? X[(size_t)c * R + r] : X[(size_t)r * Cc + c];
+    out[(size_t)r * Cp + c] = fp8_e4m3(x * inv);
+  }
+}
+
+// Aq: Mp x Kp bytes, Bq: Np x Kp bytes (B^T); sa: (Mp/128) x (Kp/128),
+// sb: (Np/128) x (Kp/128).
+__global__ __launch_bounds__(256, 1) void gemm_fp8_kernel(
+    const uint8_t* __restrict__ Aq, const uint8_t* __restrict__ Bq,
+    const float* __restrict__ sa, const float* __restrict__ sb, float* __restrict__ C, int M,
+    int N, int Kp, int ldc)
+{
+  constexpr int KB = 128;
+  constexpr int STR = KB + 16;  // bytes per LDS row
+  __shared__ __attribute__((aligned(16))) uint8_t As[BM][STR];
+  __shared__ __attribute__((aligned(16))) uint8_t Bs[BN][STR];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave >> 1, wn = wave & 1;
+  const int m0 = blockIdx.y * BM, n0 = blockIdx.x * BN;
+  const int nkb = Kp / KB;
+  f32x4 acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+  // 128 rows x 128 bytes = 1024 chunks of 16 B per operand, 4 per thread
+  i32x4 ra[4], rb[4];
+  auto gload = [&](int kb) {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int c = tid + 256 * j;
+      const int r = c >> 3, kc = c & 7;
+      ra[j] = *reinterpret_cast<const i32x4*>(Aq + (size_t)(m0 + r) * Kp + kb * KB + kc * 16);
+      rb[j] = *reinterpret_cast<const i32x4*>(Bq + (size_t)(n0 + r) * Kp + kb * KB + kc * 16);
+    }
+  };
+  gload(0);
+  for (int kb = 0; kb < nkb; ++kb) {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int c = tid + 256 * j;
+      const int r = c >> 3, kc = c & 7;
+      *reinterpret_cast<i32x4*>(&As[r][kc * 16]) = ra[j];
+      *reinterpret_cast<i32x4*>(&Bs[r][kc * 16]) = rb[j];
+    }
+    __syncthreads();
+    if (kb + 1 < nkb) gload(kb + 1);
+    const float s = sa[(m0 / 128) * nkb + kb] * sb[(n0 / 128) * nkb + kb];
+    // operand fragment: lane holds row lane%16, k bytes 32*(lane/16)..+31
+    const int fr = lane & 15, fk = (lane >> 4) * 32;
+    i32x8 a[4], b[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      a[i] = *reinterpret_cast<const i32x8*>(&As[wm * 64 + i * 16 + fr][fk]);
+      b[i] = *reinterpret_cast<const i32x8*>(&Bs[wn * 64 + i * 16 + fr][fk]);
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        f32x4 t = {0.f, 0.f, 0.f, 0.f};
+        // format 0 = e4m3 for both operands; scales 127 = 2^0 (unscaled)
+        t = __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(a[i], b[j], t, 0, 0, 0, 127, 0,
+                                                             127);
+        acc[i][j] += t * s;
+      }
+    __syncthreads();
+  }
+  const int cc = lane & 15, cr = (lane >> 4) * 4;
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int col = n0 + wn * 64 + j * 16 + cc;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = m0 + wm * 64 + i * 16 + cr + r;
+        if (row < M && col < N) C[(size_t)row * ldc + col] = acc[i][j][r];
+      }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// FP32 VALU GEMM: 128x128 block tile, 8x8 per thread (rows ty*4+{0..3} and
+// 64+ty*4+{0..3}, same split for columns), k-slices of 8.
+constexpr int FK = 8;
+__global__ __launch_bounds__(256) void gemm_f32_kernel(const float* __restrict__ A,
+                                                       const float* __restrict__ B,
+                                                       float* __restrict__ C, int M, int N,
+                                                       int K, int ta, int tb, int ldc)
+{
+  __shared__ float As[FK][BM + 4];
+  __shared__ float Bs[FK][BN + 4];
+  const int tid = threadIdx.x;
+  const int tx = tid & 15, ty = tid >> 4;
+  const int m0 = blockIdx.y * BM, n0 = blockIdx.x * BN;
+  float acc[8][8];
+#pragma unroll
+  for (int i = 0; i < 8; ++i)
+#pragma unroll
+    for (int j = 0; j < 8; ++j) acc[i][j] = 0.0f;
+  for (int k0 = 0; k0 < K; k0 += FK) {
+    // A slice: 128 rows x 8 k (4 per thread)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int e = tid + 256 * j;
+      int r, kk;
+      if (ta) {  // A stored K x M: read along m
+        kk = e >> 7;
+        r = e & 127;
+      } else {
+        r = e >> 3;
+        kk = e & 7;
+      }
+      const int gr = m0 + r, gk = k0 + kk;
+      float v = 0.0f;
+      if (gr < M && gk < K) v = ta ? A[(size_t)gk * M + gr] : A[(size_t)gr * K + gk];
+      As[kk][r] = v;
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int e = tid + 256 * j;
+      int c, kk;
+      if (tb) {  // B stored N x K: read along k
+        c = e >> 3;
+        kk = e & 7;
+      } else {
+        kk = e >> 7;
+        c = e & 127;
+      }
+      const int gc = n0 + c, gk = k0 + kk;
+      float v = 0.0f;
+      if (gc < N && gk < K) v = tb ? B[(size_t)gc * K + gk] : B[(size_t)gk * N + gc];
+      Bs[kk][c] = v;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < FK; ++kk) {
+      float a[8], b[8];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        a[i] = As[kk][ty * 4 + i];
+        a[4 + i] = As[kk][64 + ty * 4 + i];
+        b[i] = Bs[kk][tx * 4 + i];
+        b[4 + i] = Bs[kk][64 + tx * 4 + i];
+      }
+#pragma unroll
+      for (int i = 0; i < 8; ++i)
+#pragma unroll
+        for (int j = 0; j < 8; ++j) acc[i][j] = fmaf(a[i], b[j], acc[i][j]);
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int i = 0; i < 8; ++i) {
+    const int row = m0 + (i < 4 ? ty * 4 + i : 64 + ty * 4 + i - 4);
+    if (row >= M) continue;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const int col = n0 + (j < 4 ? tx * 4 + j : 64 + tx * 4 + j - 4);
+      if (col < N) C[(size_t)row * ldc + col] = acc[i][j];
+    }
+  }
+}
+
+inline size_t round_up(size_t x, size_t m) { return (x + m - 1) / m * m; }
+
+int st(hipError_t e) { return e == hipSuccess ? SKL_OK : SKL_ERR_HIP; }
+
+}  // namespace
+
+// Workspace sizes (bytes) of the MFMA forms for an M x N x K product.
+extern "C" size_t skl_internal_matmul_workspace(size_t M, size_t N, size_t K, int precision)
+{
+  const size_t Mp = round_up(M, 128), Np = round_up(N, 128);
+  if (precision == SKL_PREC_BF16X3) {
+    const size_t Kp = round_up(K, 32);
+    return 3 * (Mp + Np) * Kp * 2 + 256;
+  }
+  if (precision == SKL_PREC_FP8) {
+    const size_t Kp = round_up(K, 128);
+    return (Mp + Np) * Kp + ((Mp + Np) / 128) * (Kp / 128) * 4 + 512;
+  }
+  return 0;
+}
+
+extern "C" int skl_internal_matmul(float* C, const float* A, const float* B, int M, int N,
+                                   int K, int ta, int tb, int precision, void* ws,
+                                   hipStream_t s)
+{
+  const int Mp = (int)round_up(M, 128), Np = (int)round_up(N, 128);
+  if (precision == SKL_PREC_BF16X3) {
+    const int Kp = (int)round_up(K, 32);
+    uint16_t* Ap = static_cast<uint16_t*>(ws);
+    uint16_t* Bp = Ap + 3 * (size_t)Mp * Kp;
+    // A planes: Mp x Kp from A (M x K, or K x M when ta)
+    {
+      const size_t tot = (size_t)Mp * Kp;
+      unsigned g = (unsigned)((tot + 255) / 256);
+      if (g > 65536) g = 65536;
+      if (!ta) {
+        split3_kernel<<<g, 256, 0, s>>>(A, M, K, 0, Ap, Mp, Kp);
+      } else {
+        dim3 grid((Mp + 31) / 32, (Kp + 31) / 32);
+        split3_transpose_kernel<<<grid, 256, 0, s>>>(A, M, K, Ap, Mp, Kp);
+      }
+    }
+    // B^T planes: Np x Kp from B (K x N, or N x K when tb)
+    {
+      if (tb) {
+        const size_t tot = (size_t)Np * Kp;
+        unsigned g = (unsigned)((tot + 255) / 256);
+        if (g > 65536) g = 65536;
+        split3_kernel<<<g, 256, 0, s>>>(B, N, K, 0, Bp, Np, Kp);
+      } else {
+        dim3 grid((Np + 31) / 32, (Kp + 31) / 32);
+        split3_transpose_kernel<<<grid, 256, 0, s>>>(B, N, K, Bp, Np, Kp);
+      }
+    }
+    dim3 grid(Np / BN, Mp / BM);
+    gemm_bf16x3_kernel<<<grid, 256, 0, s>>>(Ap, Bp, C, M, N, Mp, Np, Kp, N);
+    return st(hipGetLastError());
+  }
+  if (precision == SKL_PREC_FP8) {
+    const int Kp = (int)round_up(K, 128);
+    uint8_t* Aq = static_cast<uint8_t*>(ws);
+    uint8_t* Bq = Aq + (size_t)Mp * Kp;
+    float* sa = reinterpret_cast<float*>(Bq + (size_t)Np * Kp);
+    float* sb = sa + (size_t)(Mp / 128) * (Kp / 128);
+    quant_fp8_kernel<<<dim3(Kp / 128, Mp / 128), 256, 0, s>>>(A, M, K, ta, Aq, sa, Mp, Kp);
+    // B^T: rows n, columns k; element (n, k) = B[k][n] (or B[n][k] when tb)
+    quant_fp8_kernel<<<dim3(Kp / 128, Np / 128), 256, 0, s>>>(B, N, K, tb ? 0 : 1, Bq, sb, Np,
+                                                               Kp);
+    dim3 grid(Np / BN, Mp / BM);
+    gemm_fp8_kernel<<<grid, 256, 0, s>>>(Aq, Bq, sa, sb, C, M, N, Kp, N);
+    return st(hipGetLastError());
+  }
+  dim3 grid((N + BN - 1) / BN, (M + BM - 1) / BM);
+  gemm_f32_kernel<<<grid, 256, 0, s>>>(A, B, C, M, N, K, ta, tb, N);
+  return st(hipGetLastError());
+}
+
+// ---------------------------------------------------------------------------
+// Batched products (many small AllPairs(+,*) on float) in one persistent
+// launch after a one-work-group prologue.  Work item = one 64x64 tile of one
+// product; persistent work-groups take items from a counter until none is
+// left.  FP32: VALU 4x4 per thread.  BF16X3: operands are split into three
+// bf16 planes while they are staged in LDS (no pre-pass), 4 waves of 32x32,
+// six MFMA products per 16x16x32 step as in gemm_bf16x3_kernel.
+namespace {
+
+struct BatchCtl {
+  unsigned long long work;
+  unsigned long long total;
+};
+
+__global__ __launch_bounds__(1024) void batch_prologue(const skl_gemm_desc* __restrict__ d,
+                                                       int count, long long* __restrict__ tstart,
+                                                       BatchCtl* ctl)
+{
+  __shared__ long long part[1024];
+  __shared__ long long carry;
+  if (threadIdx.x == 0) carry = 0;
+  __syncthreads();
+  for (int base = 0; base < count; base += 1024) {
+    const int p = base + threadIdx.x;
+    long long t = 0;
+    if (p < count && d[p].M > 0 && d[p].N > 0)
+      t = (long long)((d[p].M + 63) / 64) * ((d[p].N + 63) / 64);
+    part[threadIdx.x] = t;
+    __syncthreads();
+    for (int s = 1; s < 1024; s <<= 1) {
+      long long v = threadIdx.x >= (unsigned)s ? part[threadIdx.x - s] : 0;
+      __syncthreads();
+      part[threadIdx.x] += v;
+      __syncthreads();
+    }
+    if (p < count) tstart[p] = carry + part[threadIdx.x] - t;
+    __syncthreads();
+    if (threadIdx.x == 1023) carry += part[1023];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    tstart[count] = carry;
+    ctl->work = 0;
+    ctl->total = (unsigned long long)carry;
+  }
+}
+
+__device__ __forceinline__ int find_item(const long long* ts, int count, long long t)
+{
+  int lo = 0, hi = count;
+  while (hi - lo > 1) {
+    int mid = (lo + hi) >> 1;
+    if (ts[mid] <= t) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
+template <bool BF16X3>
+__global__ __launch_bounds__(256) void batch_gemm_kernel(const skl_gemm_desc* __restrict__ d,
+                                                         int count,
+                                                         const long long* __restrict__ tstart,
+                                                         BatchCtl* ctl)
+{
+  constexpr int T = 64, KS = 32;
+  // FP32: As[KS][T], Bs[KS][T]; BF16X3: 3 planes x 64 rows x (32+8) bf16
+  __shared__ __attribute__((aligned(16))) float Af[KS][T + 4];
+  __shared__ __attribute__((aligned(16))) float Bf[KS][T + 4];
+  __shared__ __attribute__((aligned(16))) uint16_t Ah[3][T][KS + 8];
+  __shared__ __attribute__((aligned(16))) uint16_t Bh[3][T][KS + 8];
+  __shared__ long long s_t;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const unsigned long long total = ctl->total;
+  while (true) {
+    if (tid == 0)
+      s_t = (long long)__hip_atomic_fetch_add(&ctl->work, 1ull, __ATOMIC_RELAXED,
+                                              __HIP_MEMORY_SCOPE_AGENT);
+    __syncthreads();
+    const long long t = s_t;
+    __syncthreads();
+    if (t >= (long long)total) break;
+    const int p = find_item(tstart, count, t);
+    const skl_gemm_desc g = d[p];
+    const long long lt = t - tstart[p];
+    const int ntn = (g.N + T - 1) / T;
+    const int m0 = (int)(lt / ntn) * T, n0 = (int)(lt % ntn) * T;
+    if (!BF16X3) {
+      const int tx = tid & 15, ty = tid >> 4;
+      float acc[4][4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = 0.0f;
+      for (int k0 = 0; k0 < g.K; k0 += KS) {
+        for (int e = tid; e < T * KS; e += 256) {
+          const int r = e / KS, kk = e % KS;
+          const int gr = m0 + r, gk = k0 + kk;
+          Af[kk][r] = (gr < g.M && gk < g.K) ? g.A[(size_t)gr * g.K + gk] : 0.0f;
+          const int kb = e / T, c = e % T;
+          const int gkb = k0 + kb, gc = n0 + c;
+          Bf[kb][c] = (gkb < g.K && gc < g.N) ? g.B[(size_t)gkb * g.N + gc] : 0.0f;
+        }
+        __syncthreads();
+#pragma unroll 8
+        for (int kk = 0; kk < KS; ++kk) {
+          float a[4], b[4];
+#pragma unroll
+          for (int i = 0; i < 4; ++i) {
+            a[i] = Af[kk][ty * 4 + i];
+            b[i] = Bf[kk][tx * 4 + i];
+          }
+#pragma unroll
+          for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) acc[i][j] = fmaf(a[i], b[j], acc[i][j]);
+        }
+        __syncthreads();
+      }
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int r = m0 + ty * 4 + i;
+        if (r >= g.M) continue;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const int c = n0 + tx * 4 + j;
+          if (c < g.N) g.C[(size_t)r * g.N + c] = acc[i][j];
+        }
+      }
+    } else {
+      const int wm = wave >> 1, wn = wave & 1;  // each wave 32x32 = 2x2 tiles
+      f32x4 acc0[2][2], acc1[2][2];
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+          acc0[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+          acc1[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+        }
+      for (int k0 = 0; k0 < g.K; k0 += KS) {
+        for (int e = tid; e < T * KS; e += 256) {
+          const int r = e / KS, kk = e % KS;  // A row r, k kk (reads along k)
+          const int gr = m0 + r, gk = k0 + kk;
+          const float a = (gr < g.M && gk < g.K) ? g.A[(size_t)gr * g.K + gk] : 0.0f;
+          uint16_t h0, h1, h2;
+          split3(a, h0, h1, h2);
+          Ah[0][r][kk] = h0;
+          Ah[1][r][kk] = h1;
+          Ah[2][r][kk] = h2;
+          const int kb = e / T, c = e % T;    // B row kb, column c (reads along n)
+          const int gkb = k0 + kb, gc = n0 + c;
+          const float b = (gkb < g.K && gc < g.N) ? g.B[(size_t)gkb * g.N + gc] : 0.0f;
+          split3(b, h0, h1, h2);
+          Bh[0][c][kb] = h0;
+          Bh[1][c][kb] = h1;
+          Bh[2][c][kb] = h2;
+        }
+        __syncthreads();
+        const int fr = lane & 15, fk = (lane >> 4) * 8;
+        bf16x8 a[2][3], b[2][3];
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+          for (int q = 0; q < 3; ++q) {
+            a[i][q] = *reinterpret_cast<const bf16x8*>(&Ah[q][wm * 32 + i * 16 + fr][fk]);
+            b[i][q] = *reinterpret_cast<const bf16x8*>(&Bh[q][wn * 32 + i * 16 + fr][fk]);
+          }
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+          for (int j = 0; j < 2; ++j) {
+            acc0[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i][0], b[j][0], acc0[i][j], 0, 0, 0);
+            acc1[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i][1], b[j][1], acc1[i][j], 0, 0, 0);
+            acc1[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i][0], b[j][2], acc1[i][j], 0, 0, 0);
+            acc1[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i][2], b[j][0], acc1[i][j], 0, 0, 0);
+            acc1[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i][0], b[j][1], acc1[i][j], 0, 0, 0);
+            acc1[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i][1], b[j][0], acc1[i][j], 0, 0, 0);
+          }
+        __syncthreads();
+      }
+      const int cc = lane & 15, cr = (lane >> 4) * 4;
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+          const int col = n0 + wn * 32 + j * 16 + cc;
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const int row = m0 + wm * 32 + i * 16 + cr + r;
+            if (row < g.M && col < g.N) g.C[(size_t)row * g.N + col] = acc0[i][j][r] + acc1[i][j][r];
+          }
+        }
+    }
+  }
+}
+
+}  // namespace
+
+extern "C" size_t skl_internal_batched_workspace(int count)
+{
+  return 256 + (size_t)(count + 1) * 8;
+}
+
+extern "C" int skl_internal_batched_gemm_ws(const skl_gemm_desc* descs, int count, int precision,
+                                            void* ws, hipStream_t s)
+{
+  BatchCtl* ctl = static_cast<BatchCtl*>(ws);
+  long long* tstart = reinterpret_cast<long long*>(static_cast<char*>(ws) + 256);
+  int dev = 0, cus = 256;
+  hipDeviceProp_t prop;
+  if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess)
+    cus = prop.multiProcessorCount;
+  batch_prologue<<<1, 1024, 0, s>>>(descs, count, tstart, ctl);
+  if (precision == SKL_PREC_BF16X3)
+    batch_gemm_kernel<true><<<cus * 2, 256, 0, s>>>(descs, count, tstart, ctl);
+  else
+    batch_gemm_kernel<false><<<cus * 2, 256, 0, s>>>(descs, count, tstart, ctl);
+  return st(hipGetLastError());
+}

@@ -47,6 +47,9 @@ struct skl_skeleton_s {
   // scratch: [0,8) ticket, [8,12) epoch, partials / look-back words at 256
   void* scratch = nullptr;
   size_t scratch_bytes = 0;
+  // scratch of the batched segmented forms (zeroed when allocated)
+  void* seg_scratch = nullptr;
+  size_t seg_scratch_bytes = 0;
   // the float (+) forms with compensated / FP64 combine
   bool precise_add() const;
 };

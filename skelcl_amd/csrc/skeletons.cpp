@@ -339,6 +339,7 @@ static int generate(const skl_skeleton_s* s, const std::string& key, std::string
       }
       g.plumbing(make_extra_plumbing(s->extras, ""));
       emit_reduce_scan_hooks(o, s);
+      if (key == "reverse") o << "#define SKL_SCAN_REVERSE 1\n";
       o << (s->kind == SKL_REDUCE ? skl_tmpl_reduce : skl_tmpl_scan);
       break;
     }
@@ -608,10 +609,9 @@ extern "C" int skl_skeleton_release(skl_skeleton s)
 {
   if (!s) return SKL_OK;
   for (auto& kv : s->progs) skl_program_release(kv.second.program);
-  if (s->scratch) {
-    hipDeviceSynchronize();
-    hipFree(s->scratch);
-  }
+  if (s->scratch || s->seg_scratch) hipDeviceSynchronize();
+  if (s->scratch) hipFree(s->scratch);
+  if (s->seg_scratch) hipFree(s->seg_scratch);
   delete s;
   return SKL_OK;
 }
@@ -899,10 +899,64 @@ extern "C" int skl_zip_reduce_run(skl_skeleton zip, skl_skeleton reduce, void* o
   return reduce_launch(reduce, key, out, {left, right}, n, pk, stream, zip);
 }
 
+static int scan_launch(skl_skeleton s, const std::string& key, void* out, const void* in,
+                       size_t n, int inclusive, const skl_arg* args, int nargs,
+                       skl_stream stream);
+
 extern "C" int skl_scan_run(skl_skeleton s, void* out, const void* in, size_t n, int inclusive,
                             const skl_arg* args, int nargs, skl_stream stream)
 {
   SKL_TRY(check_kind(s, SKL_SCAN));
+  return scan_launch(s, "main", out, in, n, inclusive, args, nargs, stream);
+}
+
+// The scan of the array read from its end (adjoint of Scan(+)).
+int skl_scan_run_reverse(skl_skeleton s, void* out, const void* in, size_t n, int inclusive,
+                         skl_stream stream)
+{
+  SKL_TRY(check_kind(s, SKL_SCAN));
+  return scan_launch(s, "reverse", out, in, n, inclusive, nullptr, 0, stream);
+}
+
+int skl_run_backward_map(skl_skeleton s, void* d_in, const void* in, const void* d_out,
+                         size_t n, const skl_arg* args, int nargs, skl_stream stream)
+{
+  SKL_TRY(check_kind(s, SKL_MAP));
+  if (!s->has_deriv)
+    return fail(SKL_ERR_INVALID, "Map backward needs a derivative function (deriv_source)");
+  ArgPack pk;
+  SKL_TRY(marshal_extras(s->extras, args, nargs, &pk));
+  if (n == 0) return SKL_OK;
+  if (!d_in || !in || !d_out) return fail(SKL_ERR_INVALID, "NULL buffer");
+  unsigned long long nn = n;
+  skl_kernel k;
+  SKL_TRY(get_kernel(s, "backward", "skl_map_backward", &k));
+  auto a = with_extras({&in, &d_out, &d_in, &nn}, pk);
+  return launch(k, blocks_for(n, 256ull * 4, s->max_blocks), 1, 256, 1, pk.dyn_lds, a, stream);
+}
+
+int skl_run_backward_zip(skl_skeleton s, void* d_left, void* d_right, const void* left,
+                         const void* right, const void* d_out, size_t n, const skl_arg* args,
+                         int nargs, skl_stream stream)
+{
+  SKL_TRY(check_kind(s, SKL_ZIP));
+  if (!s->has_deriv)
+    return fail(SKL_ERR_INVALID, "Zip backward needs derivative functions (deriv_source)");
+  ArgPack pk;
+  SKL_TRY(marshal_extras(s->extras, args, nargs, &pk));
+  if (n == 0) return SKL_OK;
+  if (!left || !right || !d_out) return fail(SKL_ERR_INVALID, "NULL buffer");
+  unsigned long long nn = n;
+  skl_kernel k;
+  SKL_TRY(get_kernel(s, "backward", "skl_zip_backward", &k));
+  auto a = with_extras({&left, &right, &d_out, &d_left, &d_right, &nn}, pk);
+  return launch(k, blocks_for(n, 256ull * 4, s->max_blocks), 1, 256, 1, pk.dyn_lds, a, stream);
+}
+
+static int scan_launch(skl_skeleton s, const std::string& key, void* out, const void* in,
+                       size_t n, int inclusive, const skl_arg* args, int nargs,
+                       skl_stream stream)
+{
   ArgPack pk;
   SKL_TRY(marshal_extras(s->extras, args, nargs, &pk));
   if (n == 0) return SKL_OK;
@@ -921,7 +975,7 @@ extern "C" int skl_scan_run(skl_skeleton s, void* out, const void* in, size_t n,
   unsigned long long nn = n;
   int incl = inclusive ? 1 : 0;
   skl_kernel k;
-  SKL_TRY(get_kernel(s, "main", "skl_scan", &k));
+  SKL_TRY(get_kernel(s, key, "skl_scan", &k));
   auto a = with_extras({&in, &out, &nn, &incl, &lb, &ctl}, pk);
   return launch(k, (unsigned)tiles, 1, 256, 1, 0, a, stream);
 }

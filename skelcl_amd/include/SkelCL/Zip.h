@@ -1,0 +1,120 @@
+// skelcl::Zip (Zip.h; detail/ZipDef.h:70-442): Zip<Tout(Tl, Tr)> and
+// Zip<void(Tl, Tr)>.  Inputs must have equal sizes (ZipDef.h:101,126).
+#pragma once
+
+#include <string>
+
+#include "detail/Skeleton.h"
+
+namespace skelcl {
+
+template <typename> class Zip;
+
+template <typename Tout, typename Tl, typename Tr>
+class Zip<Tout(Tl, Tr)> : public detail::SkeletonBase {
+public:
+  Zip(const Source& source, const std::string& funcName = "func") : _src(source), _func(funcName)
+  {
+    skl_skeleton_desc d = blankDesc(SKL_ZIP);
+    d.source = _src.c_str();
+    d.func = _func.c_str();
+    d.types[0] = typeOf<Tout>();
+    d.types[1] = typeOf<Tl>();
+    d.types[2] = typeOf<Tr>();
+    d.ntypes = 3;
+    create(d);
+  }
+
+  template <typename... Args>
+  Vector<Tout> operator()(const Vector<Tl>& l, const Vector<Tr>& r, Args&&... args)
+  {
+    Vector<Tout> output;
+    (*this)(out(output), l, r, std::forward<Args>(args)...);
+    return output;
+  }
+  template <typename... Args>
+  Vector<Tout>& operator()(Out<Vector<Tout>> output, const Vector<Tl>& l, const Vector<Tr>& r,
+                           Args&&... args)
+  {
+    if (l.size() != r.size()) detail::abortWithError("Zip: inputs differ in size");
+    auto L = detail::makeArgs(std::forward<Args>(args)...);
+    const void* lp = l.devicePointer();
+    const void* rp = r.devicePointer();
+    Vector<Tout>& o = output.container();
+    const void* op = &o;
+    void* dst = (op == static_cast<const void*>(&l) || op == static_cast<const void*>(&r) ||
+                 o.size() > l.size())
+                    ? o.devicePointer()
+                    : (o.prepareOutput(l.size()), o.deviceOutputPointer());
+    detail::check(skl_zip_run(_h, dst, lp, rp, l.size(), L.data(), L.count(), nullptr), "Zip");
+    o.dataOnDeviceModified();
+    L.finish();
+    return o;
+  }
+  template <typename... Args>
+  Matrix<Tout> operator()(const Matrix<Tl>& l, const Matrix<Tr>& r, Args&&... args)
+  {
+    Matrix<Tout> output;
+    (*this)(out(output), l, r, std::forward<Args>(args)...);
+    return output;
+  }
+  template <typename... Args>
+  Matrix<Tout>& operator()(Out<Matrix<Tout>> output, const Matrix<Tl>& l, const Matrix<Tr>& r,
+                           Args&&... args)
+  {
+    if (!(l.size() == r.size())) detail::abortWithError("Zip: inputs differ in size");
+    auto L = detail::makeArgs(std::forward<Args>(args)...);
+    const void* lp = l.devicePointer();
+    const void* rp = r.devicePointer();
+    Matrix<Tout>& o = output.container();
+    const void* op = &o;
+    void* dst;
+    if (op == static_cast<const void*>(&l) || op == static_cast<const void*>(&r)) {
+      dst = o.devicePointer();
+    } else {
+      o.prepareOutputMatrix(l.size());
+      dst = o.deviceOutputPointer();
+    }
+    detail::check(
+        skl_zip_run(_h, dst, lp, rp, l.size().elemCount(), L.data(), L.count(), nullptr), "Zip");
+    o.dataOnDeviceModified();
+    L.finish();
+    return o;
+  }
+  std::string source() const { return _src; }
+  std::string func() const { return _func; }
+
+private:
+  std::string _src, _func;
+};
+
+template <typename Tl, typename Tr>
+class Zip<void(Tl, Tr)> : public detail::SkeletonBase {
+public:
+  Zip(const Source& source, const std::string& funcName = "func") : _src(source), _func(funcName)
+  {
+    skl_skeleton_desc d = blankDesc(SKL_ZIP);
+    d.source = _src.c_str();
+    d.func = _func.c_str();
+    d.types[0] = typeOf<void>();
+    d.types[1] = typeOf<Tl>();
+    d.types[2] = typeOf<Tr>();
+    d.ntypes = 3;
+    create(d);
+  }
+  template <typename... Args>
+  void operator()(const Vector<Tl>& l, const Vector<Tr>& r, Args&&... args)
+  {
+    if (l.size() != r.size()) detail::abortWithError("Zip: inputs differ in size");
+    auto L = detail::makeArgs(std::forward<Args>(args)...);
+    detail::check(skl_zip_run(_h, nullptr, l.devicePointer(), r.devicePointer(), l.size(),
+                              L.data(), L.count(), nullptr),
+                  "Zip");
+    L.finish();
+  }
+
+private:
+  std::string _src, _func;
+};
+
+}  // namespace skelcl
