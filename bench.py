@@ -64,12 +64,243 @@ def cpu_baseline(nthreads: int):
     }
 
 
+def _time(stream, fn, reps=3, warm=1):
+    """Median of `reps` HIP-event timings (ms) of fn() on `stream`."""
+    import skelcl_amd as skelcl
+    for _ in range(warm):
+        fn()
+    ts = []
+    for _ in range(reps):
+        a, b = skelcl.Event(), skelcl.Event()
+        a.record(stream)
+        fn()
+        b.record(stream)
+        b.synchronize()
+        ts.append(a.elapsed_ms(b))
+    return statistics.median(ts)
+
+
+def run_extras(skelcl, L, ck, stream, args):
+    """The other configurations, one measurement each (not part of `value`)."""
+    import ctypes as C
+    import oracle
+    st = stream.handle
+    ex = {}
+
+    # ---- configuration 1: dot product N=2^20 (Zip x*y then Reduce +), int Reduce
+    n1 = 1 << 20
+    x1, y1 = oracle.gen_f32_sym(0x1001, n1), oracle.gen_f32_sym(0x1002, n1)
+    X1, Y1 = skelcl.DeviceBuffer.from_array(x1), skelcl.DeviceBuffer.from_array(y1)
+    T1, o1 = skelcl.DeviceBuffer(4 * n1), skelcl.DeviceBuffer(16)
+    mult = skelcl.Zip("float(float, float)", "float func(float x, float y){ return x*y; }")
+    add = skelcl.Reduce("float(float)", "float func(float x, float y){ return x+y; }", "0")
+    addi = skelcl.Reduce("int(int)", "int func(int x, int y){ return x+y; }", "0")
+    k1 = oracle.gen_i32_below(0x1003, n1, 1000)
+    K1 = skelcl.DeviceBuffer.from_array(k1)
+
+    def dot_unfused():
+        ck(L.skl_zip_run(mult.handle, T1.ptr, X1.ptr, Y1.ptr, n1, None, 0, st))
+        ck(L.skl_reduce_run(add.handle, o1.ptr, T1.ptr, n1, None, 0, st))
+
+    def dot_fused():
+        ck(L.skl_zip_reduce_run(mult.handle, add.handle, o1.ptr, X1.ptr, Y1.ptr, n1, st))
+
+    t_unf = _time(stream, dot_unfused, reps=20)
+    got_unf = float(o1.to_numpy(np.float32, 1)[0])
+    t_fus = _time(stream, dot_fused, reps=20)
+    got_fus = float(o1.to_numpy(np.float32, 1)[0])
+    # graph-captured fused pipeline
+    g = C.c_void_p()
+    ck(L.skl_graph_begin_capture(st))
+    dot_fused()
+    ck(L.skl_graph_end_capture(st, C.byref(g)))
+    t_graph = _time(stream, lambda: ck(L.skl_graph_launch(g, st)), reps=20)
+    got_graph = float(o1.to_numpy(np.float32, 1)[0])
+    ck(L.skl_graph_destroy(g))
+    t_int = _time(stream, lambda: ck(L.skl_reduce_run(addi.handle, o1.ptr, K1.ptr, n1, None, 0, st)),
+                  reps=20)
+    got_int = int(o1.to_numpy(np.int32, 1)[0])
+    ref = oracle.dot_f64(x1, y1)
+    ex["dot_2^20"] = {
+        "unfused_ms": round(t_unf, 4), "fused_ms": round(t_fus, 4), "graph_ms": round(t_graph, 4),
+        "fused_GBps": round(8 * n1 / (t_fus * 1e-3) / 1e9, 1),
+        "err_vs_fp64": {"unfused": abs(got_unf - ref), "fused": abs(got_fus - ref),
+                        "graph": abs(got_graph - ref),
+                        "cpu_reference_order": abs(float(oracle.reduce_add_f32_reforder(
+                            oracle.zip_mul_f32(x1, y1))) - ref)},
+        "int_reduce_ms": round(t_int, 4),
+        "int_reduce_exact": bool(np.uint32(got_int & 0xFFFFFFFF)
+                                 == oracle.reduce_add_u32_reforder(k1)),
+    }
+
+    # ---- configuration 3: MapOverlap 8192^2 Gaussian R=5 NEUTRAL + adjoint
+    R, S = 5, 8192
+    w = oracle.gaussian_weights(R, 2.0)
+    img = skelcl.DeviceBuffer(4 * S * S)
+    Wb = skelcl.DeviceBuffer(4 * S * S)
+    ck(L.skl_generate_f32_unit(img.ptr, S * S, 0x3001, 255.0, st))
+    ck(L.skl_generate_f32_sym(Wb.ptr, S * S, 0x3002, st))
+    outb, dinb = skelcl.DeviceBuffer(4 * S * S), skelcl.DeviceBuffer(4 * S * S)
+    wv = skelcl.DeviceBuffer.from_array(w.ravel())
+    gauss = skelcl.MapOverlap("float(float)", """float func(input_matrix_t img, __global float* w) {
+  float sum = 0.0f;
+  for (int dy = -5; dy <= 5; ++dy)
+    for (int dx = -5; dx <= 5; ++dx)
+      sum += getData(img, dx, dy) * w[(dy + 5) * 11 + (dx + 5)];
+  return sum;
+}""", R, skelcl.NEUTRAL, 0)
+    warg, nw, keep, _ = gauss._marshal((wv,))
+    t_mo = _time(stream, lambda: ck(L.skl_mapoverlap_run(gauss.handle, outb.ptr, img.ptr, S, S,
+                                                         warg, nw, st)), reps=5)
+    t_adj = _time(stream, lambda: ck(L.skl_mapoverlap_stencil_backward(
+        gauss.handle, dinb.ptr, Wb.ptr, wv.ptr, S, S, st)), reps=5)
+    himg = oracle.gen_f32_unit(0x3001, S * S, 255.0).reshape(S, S)
+    rng = np.random.default_rng(0x3003)
+    pix = rng.integers(0, S * S, 64).astype(np.int64)
+    o = outb.to_numpy(np.float32)
+    ref = oracle.mapoverlap_stencil_f64(himg, w, R, pix, 0)
+    hW = oracle.gen_f32_sym(0x3002, S * S).reshape(S, S)
+    dref = oracle.mapoverlap_stencil_adjoint_f64(hW, w, R, pix, 0)
+    d = dinb.to_numpy(np.float32)
+    flop = 2 * (2 * R + 1) ** 2 * S * S
+    ex["mapoverlap_8192^2_r5"] = {
+        "forward_ms": round(t_mo, 4), "forward_GBps": round(8 * S * S / (t_mo * 1e-3) / 1e9, 1),
+        "forward_hbm_frac": round(8 * S * S / (t_mo * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+        "forward_TFLOPs": round(flop / (t_mo * 1e-3) / 1e12, 2),
+        "forward_fp32_frac_of_157.3": round(flop / (t_mo * 1e-3) / 1e12 / 157.3, 4),
+        "adjoint_ms": round(t_adj, 4),
+        "max_rel_err_vs_fp64": float(np.max(np.abs(o[pix] - ref) / np.maximum(1, np.abs(ref)))),
+        "adjoint_max_abs_err_vs_fp64": float(np.max(np.abs(d[pix] - dref))),
+    }
+    del img, Wb, outb, dinb, o, d, himg, hW
+
+    # ---- configuration 4: AllPairs 8192^3 (FP32 / BF16X3 / FP8), sq-Euclidean
+    M = 8192
+    A = skelcl.DeviceBuffer(4 * M * M)
+    B = skelcl.DeviceBuffer(4 * M * M)
+    Cb = skelcl.DeviceBuffer(4 * M * M)
+    ck(L.skl_generate_f32_sym(A.ptr, M * M, 0x4001, st))
+    ck(L.skl_generate_f32_sym(B.ptr, M * M, 0x4002, st))
+    hA = oracle.gen_f32_sym(0x4001, M * M).reshape(M, M)
+    hB = oracle.gen_f32_sym(0x4002, M * M).reshape(M, M)
+    rows = np.sort(np.random.default_rng(7).choice(M, 32, replace=False)).astype(np.int32)
+    ref = oracle.allpairs_mm_f64(hA, hB, rows=rows)
+    flop = 2.0 * M * M * M
+    ap = {}
+    for name, prec, peak in (("fp32", skelcl.PREC_FP32, 157.3), ("bf16x3", skelcl.PREC_BF16X3, 2500.0),
+                             ("fp8", skelcl.PREC_FP8, 5000.0)):
+        t = _time(stream, lambda: ck(L.skl_matmul_f32(Cb.ptr, A.ptr, B.ptr, M, M, M, 0, 0, prec, st)),
+                  reps=3)
+        c = Cb.to_numpy(np.float32).reshape(M, M)[rows].astype(np.float64)
+        err = float(np.linalg.norm(c - ref) / np.linalg.norm(ref))
+        tf = flop / (t * 1e-3) / 1e12
+        mfma_work = flop * (6 if name == "bf16x3" else 1)
+        ap[name] = {"ms": round(t, 3), "TFLOPs_effective": round(tf, 1),
+                    "unit_utilisation": round(mfma_work / (t * 1e-3) / 1e12 / peak, 4),
+                    "normwise_err_vs_fp64": err}
+    ap["bf16x3_speedup_vs_fp32"] = round(ap["fp32"]["ms"] / ap["bf16x3"]["ms"], 2)
+    dC = skelcl.DeviceBuffer(4 * M * M)
+    ck(L.skl_generate_f32_sym(dC.ptr, M * M, 0x4004, st))
+    zip_ = skelcl.Zip("float(float, float)", "float func(float x, float y){ return x*y; }")
+    apk = skelcl.AllPairs("float(float, float)", reduce=add, zip=zip_,
+                          precision=skelcl.PREC_BF16X3)
+    t_bw = _time(stream, lambda: ck(L.skl_allpairs_backward(apk.handle, A.ptr, B.ptr, Cb.ptr,
+                                                            Cb.ptr, dC.ptr, M, M, M, st)),
+                 reps=2)
+    ap["backward_bf16x3_ms"] = round(t_bw, 3)
+    del A, B, Cb, dC, hA, hB
+    n, d = 16384, 256
+    P = skelcl.DeviceBuffer(4 * n * d)
+    ck(L.skl_generate_f32_sym(P.ptr, n * d, 0x4003, st))
+    hP = oracle.gen_f32_sym(0x4003, n * d).reshape(n, d)
+    Pt = skelcl.DeviceBuffer.from_array(np.ascontiguousarray(hP.T))
+    D = skelcl.DeviceBuffer(4 * n * n)
+    sq = skelcl.AllPairs("float(float, float)",
+                         "float func(lmatrix_t* row, rmatrix_t* col, const unsigned int dim) {"
+                         " float s = 0; for (unsigned k = 0; k < dim; ++k) {"
+                         " float t = getElementFromRow(row, k) - getElementFromColumn(col, k);"
+                         " s += t*t; } return s; }")
+    t_sq = _time(stream, lambda: ck(L.skl_allpairs_run(sq.handle, D.ptr, P.ptr, Pt.ptr, n, n, d,
+                                                       None, 0, st)), reps=3)
+    rows = np.array([0, 1, 4097, 16383], np.int32)
+    dref = oracle.allpairs_sqdist_f64(hP, np.ascontiguousarray(hP.T), rows=rows)
+    dg = D.to_numpy(np.float32).reshape(n, n)[rows]
+    ap["sqeuclid_16384x256"] = {"ms": round(t_sq, 3),
+                               "TFLOPs": round(3.0 * n * n * d / (t_sq * 1e-3) / 1e12, 1),
+                               "max_rel_err_vs_fp64": float(np.max(np.abs(dg - dref) /
+                                                                   np.maximum(1, dref)))}
+    ex["allpairs_8192^3"] = ap
+    del P, Pt, D
+
+    # ---- configuration 5: batched segmented Reduce/Scan + 512 small products
+    import json as _json
+    with open(os.path.join(ROOT, "tests", "golden", "batched_shapes.json")) as f:
+        fx = _json.load(f)
+    lens = np.array(fx["segment_lengths"], np.int64)
+    off = np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)
+    tot = int(off[-1])
+    segx = skelcl.DeviceBuffer(4 * tot)
+    segy = skelcl.DeviceBuffer(4 * tot)
+    ck(L.skl_generate_f32_sym(segx.ptr, tot, 0x5002, st))
+    offb = skelcl.DeviceBuffer.from_array(off)
+    sums = skelcl.DeviceBuffer(4 * len(lens))
+    scan = skelcl.Scan("float(float)", "float func(float x, float y){ return x+y; }", "0.0f")
+    t_sr = _time(stream, lambda: ck(L.skl_reduce_segmented_run(add.handle, sums.ptr, segx.ptr,
+                                                               offb.ptr, len(lens), st)), reps=3)
+    t_ss = _time(stream, lambda: ck(L.skl_scan_segmented_run(scan.handle, segy.ptr, segx.ptr,
+                                                             offb.ptr, len(lens), 1, st)), reps=3)
+    t_sb = _time(stream, lambda: ck(L.skl_scan_backward(scan.handle, segy.ptr, segx.ptr, tot, 1,
+                                                        st)), reps=3)
+    shapes = fx["gemm_shapes"]
+    descs = (skelcl._lib.skl_gemm_desc * len(shapes))()
+    bufs = []
+    flops = 0.0
+    for i, (m, nn, k) in enumerate(shapes):
+        a_, b_, c_ = (skelcl.DeviceBuffer(4 * m * k), skelcl.DeviceBuffer(4 * k * nn),
+                      skelcl.DeviceBuffer(4 * m * nn))
+        ck(L.skl_generate_f32_sym(a_.ptr, m * k, 0x5004 + 2 * i, st))
+        ck(L.skl_generate_f32_sym(b_.ptr, k * nn, 0x5005 + 2 * i, st))
+        bufs += [a_, b_, c_]
+        descs[i].A, descs[i].B, descs[i].C = a_.value, b_.value, c_.value
+        descs[i].M, descs[i].N, descs[i].K = m, nn, k
+        flops += 2.0 * m * nn * k
+    dd = skelcl.DeviceBuffer(C.sizeof(descs))
+    ck(L.skl_memcpy_h2d(dd.ptr, 0, C.cast(descs, C.c_void_p), C.sizeof(descs), None, 1))
+    bt = {}
+    for name, prec in (("fp32", skelcl.PREC_FP32), ("bf16x3", skelcl.PREC_BF16X3)):
+        t = _time(stream, lambda: ck(L.skl_matmul_batched_f32(dd.ptr, len(shapes), prec, st)),
+                  reps=3)
+        bt[name] = {"ms": round(t, 3), "TFLOPs": round(flops / (t * 1e-3) / 1e12, 1)}
+    hs = sums.to_numpy(np.float32)
+    chk = [0, 1, 2, 100]
+    hx = {s: oracle.gen_f32_sym(0x5002, int(off[s + 1]))[off[s]:off[s + 1]] for s in chk}
+    ex["batched"] = {
+        "segments": len(lens), "elements": tot,
+        "segmented_reduce_ms": round(t_sr, 3),
+        "segmented_reduce_GBps": round(4 * tot / (t_sr * 1e-3) / 1e9, 1),
+        "segmented_scan_ms": round(t_ss, 3),
+        "segmented_scan_GBps": round(8 * tot / (t_ss * 1e-3) / 1e9, 1),
+        "scan_adjoint_ms": round(t_sb, 3),
+        "segment_sum_max_abs_err_vs_fp64": float(max(abs(float(hs[s]) - oracle.sum_f64(hx[s]))
+                                                     for s in chk)),
+        "gemms": len(shapes), "gemm_flop": flops, "gemm": bt,
+    }
+    return ex
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--extras", type=int, default=1,
+                    help="1: also measure the other configurations (AllPairs 8192^3 in "
+                         "FP32 / BF16X3 / FP8 with errors vs FP64, MapOverlap 8192^2 R=5 "
+                         "forward + adjoint, batched segmented Reduce/Scan + 512 GEMMs, "
+                         "fused dot product)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also time an identity Map (copy) leg, for counter calibration")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py drives one GPU; --gpus {args.gpus} is not supported by this build "
@@ -115,6 +346,15 @@ def main() -> int:
 
     legs = [("scan", run_scan, 8 * N), ("reduce", run_reduce, 4 * N),
             ("saxpy", run_saxpy, 12 * N)]
+    if args.copy_calibration:
+        # identity Map over N floats: a plain copy kernel of known size
+        # (4N bytes read, 4N written) that calibrates the HBM byte counters
+        copy = skelcl.Map("float(float)", "float func(float x){ return x; }")
+
+        def run_copy():
+            ck(L.skl_map_run(copy.handle, ys.ptr, x.ptr, N, None, 0, st))
+
+        legs.append(("copy", run_copy, 8 * N))
 
     for _ in range(args.warmup):
         for _, f, _ in legs:
@@ -140,8 +380,10 @@ def main() -> int:
     per = {name: [ev[s][i].elapsed_ms(ev[s][i + 1]) for s in range(args.steps)]
            for i, (name, _, _) in enumerate(legs)}
 
-    total_bytes = sum(b for _, _, b in legs)
+    total_bytes = sum(b for name, _, b in legs if name != "copy")
     ms_per_step = total_ms / args.steps
+    if args.copy_calibration:  # profiling run: the step is the workload legs only
+        ms_per_step = sum(statistics.mean(per[name]) for name, _, _ in legs if name != "copy")
     value = total_bytes / (ms_per_step * 1e-3) / 1e9
     legs_out = {}
     for name, _, b in legs:
@@ -201,6 +443,10 @@ def main() -> int:
                       f"scan {cb['times_s']['scan']:.2f}s, reduce "
                       f"{cb['times_s']['reduce']:.2f}s, saxpy {cb['times_s']['saxpy']:.2f}s"}
         result["scan_last_matches_cpu"] = bool(result["scan_last_matches_cpu"])
+    if args.extras:
+        # free the bandwidth workload's buffers first (7 GiB)
+        del xs, ys, xr, x, y, sout
+        result["extras"] = run_extras(skelcl, L, ck, stream, args)
     print(json.dumps(result), flush=True)
     return 0
 

@@ -248,9 +248,20 @@ typedef struct {
 int skl_matmul_batched_f32(const skl_gemm_desc* descs_device, int count,
                            int precision, skl_stream stream);
 
-/* Autotuner: time candidate launch shapes for this skeleton at this size
- * and cache the best per (kernel source hash, size class). */
+/* Autotuner: time candidate launch shapes for this skeleton at size n
+ * (elements; for MapOverlap an n-pixel square, for AllPairs n-sized square
+ * products) on scratch buffers, keep the fastest and cache it per
+ * (kind, user source, types, log2 size) in memory and in
+ * $SKELCL_CACHE_DIR/autotune.txt.  Tuned: packs in flight per thread and
+ * grid cap (Map/Zip/Reduce), packs per lane (Scan tile), rows per thread
+ * (MapOverlap).  enable = 0 restores the defaults.  skl_autotune needs a user
+ * function without extra parameters; skl_autotune_args times with the
+ * caller's extra arguments. */
 int skl_autotune(skl_skeleton s, size_t n, int enable);
+int skl_autotune_args(skl_skeleton s, size_t n, const skl_arg* args, int nargs);
+/* the tuned shape in use: {unroll, grid blocks per CU, scan packs per lane,
+ * MapOverlap rows per thread} */
+int skl_skeleton_launch_shape(skl_skeleton s, int shape[4]);
 
 #ifdef __cplusplus
 }

@@ -581,6 +581,16 @@ class Map(_Skeleton):
         self._written(written)
         return o
 
+    def backward(self, inp, d_out, *args, stream=None):
+        """Adjoint (needs deriv_source / deriv_func = f'): d_in = d_out * f'(in)."""
+        arr, n_args, keep, _ = self._marshal(args)
+        n = len(inp)
+        d = Vector(size=n, etype=self.types[1])
+        check(lib().skl_map_backward(self.handle, d.device_buffer_for_output(), inp.device_ptr(),
+                                     d_out.device_ptr(), n, arr, n_args, _stream(stream)))
+        d.data_on_device_modified()
+        return d
+
 
 class Zip(_Skeleton):
     """skelcl::Zip<Tout(Tl, Tr)> (ZipDef.h:70-442)."""
@@ -614,6 +624,21 @@ class Zip(_Skeleton):
         self._written(written)
         return o
 
+    def backward(self, left, right, d_out, *args, stream=None):
+        """Adjoint (needs deriv_source with deriv_func = df/dl, deriv_func2 =
+        df/dr): returns (d_left, d_right)."""
+        arr, n_args, keep, _ = self._marshal(args)
+        n = len(left)
+        dl = Vector(size=n, etype=self.types[1])
+        dr = Vector(size=n, etype=self.types[2])
+        check(lib().skl_zip_backward(self.handle, dl.device_buffer_for_output(),
+                                     dr.device_buffer_for_output(), left.device_ptr(),
+                                     right.device_ptr(), d_out.device_ptr(), n, arr, n_args,
+                                     _stream(stream)))
+        dl.data_on_device_modified()
+        dr.data_on_device_modified()
+        return dl, dr
+
 
 class Reduce(_Skeleton):
     """skelcl::Reduce<T(T)> (ReduceDef.h:79-293): returns a Vector of size 1."""
@@ -633,6 +658,35 @@ class Reduce(_Skeleton):
                                    n, arr, n_args, _stream(stream)))
         o.data_on_device_modified()
         self._written(written)
+        return o
+
+    def fused(self, zip_: "Zip", left, right, stream=None):
+        """Zip -> Reduce in one kernel (extension): reduce_i zip(l[i], r[i])."""
+        o = Vector(size=1, etype=self.types[0])
+        check(lib().skl_zip_reduce_run(zip_.handle, self.handle, o.device_buffer_for_output(),
+                                       left.device_ptr(), right.device_ptr(), len(left),
+                                       _stream(stream)))
+        o.data_on_device_modified()
+        return o
+
+    def backward(self, d_out, n, stream=None):
+        """Adjoint of Reduce(+): every input gets d_out[0]."""
+        d = Vector(size=n, etype=self.types[0])
+        check(lib().skl_reduce_backward(self.handle, d.device_buffer_for_output(),
+                                        d_out.device_ptr(), n, _stream(stream)))
+        d.data_on_device_modified()
+        return d
+
+    def segmented(self, inp, offsets, stream=None):
+        """Batched segmented Reduce(+) (extension): out[s] = sum of
+        inp[offsets[s]:offsets[s+1]], all segments in one persistent launch."""
+        off = offsets if isinstance(offsets, Vector) else Vector(np.asarray(offsets, np.int64))
+        nseg = len(off) - 1
+        o = Vector(size=nseg, etype=self.types[0])
+        check(lib().skl_reduce_segmented_run(self.handle, o.device_buffer_for_output(),
+                                             inp.device_ptr(), off.device_ptr(), nseg,
+                                             _stream(stream)))
+        o.data_on_device_modified()
         return o
 
 
@@ -656,6 +710,28 @@ class Scan(_Skeleton):
                                  _stream(stream)))
         o.data_on_device_modified()
         self._written(written)
+        return o
+
+    def backward(self, d_out, inclusive=False, stream=None):
+        """Adjoint of Scan(+): the scan of d_out read from the end."""
+        n = len(d_out)
+        d = Vector(size=n, etype=self.types[0])
+        check(lib().skl_scan_backward(self.handle, d.device_buffer_for_output(),
+                                      d_out.device_ptr(), n, int(bool(inclusive)),
+                                      _stream(stream)))
+        d.data_on_device_modified()
+        return d
+
+    def segmented(self, inp, offsets, inclusive=False, stream=None):
+        """Batched segmented Scan(+) (extension): every segment scanned on its
+        own, all segments in one persistent launch."""
+        off = offsets if isinstance(offsets, Vector) else Vector(np.asarray(offsets, np.int64))
+        nseg = len(off) - 1
+        o = Vector(size=len(inp), etype=self.types[0])
+        check(lib().skl_scan_segmented_run(self.handle, o.device_buffer_for_output(),
+                                           inp.device_ptr(), off.device_ptr(), nseg,
+                                           int(bool(inclusive)), _stream(stream)))
+        o.data_on_device_modified()
         return o
 
 
@@ -682,6 +758,17 @@ class MapOverlap(_Skeleton):
         o.data_on_device_modified()
         self._written(written)
         return o
+
+    def stencil_backward(self, d_out: Matrix, weights: Vector, stream=None):
+        """Adjoint of a weighted-stencil user function: the transposed stencil
+        of d_out under the same padding."""
+        rows, cols = d_out.shape
+        d = Matrix(shape=(rows, cols), etype=self.types[1])
+        check(lib().skl_mapoverlap_stencil_backward(self.handle, d.device_buffer_for_output(),
+                                                    d_out.device_ptr(), weights.device_ptr(),
+                                                    rows, cols, _stream(stream)))
+        d.data_on_device_modified()
+        return d
 
 
 class AllPairs(_Skeleton):
@@ -716,6 +803,20 @@ class AllPairs(_Skeleton):
         o.data_on_device_modified()
         self._written(written)
         return o
+
+    def backward(self, left: Matrix, right: Matrix, d_out: Matrix, stream=None):
+        """Adjoint of the (+,*) contraction: (dA, dB) = (dC.B^T, A^T.dC)."""
+        M, K = left.shape
+        N = right.column_count()
+        dA = Matrix(shape=(M, K), etype=self.types[1])
+        dB = Matrix(shape=(K, N), etype=self.types[2])
+        check(lib().skl_allpairs_backward(self.handle, dA.device_buffer_for_output(),
+                                          dB.device_buffer_for_output(), left.device_ptr(),
+                                          right.device_ptr(), d_out.device_ptr(), M, N, K,
+                                          _stream(stream)))
+        dA.data_on_device_modified()
+        dB.data_on_device_modified()
+        return dA, dB
 
 
 def check_source(kind: int, types: Sequence[str], source: str, func: str = "func",

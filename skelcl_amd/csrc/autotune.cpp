@@ -1,0 +1,233 @@
+// Autotuner: times candidate launch shapes of a skeleton on scratch buffers
+// and keeps the fastest, cached per (kind, user source, types, log2 size) in
+// memory and in a text file in the code-object cache directory.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <map>
+#include <mutex>
+#include <sstream>
+#include <vector>
+
+#include "common.h"
+#include "skeleton.h"
+#include "skelcl_skeletons.h"
+
+using namespace skl;
+
+namespace {
+
+struct Shape {
+  int unroll, blocks_per_cu, scan_j, mo_rb;
+};
+
+std::mutex g_mutex;
+std::map<std::string, Shape> g_cache;
+bool g_loaded = false;
+
+std::string tune_path()
+{
+  const char* e = getenv("SKELCL_CACHE_DIR");
+  std::string dir;
+  if (e && *e) dir = e;
+  else {
+    const char* home = getenv("HOME");
+    dir = std::string(home && *home ? home : "/tmp") + "/.cache/skelcl_amd";
+  }
+  return dir + "/autotune.txt";
+}
+
+void load_cache()
+{
+  if (g_loaded) return;
+  g_loaded = true;
+  std::ifstream f(tune_path());
+  std::string key;
+  Shape s;
+  while (f >> key >> s.unroll >> s.blocks_per_cu >> s.scan_j >> s.mo_rb) g_cache[key] = s;
+}
+
+void save_entry(const std::string& key, const Shape& s)
+{
+  std::ofstream f(tune_path(), std::ios::app);
+  if (f) f << key << " " << s.unroll << " " << s.blocks_per_cu << " " << s.scan_j << " " << s.mo_rb
+           << "\n";
+}
+
+std::string tune_key(const skl_skeleton_s* s, size_t n)
+{
+  std::ostringstream o;
+  o << s->kind << "|" << s->src << "|" << s->src2 << "|" << s->identity << "|" << s->precision
+    << "|" << s->overlap << "|" << s->padding;
+  for (auto& t : s->types) o << "|" << t.name;
+  int cls = n ? (int)std::floor(std::log2((double)n)) : 0;
+  return sha1_hex(o.str()) + ":" + std::to_string(cls) + ":" + device_props().gcnArchName;
+}
+
+Shape current(const skl_skeleton_s* s)
+{
+  int cus = device_props().multiProcessorCount > 0 ? device_props().multiProcessorCount : 256;
+  return Shape{s->unroll, s->max_blocks / cus, s->scan_j, s->mo_rb};
+}
+
+void apply(skl_skeleton_s* s, const Shape& sh)
+{
+  int cus = device_props().multiProcessorCount > 0 ? device_props().multiProcessorCount : 256;
+  bool changed = sh.unroll != s->unroll || sh.scan_j != s->scan_j || sh.mo_rb != s->mo_rb;
+  s->unroll = sh.unroll;
+  s->scan_j = sh.scan_j;
+  s->mo_rb = sh.mo_rb;
+  s->max_blocks = cus * sh.blocks_per_cu;
+  if (changed) {
+    for (auto& kv : s->progs) skl_program_release(kv.second.program);
+    s->progs.clear();  // the shape is compiled into the kernels
+  }
+}
+
+std::vector<Shape> candidates(const skl_skeleton_s* s)
+{
+  Shape d = current(s);
+  std::vector<Shape> c;
+  switch (s->kind) {
+    case SKL_MAP: case SKL_ZIP: case SKL_REDUCE:
+      for (int u : {1, 2, 4, 8})
+        for (int b : {4, 8, 16, 64}) c.push_back(Shape{u, b, d.scan_j, d.mo_rb});
+      break;
+    case SKL_SCAN:
+      for (int j : {2, 4, 8, 16}) c.push_back(Shape{d.unroll, d.blocks_per_cu, j, d.mo_rb});
+      break;
+    case SKL_MAPOVERLAP:
+      for (int rb : {2, 4, 8, 16}) c.push_back(Shape{d.unroll, d.blocks_per_cu, d.scan_j, rb});
+      break;
+    default:
+      c.push_back(d);
+  }
+  return c;
+}
+
+struct Buf {
+  void* p = nullptr;
+  ~Buf()
+  {
+    if (p) hipFree(p);
+  }
+};
+
+int alloc(Buf& b, size_t bytes)
+{
+  SKL_HIP_TRY(hipMalloc(&b.p, bytes ? bytes : 16));
+  SKL_HIP_TRY(hipMemset(b.p, 0, bytes ? bytes : 16));
+  return SKL_OK;
+}
+
+}  // namespace
+
+static int tune(skl_skeleton s, size_t n, const skl_arg* args, int nargs)
+{
+  SKL_TRY(ensure_initialized());
+  if (!s) return fail(SKL_ERR_INVALID, "skeleton is NULL");
+  if (n == 0) return fail(SKL_ERR_INVALID, "autotune size must be positive");
+  if (s->kind == SKL_ALLPAIRS || s->kind == SKL_ALLPAIRS_GENERAL ||
+      s->kind == SKL_MAP_INDEX || s->kind == SKL_MAP_INDEXPOINT)
+    return SKL_OK;  // fixed shapes
+  if (s->simple) return SKL_OK;  // one element per work-item: nothing to tune
+  const std::string key = tune_key(s, n);
+  {
+    std::lock_guard<std::mutex> lk(g_mutex);
+    load_cache();
+    auto it = g_cache.find(key);
+    if (it != g_cache.end()) {
+      apply(s, it->second);
+      return SKL_OK;
+    }
+  }
+  const size_t es = s->types.size() > 1 ? std::max(s->types[0].size, s->types[1].size)
+                                        : s->types[0].size;
+  size_t rows = 0, cols = 0;
+  if (s->kind == SKL_MAPOVERLAP) {
+    rows = cols = (size_t)std::ceil(std::sqrt((double)n));
+    n = rows * cols;
+  }
+  Buf in, in2, out;
+  SKL_TRY(alloc(in, n * es));
+  SKL_TRY(alloc(in2, n * es));
+  SKL_TRY(alloc(out, n * es + 64));
+  hipStream_t st = resolve_stream(nullptr);
+  auto run = [&]() -> int {
+    switch (s->kind) {
+      case SKL_MAP: return skl_map_run(s, out.p, in.p, n, args, nargs, nullptr);
+      case SKL_ZIP: return skl_zip_run(s, out.p, in.p, in2.p, n, args, nargs, nullptr);
+      case SKL_REDUCE: return skl_reduce_run(s, out.p, in.p, n, args, nargs, nullptr);
+      case SKL_SCAN: return skl_scan_run(s, out.p, in.p, n, 1, args, nargs, nullptr);
+      case SKL_MAPOVERLAP:
+        return skl_mapoverlap_run(s, out.p, in.p, rows, cols, args, nargs, nullptr);
+    }
+    return SKL_OK;
+  };
+  hipEvent_t e0, e1;
+  SKL_HIP_TRY(hipEventCreate(&e0));
+  SKL_HIP_TRY(hipEventCreate(&e1));
+  Shape best = current(s);
+  float best_ms = 1e30f;
+  for (const Shape& c : candidates(s)) {
+    apply(s, c);
+    int stt = run();  // compiles this shape, warms up
+    if (stt != SKL_OK) continue;
+    SKL_HIP_TRY(hipStreamSynchronize(st));
+    const int reps = 5;
+    SKL_HIP_TRY(hipEventRecord(e0, st));
+    for (int r = 0; r < reps; ++r) SKL_TRY(run());
+    SKL_HIP_TRY(hipEventRecord(e1, st));
+    SKL_HIP_TRY(hipEventSynchronize(e1));
+    float ms = 0;
+    SKL_HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+    ms /= reps;
+    logmsg(2, "autotune %s: unroll %d blocks/CU %d scan_j %d rb %d -> %.4f ms", key.c_str(),
+           c.unroll, c.blocks_per_cu, c.scan_j, c.mo_rb, ms);
+    if (ms < best_ms) {
+      best_ms = ms;
+      best = c;
+    }
+  }
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  apply(s, best);
+  std::lock_guard<std::mutex> lk(g_mutex);
+  g_cache[key] = best;
+  save_entry(key, best);
+  return SKL_OK;
+}
+
+extern "C" int skl_autotune(skl_skeleton s, size_t n, int enable)
+{
+  if (!s) return fail(SKL_ERR_INVALID, "skeleton is NULL");
+  if (!enable) {
+    int cus = device_props().multiProcessorCount > 0 ? device_props().multiProcessorCount : 256;
+    apply(s, Shape{4, 8, 4, 8});
+    s->max_blocks = cus * 8;
+    return SKL_OK;
+  }
+  if (!s->extras.empty())
+    return fail(SKL_ERR_INVALID,
+                "the user function takes extra arguments: use skl_autotune_args");
+  return tune(s, n, nullptr, 0);
+}
+
+extern "C" int skl_autotune_args(skl_skeleton s, size_t n, const skl_arg* args, int nargs)
+{
+  return tune(s, n, args, nargs);
+}
+
+extern "C" int skl_skeleton_launch_shape(skl_skeleton s, int shape[4])
+{
+  if (!s || !shape) return fail(SKL_ERR_INVALID, "NULL argument");
+  Shape c = current(s);
+  shape[0] = c.unroll;
+  shape[1] = c.blocks_per_cu;
+  shape[2] = c.scan_j;
+  shape[3] = c.mo_rb;
+  return SKL_OK;
+}

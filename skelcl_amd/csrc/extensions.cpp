@@ -234,7 +234,3 @@ extern "C" int skl_matmul_batched_f32(const skl_gemm_desc* descs_device, int cou
   return skl_internal_batched_gemm_ws(descs_device, count, precision, ws, resolve_stream(stream));
 }
 
-extern "C" int skl_autotune(skl_skeleton, size_t, int)
-{
-  return fail(SKL_ERR_UNSUPPORTED, "autotuning is not available");
-}

@@ -1021,7 +1021,12 @@ extern "C" int skl_allpairs_run(skl_skeleton s, void* out, const void* left, con
   auto a = with_extras({&left, &right, &out, &M, &N, &K}, pk);
   if (s->kind == SKL_ALLPAIRS) {
     std::string key = first_key(s);
-    if (key == "fma" && s->precision != SKL_PREC_DEFAULT && s->precision != SKL_PREC_FP32)
+    // the (+,*) contraction on float from a zero identity is a matmul: the
+    // native kernels (FP32 VALU with 8x8 register blocks, or the MFMA forms)
+    const bool zero_id = s->identity == "0" || s->identity == "0.0" || s->identity == "0.0f" ||
+                         s->identity == "0.f" || s->identity == "0.";
+    if (key == "fma" && (zero_id || (s->precision != SKL_PREC_DEFAULT &&
+                                     s->precision != SKL_PREC_FP32)))
       return skl_matmul_f32((float*)out, (const float*)left, (const float*)right, rows, cols,
                             dim, 0, 0, s->precision, stream);
     if (pk.dyn_lds) return fail(SKL_ERR_UNSUPPORTED, "AllPairs user functions cannot take Local");

@@ -26,6 +26,35 @@ public:
     d.ntypes = 2;
     create(d);
   }
+  // extension: with the derivative f'(x, extras...) for backward()
+  Map(const Source& source, const std::string& funcName, const Source& derivSource,
+      const std::string& derivFunc)
+      : _src(source), _func(funcName), _dsrc(derivSource), _dfunc(derivFunc)
+  {
+    skl_skeleton_desc d = blankDesc(SKL_MAP);
+    d.source = _src.c_str();
+    d.func = _func.c_str();
+    d.deriv_source = _dsrc.c_str();
+    d.deriv_func = _dfunc.c_str();
+    d.types[0] = typeOf<Tout>();
+    d.types[1] = typeOf<Tin>();
+    d.ntypes = 2;
+    create(d);
+  }
+
+  // extension: d_in = d_out * f'(in, extras...)
+  template <typename... Args>
+  Vector<Tin> backward(const Vector<Tin>& in, const Vector<Tout>& dOut, Args&&... args)
+  {
+    auto L = detail::makeArgs(std::forward<Args>(args)...);
+    Vector<Tin> d;
+    d.prepareOutput(in.size());
+    detail::check(skl_map_backward(_h, d.deviceOutputPointer(), in.devicePointer(),
+                                   dOut.devicePointer(), in.size(), L.data(), L.count(), nullptr),
+                  "Map backward");
+    d.dataOnDeviceModified();
+    return d;
+  }
 
   template <typename... Args>
   Vector<Tout> operator()(const Vector<Tin>& in, Args&&... args)
@@ -75,7 +104,7 @@ public:
   std::string func() const { return _func; }
 
 private:
-  std::string _src, _func;
+  std::string _src, _func, _dsrc, _dfunc;
 };
 
 template <typename Tin>

@@ -60,6 +60,22 @@ public:
   }
   unsigned int overlapRange() const { return _range; }
 
+  // extension: adjoint of a weighted-stencil user function
+  // sum_{dy,dx} w[(dy+R)(2R+1)+dx+R] * getData(m, dx, dy): the transposed
+  // stencil of dOut under the same padding
+  Matrix<Tin> stencilBackward(const Matrix<Tout>& dOut, const Vector<float>& weights)
+  {
+    Matrix<Tin> d;
+    d.prepareOutputMatrix(dOut.size());
+    detail::check(skl_mapoverlap_stencil_backward(
+                      _h, d.deviceOutputPointer(), dOut.devicePointer(),
+                      static_cast<const float*>(weights.devicePointer()), dOut.rowCount(),
+                      dOut.columnCount(), nullptr),
+                  "MapOverlap backward");
+    d.dataOnDeviceModified();
+    return d;
+  }
+
 private:
   std::string _src, _func;
   unsigned int _range;

@@ -63,6 +63,33 @@ public:
     return o;
   }
 
+  // extension: adjoint of Reduce(+), d in_i = d out[0] for n inputs
+  Vector<T> backward(const Vector<T>& dOut, size_t n)
+  {
+    Vector<T> d;
+    d.prepareOutput(n);
+    detail::check(skl_reduce_backward(_h, d.deviceOutputPointer(), dOut.devicePointer(), n,
+                                      nullptr),
+                  "Reduce backward");
+    d.dataOnDeviceModified();
+    return d;
+  }
+
+  // extension: batched segmented Reduce(+) -- out[s] folds
+  // in[offsets[s] .. offsets[s+1]) -- all segments in one persistent launch
+  Vector<T> segmented(const Vector<T>& in, const Vector<long>& offsets)
+  {
+    Vector<T> o;
+    int nseg = (int)offsets.size() - 1;
+    o.prepareOutput((size_t)(nseg > 0 ? nseg : 0));
+    detail::check(skl_reduce_segmented_run(_h, o.deviceOutputPointer(), in.devicePointer(),
+                                           static_cast<const int64_t*>(offsets.devicePointer()),
+                                           nseg, nullptr),
+                  "segmented Reduce");
+    o.dataOnDeviceModified();
+    return o;
+  }
+
   std::string source() const { return _src; }
   std::string func() const { return _func; }
   std::string id() const { return _id; }

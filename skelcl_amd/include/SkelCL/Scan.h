@@ -47,6 +47,33 @@ public:
     run(output, in, 1, std::forward<Args>(args)...);
     return output;
   }
+  // extension: adjoint of Scan(+), the scan of dOut read from its end
+  Vector<T> backward(const Vector<T>& dOut, bool inclusiveForward = false)
+  {
+    Vector<T> d;
+    d.prepareOutput(dOut.size());
+    detail::check(skl_scan_backward(_h, d.deviceOutputPointer(), dOut.devicePointer(),
+                                    dOut.size(), inclusiveForward ? 1 : 0, nullptr),
+                  "Scan backward");
+    d.dataOnDeviceModified();
+    return d;
+  }
+
+  // extension: batched segmented Scan(+), every segment on its own, one
+  // persistent launch
+  Vector<T> segmented(const Vector<T>& in, const Vector<long>& offsets, bool inclusive = false)
+  {
+    Vector<T> o;
+    o.prepareOutput(in.size());
+    int nseg = (int)offsets.size() - 1;
+    detail::check(skl_scan_segmented_run(_h, o.deviceOutputPointer(), in.devicePointer(),
+                                         static_cast<const int64_t*>(offsets.devicePointer()),
+                                         nseg, inclusive ? 1 : 0, nullptr),
+                  "segmented Scan");
+    o.dataOnDeviceModified();
+    return o;
+  }
+
   std::string source() const { return _src; }
   std::string func() const { return _func; }
   std::string id() const { return _id; }

@@ -24,6 +24,46 @@ public:
     d.ntypes = 3;
     create(d);
   }
+  // extension: with df/dl and df/dr (l, r, extras...) for backward()
+  Zip(const Source& source, const std::string& funcName, const Source& derivSource,
+      const std::string& dLeft, const std::string& dRight)
+      : _src(source), _func(funcName), _dsrc(derivSource), _dl(dLeft), _dr(dRight)
+  {
+    skl_skeleton_desc d = blankDesc(SKL_ZIP);
+    d.source = _src.c_str();
+    d.func = _func.c_str();
+    d.deriv_source = _dsrc.c_str();
+    d.deriv_func = _dl.c_str();
+    d.deriv_func2 = _dr.c_str();
+    d.types[0] = typeOf<Tout>();
+    d.types[1] = typeOf<Tl>();
+    d.types[2] = typeOf<Tr>();
+    d.ntypes = 3;
+    create(d);
+  }
+
+  // extension: d_l = d_out * df/dl, d_r = d_out * df/dr (either may be null)
+  template <typename... Args>
+  void backward(const Vector<Tl>& l, const Vector<Tr>& r, const Vector<Tout>& dOut,
+                Vector<Tl>* dL, Vector<Tr>* dR, Args&&... args)
+  {
+    auto L = detail::makeArgs(std::forward<Args>(args)...);
+    void* dl = nullptr;
+    void* dr = nullptr;
+    if (dL) {
+      dL->prepareOutput(l.size());
+      dl = dL->deviceOutputPointer();
+    }
+    if (dR) {
+      dR->prepareOutput(r.size());
+      dr = dR->deviceOutputPointer();
+    }
+    detail::check(skl_zip_backward(_h, dl, dr, l.devicePointer(), r.devicePointer(),
+                                   dOut.devicePointer(), l.size(), L.data(), L.count(), nullptr),
+                  "Zip backward");
+    if (dL) dL->dataOnDeviceModified();
+    if (dR) dR->dataOnDeviceModified();
+  }
 
   template <typename... Args>
   Vector<Tout> operator()(const Vector<Tl>& l, const Vector<Tr>& r, Args&&... args)
@@ -85,7 +125,7 @@ public:
   std::string func() const { return _func; }
 
 private:
-  std::string _src, _func;
+  std::string _src, _func, _dsrc, _dl, _dr;
 };
 
 template <typename Tl, typename Tr>

@@ -2,6 +2,7 @@
 // ReduceTests.cpp, ScanTests.cpp, MapOverlapTests.cpp, AllPairsTests.cpp,
 // IndexVectorTests.cpp, IndexMatrixTests.cpp, VectorTests.cpp) restated
 // against the skelcl_amd C++ API, which compiles the same host code.
+#include <SkelCL/Autodiff.h>
 #include <SkelCL/SkelCL.h>
 
 #include <cstdlib>
@@ -504,6 +505,46 @@ TEST(VectorTest, LazyCoherence)
   EXPECT_EQ(2, x[99]);
   skelcl::Vector<int> copy = x;  // deep copy through the host
   EXPECT_EQ(42, copy[0]);
+}
+
+// ------------------------------------------------------------------ autodiff
+TEST(AutodiffTest, FunctionPairs)
+{
+  using namespace skelcl::autodiff;
+  // Map: y = x*x, dy/dx = 2x
+  skelcl::Map<float(float)> sq("float func(float x){ return x*x; }", "func",
+                               "float d(float x){ return 2.0f*x; }", "d");
+  skelcl::Vector<float> x(1000), g(1000, 1.0f);
+  for (size_t i = 0; i < 1000; ++i) x[i] = 0.001f * i;
+  Context c1;
+  skelcl::Vector<float> y = MapFn<float>::forward(c1, sq, x);
+  skelcl::Vector<float> dx = MapFn<float>::backward(c1, sq, g);
+  for (size_t i = 0; i < 1000; ++i) EXPECT_EQ(2.0f * x[i], dx[i]);
+  // Reduce(+): broadcast
+  skelcl::Reduce<float(float)> sum("float func(float x, float y){ return x+y; }");
+  Context c2;
+  skelcl::Vector<float> s = ReduceSumFn<float>::forward(c2, sum, x);
+  skelcl::Vector<float> seed(1, 3.0f);
+  skelcl::Vector<float> ds = ReduceSumFn<float>::backward(c2, sum, seed);
+  EXPECT_EQ((size_t)1000, ds.size());
+  EXPECT_EQ(3.0f, ds[999]);
+  // Scan(+) exclusive: dx_i = sum_{j>i} dy_j
+  skelcl::Scan<float(float)> scan("float func(float x, float y){ return x+y; }", "0.0f");
+  Context c3;
+  skelcl::Vector<float> sc = ScanSumFn<float>::forward(c3, scan, x, false);
+  skelcl::Vector<float> dsc = ScanSumFn<float>::backward(c3, scan, g);
+  for (size_t i = 0; i < 1000; ++i) EXPECT_EQ((float)(999 - i), dsc[i]);
+  // MatMul: dA = dC . B^T
+  skelcl::Zip<float(float, float)> mul("float func(float x, float y){ return x*y; }");
+  skelcl::AllPairs<float(float, float)> mm(sum, mul);
+  skelcl::Matrix<float> A(skelcl::MatrixSize{4, 3}, 1.0f), B(skelcl::MatrixSize{3, 5}, 2.0f);
+  Context c4;
+  skelcl::Matrix<float> C = MatMulFn::forward(c4, mm, A, B);
+  EXPECT_EQ(6.0f, C[3][4]);
+  skelcl::Matrix<float> dC(skelcl::MatrixSize{4, 5}, 1.0f);
+  auto grads = MatMulFn::backward(c4, mm, dC);
+  EXPECT_EQ(10.0f, grads.first[0][0]);   // sum_j dC[i][j] * B[k][j] = 5 * 2
+  EXPECT_EQ(4.0f, grads.second[2][4]);   // sum_i A[i][k] * dC[i][j] = 4 * 1
 }
 
 int main(int argc, char** argv)

@@ -46,6 +46,18 @@ def main():
     }
     with open(os.path.join(here, "reference_kats.json"), "w") as f:
         json.dump(kats, f, indent=1)
+    # batched configuration: 4096 segment lengths floor(10^(3+3d)),
+    # d = (z>>11)*2^-53 (seed 0x5001); 512 product shapes M,N,K =
+    # 64*(1+U{0..15}), U = ((z>>32)*16)>>32 from seed 0x5003 (3 draws each)
+    import math
+    lens = [int(math.floor(10.0 ** (3.0 + 3.0 * ((splitmix64(0x5001, i) >> 11) * 2.0 ** -53))))
+            for i in range(4096)]
+    u = lambda i: ((splitmix64(0x5003, i) >> 32) * 16) >> 32
+    shapes = [[64 * (1 + u(3 * i)), 64 * (1 + u(3 * i + 1)), 64 * (1 + u(3 * i + 2))]
+              for i in range(512)]
+    with open(os.path.join(here, "batched_shapes.json"), "w") as f:
+        json.dump({"segment_lengths": lens, "gemm_shapes": shapes,
+                   "note": "gemm_shapes are [M, N, K]"}, f)
 
 
 if __name__ == "__main__":

@@ -110,3 +110,14 @@ def test_hip_extension_required():
         _lib.LIB_PATH = saved
         _lib._lib = old
         importlib.invalidate_caches()
+
+
+def test_cpp_api_headers_compile():
+    """The drop-in C++ headers and the restated reference test suite compile
+    with g++ (host only)."""
+    src = os.path.join(ROOT, "tests", "cpp", "skeleton_tests.cpp")
+    r = subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-Wall", "-Werror",
+                        "-I", os.path.join(ROOT, "include"),
+                        "-I", os.path.join(ROOT, "skelcl_amd", "include"), src],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr[-4000:]

@@ -1,0 +1,219 @@
+"""Adjoints, batched segmented Reduce/Scan, batched small products, fused
+Zip->Reduce under graph capture (north-star extensions, no reference
+counterpart).  Adjoints are checked against FP64 central finite differences
+and FP64 reference gradients."""
+import ctypes as C
+import json
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def test_map_backward_vs_fd(skl):
+    src = "float func(float x, float a){ return a*x*x + sinf(x); }"
+    dsrc = "float dfunc(float x, float a){ return 2.0f*a*x + cosf(x); }"
+    m = skl.Map("float(float)", src, deriv_source=dsrc, deriv_func="dfunc")
+    x = oracle.gen_f32_sym(1, 4097)
+    g = oracle.gen_f32_sym(2, 4097)
+    d = np.asarray(m.backward(skl.Vector(x), skl.Vector(g), np.float32(1.5))).astype(np.float64)
+    xd = x.astype(np.float64)
+    ref = g * (3.0 * xd + np.cos(xd))
+    assert np.max(np.abs(d - ref)) < 1e-5
+    # FP64 central finite differences of loss = sum(g * f(x)) at a few points
+    for i in (0, 100, 4096):
+        h = 1e-4
+        f = lambda v: 1.5 * v * v + np.sin(v)
+        fd = g[i] * (f(xd[i] + h) - f(xd[i] - h)) / (2 * h)
+        assert abs(fd - d[i]) < 1e-4
+
+
+def test_zip_backward(skl):
+    src = "float func(float x, float y){ return x*y + x; }"
+    dsrc = ("float dl(float x, float y){ return y + 1.0f; }\n"
+            "float dr(float x, float y){ return x; }")
+    z = skl.Zip("float(float, float)", src, deriv_source=dsrc, deriv_func="dl", deriv_func2="dr")
+    x, y, g = (oracle.gen_f32_sym(s, 1000) for s in (3, 4, 5))
+    dl, dr = z.backward(skl.Vector(x), skl.Vector(y), skl.Vector(g))
+    assert np.allclose(np.asarray(dl), g * (y + 1), atol=1e-6)
+    assert np.allclose(np.asarray(dr), g * x, atol=1e-6)
+
+
+def test_reduce_backward_broadcast(skl):
+    r = skl.Reduce("float(float)", "float func(float x, float y){ return x+y; }")
+    d = np.asarray(r.backward(skl.Vector(np.array([0.75], np.float32)), 12345))
+    assert d.shape == (12345,) and np.all(d == np.float32(0.75))
+
+
+@pytest.mark.parametrize("n", [1, 1000, 4097, 300001])
+@pytest.mark.parametrize("inclusive", [False, True])
+def test_scan_backward_reversed_scan(skl, n, inclusive):
+    s = skl.Scan("float(float)", "float func(float x, float y){ return x+y; }", "0.0f")
+    g = oracle.gen_f32_sym(0x5005, n)
+    d = np.asarray(s.backward(skl.Vector(g), inclusive=inclusive)).astype(np.float64)
+    ref = oracle.scan_add_adjoint_f64(g, inclusive)
+    tol = np.spacing(np.abs(ref).astype(np.float32)).astype(np.float64) + 1e-9
+    assert np.all(np.abs(d - ref) <= tol)
+
+
+GAUSS = """float func(input_matrix_t img, __global float* w) {
+  float sum = 0.0f;
+  for (int dy = -5; dy <= 5; ++dy)
+    for (int dx = -5; dx <= 5; ++dx)
+      sum += getData(img, dx, dy) * w[(dy + 5) * 11 + (dx + 5)];
+  return sum;
+}"""
+
+
+@pytest.mark.parametrize("padding", [0, 1])
+def test_mapoverlap_stencil_backward(skl, padding):
+    """loss = sum(W * out): d loss / d img is the transposed stencil of W;
+    checked against the FP64 adjoint and FP64 central FD on 64 pixels."""
+    rows, cols = 200, 333
+    w = oracle.gaussian_weights(5, 2.0)
+    img = oracle.gen_f32_unit(0x3001, rows * cols, 255.0).reshape(rows, cols)
+    W = oracle.gen_f32_sym(0x3002, rows * cols).reshape(rows, cols)
+    m = skl.MapOverlap("float(float)", GAUSS, 5, padding, 0)
+    wv = skl.Vector(w.ravel())
+    d = np.asarray(m.stencil_backward(skl.Matrix(W), wv)).astype(np.float64)
+    rng = np.random.default_rng(0x3003)
+    pix = np.unique(np.concatenate([rng.integers(0, rows * cols, 60),
+                                    [0, cols - 1, (rows - 1) * cols, rows * cols - 1]])
+                    ).astype(np.int64)
+    ref = oracle.mapoverlap_stencil_adjoint_f64(W, w, 5, pix, padding)
+    assert np.max(np.abs(d.ravel()[pix] - ref)) < 1e-5
+    # central finite differences of the FP64 loss (linear: exact up to rounding)
+    for p in pix[:8]:
+        r, c = divmod(int(p), cols)
+        win = np.arange(max(0, r - 6) * cols, min(rows, r + 7) * cols, dtype=np.int64)
+        h = 0.5
+        ip, im = img.astype(np.float64).copy(), img.astype(np.float64).copy()
+        ip[r, c] += h
+        im[r, c] -= h
+        lp = np.dot(W.ravel()[win], oracle.mapoverlap_stencil_f64(ip.astype(np.float32), w, 5, win,
+                                                                  padding))
+        lm = np.dot(W.ravel()[win], oracle.mapoverlap_stencil_f64(im.astype(np.float32), w, 5, win,
+                                                                  padding))
+        fd = (lp - lm) / (2 * h)
+        assert abs(fd - d[r, c]) < 1e-3 * max(1.0, abs(fd))
+
+
+def test_allpairs_backward(skl):
+    zip_ = skl.Zip("float(float, float)", "float func(float x, float y){ return x*y; }")
+    red = skl.Reduce("float(float)", "float func(float x, float y){ return x+y; }")
+    ap = skl.AllPairs("float(float, float)", reduce=red, zip=zip_)
+    M, K, N = 96, 130, 70
+    A = oracle.gen_f32_sym(0x4001, M * K).reshape(M, K)
+    B = oracle.gen_f32_sym(0x4002, K * N).reshape(K, N)
+    dC = oracle.gen_f32_sym(0x4004, M * N).reshape(M, N)
+    for prec in (skl.PREC_FP32, skl.PREC_BF16X3):
+        ap.set_precision(prec)
+        dA, dB = ap.backward(skl.Matrix(A), skl.Matrix(B), skl.Matrix(dC))
+        rA = dC.astype(np.float64) @ B.T.astype(np.float64)
+        rB = A.T.astype(np.float64) @ dC.astype(np.float64)
+        assert np.linalg.norm(np.asarray(dA) - rA) / np.linalg.norm(rA) < 1e-6
+        assert np.linalg.norm(np.asarray(dB) - rB) / np.linalg.norm(rB) < 1e-6
+
+
+# ------------------------------------------------------------- batched
+def _segments(n_seg, seed, lo=3.0, span=3.0):
+    lens = oracle.gen_loguniform(seed, n_seg, lo, span)
+    off = np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)
+    return lens, off
+
+
+def test_segmented_reduce_small(skl):
+    lens = np.array([0, 1, 5, 4096, 4097, 70000, 0, 3], np.int64)
+    off = np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)
+    x = oracle.gen_f32_sym(0x5002, int(off[-1]))
+    r = skl.Reduce("float(float)", "float func(float x, float y){ return x+y; }")
+    got = np.asarray(r.segmented(skl.Vector(x), off)).astype(np.float64)
+    ref = oracle.segmented_sum_f64(x, off)
+    assert np.all(np.abs(got - ref) <= np.spacing(np.abs(ref).astype(np.float32)) + 1e-12)
+
+
+def test_segmented_reduce_scan_cfg5_subset(skl):
+    """The batched configuration's segment lengths (log-uniform [1e3, 1e6),
+    committed fixture), first 256 segments."""
+    with open(os.path.join(os.path.dirname(__file__), "golden", "batched_shapes.json")) as f:
+        fx = json.load(f)
+    lens = np.array(fx["segment_lengths"][:256], np.int64)
+    off = np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)
+    x = oracle.gen_f32_sym(0x5002, int(off[-1]))
+    X = skl.Vector(x)
+    r = skl.Reduce("float(float)", "float func(float x, float y){ return x+y; }")
+    got = np.asarray(r.segmented(X, off)).astype(np.float64)
+    ref = oracle.segmented_sum_f64(x, off)
+    assert np.all(np.abs(got - ref) <= np.spacing(np.abs(ref).astype(np.float32)) + 1e-12)
+    s = skl.Scan("float(float)", "float func(float x, float y){ return x+y; }", "0.0f")
+    for incl in (False, True):
+        got = np.asarray(s.segmented(X, off, inclusive=incl)).astype(np.float64)
+        ref = oracle.segmented_scan_f64(x, off, incl)
+        tol = np.spacing(np.abs(ref).astype(np.float32)).astype(np.float64) + 1e-9
+        assert np.all(np.abs(got - ref) <= tol), incl
+
+
+@pytest.mark.parametrize("prec", [1, 2])
+def test_batched_matmul(skl, prec):
+    from skelcl_amd import _lib
+    with open(os.path.join(os.path.dirname(__file__), "golden", "batched_shapes.json")) as f:
+        shapes = json.load(f)["gemm_shapes"][:40]
+    descs = (_lib.skl_gemm_desc * len(shapes))()
+    keep, refs = [], []
+    for i, (M, N, K) in enumerate(shapes):
+        A = oracle.gen_f32_sym(0x5004 + 3 * i, M * K).reshape(M, K)
+        B = oracle.gen_f32_sym(0x5004 + 3 * i + 1, K * N).reshape(K, N)
+        dA, dB = skl.DeviceBuffer.from_array(A), skl.DeviceBuffer.from_array(B)
+        dC = skl.DeviceBuffer(4 * M * N)
+        keep += [dA, dB, dC]
+        descs[i].A, descs[i].B, descs[i].C = dA.value, dB.value, dC.value
+        descs[i].M, descs[i].N, descs[i].K = M, N, K
+        refs.append((dC, M, N, A.astype(np.float64) @ B.astype(np.float64)))
+    dd = skl.DeviceBuffer(C.sizeof(descs))
+    _lib.check(_lib.lib().skl_memcpy_h2d(dd.ptr, 0, C.cast(descs, C.c_void_p), C.sizeof(descs),
+                                         None, 1))
+    _lib.check(_lib.lib().skl_matmul_batched_f32(dd.ptr, len(shapes), prec, None))
+    for dC, M, N, ref in refs:
+        c = dC.to_numpy(np.float32, M * N).reshape(M, N)
+        assert np.linalg.norm(c - ref) / np.linalg.norm(ref) < 1e-6
+
+
+def test_graph_capture_zip_reduce(skl):
+    """Zip->Reduce fused, captured into a graph and replayed: every replay
+    equals the eager result (the reduce counter re-arms itself, no memset
+    nodes are recorded)."""
+    from skelcl_amd import _lib
+    L = _lib.lib()
+    n = (1 << 20) + 3
+    x, y = oracle.gen_f32_sym(0x1001, n), oracle.gen_f32_sym(0x1002, n)
+    z = skl.Zip("float(float, float)", "float func(float x, float y){ return x*y; }")
+    r = skl.Reduce("float(float)", "float func(float x, float y){ return x+y; }")
+    X, Y = skl.Vector(x), skl.Vector(y)
+    st = skl.Stream()
+    out = skl.DeviceBuffer(4)
+    # eager (also allocates the reduce scratch outside the capture)
+    _lib.check(L.skl_zip_reduce_run(z.handle, r.handle, out.ptr, X.device_ptr(), Y.device_ptr(),
+                                    n, st.handle))
+    st.synchronize()
+    eager = out.to_numpy(np.float32, 1)[0]
+    g = C.c_void_p()
+    _lib.check(L.skl_graph_begin_capture(st.handle))
+    _lib.check(L.skl_zip_reduce_run(z.handle, r.handle, out.ptr, X.device_ptr(), Y.device_ptr(),
+                                    n, st.handle))
+    _lib.check(L.skl_graph_end_capture(st.handle, C.byref(g)))
+    for k in range(5):
+        out.fill(b"\x00\x00\xc0\x7f", 4, 1, st)  # NaN between replays
+        _lib.check(L.skl_graph_launch(g, st.handle))
+        st.synchronize()
+        assert out.to_numpy(np.float32, 1)[0] == eager, k
+        # eager work between replays
+        _lib.check(L.skl_zip_reduce_run(z.handle, r.handle, out.ptr, X.device_ptr(),
+                                        Y.device_ptr(), n, st.handle))
+        st.synchronize()
+    _lib.check(L.skl_graph_destroy(g))
+    ref = oracle.dot_f64(x, y)
+    assert abs(float(eager) - ref) <= 2 * np.spacing(np.float32(abs(ref))) + 1e-12 * n
