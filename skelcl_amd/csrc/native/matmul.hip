@@ -22,6 +22,7 @@
 #include <cstdint>
 
 #include "skelcl_skeletons.h"
+#include "work_ticket.h"
 
 namespace {
 
@@ -579,15 +580,13 @@ __global__ __launch_bounds__(256) void batch_gemm_kernel(const skl_gemm_desc* __
   __shared__ __attribute__((aligned(16))) uint16_t Bh[3][T][KS + 8];
   __shared__ long long s_t;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const unsigned long long total = ctl->total;
-  while (true) {
-    if (tid == 0)
-      s_t = (long long)__hip_atomic_fetch_add(&ctl->work, 1ull, __ATOMIC_RELAXED,
-                                              __HIP_MEMORY_SCOPE_AGENT);
+  const long long total = (long long)ctl->total;
+  if (tid == 0) s_t = ticket_draw(&ctl->work);
+  __syncthreads();
+  for (;;) {
+    const long long t = ticket_read(s_t);
     __syncthreads();
-    const long long t = s_t;
-    __syncthreads();
-    if (t >= (long long)total) break;
+    if (t >= total) break;
     const int p = find_item(tstart, count, t);
     const skl_gemm_desc g = d[p];
     const long long lt = t - tstart[p];
@@ -699,6 +698,8 @@ __global__ __launch_bounds__(256) void batch_gemm_kernel(const skl_gemm_desc* __
           }
         }
     }
+    if (tid == 0) s_t = ticket_draw(&ctl->work);
+    __syncthreads();
   }
 }
 

@@ -21,6 +21,7 @@
 #include <cstdint>
 
 #include "skelcl_rt.h"
+#include "work_ticket.h"
 
 namespace {
 
@@ -140,15 +141,13 @@ __global__ __launch_bounds__(WG) void seg_sum_kernel(const float* __restrict__ i
 {
   __shared__ long long s_c;
   __shared__ double red[WG / 64];
-  const unsigned long long total = ctl->total;
-  while (true) {
-    if (threadIdx.x == 0)
-      s_c = (long long)__hip_atomic_fetch_add(&ctl->work, 1ull, __ATOMIC_RELAXED,
-                                              __HIP_MEMORY_SCOPE_AGENT);
+  const long long total = (long long)ctl->total;
+  if (threadIdx.x == 0) s_c = ticket_draw(&ctl->work);
+  __syncthreads();
+  for (;;) {
+    const long long c = ticket_read(s_c);
     __syncthreads();
-    const long long c = s_c;
-    __syncthreads();
-    if (c >= (long long)total) break;
+    if (c >= total) break;
     const int s = find_segment(cstart, nseg, c);
     const long long k = c - cstart[s];
     const long long b0 = off[s], len = off[s + 1] - b0;
@@ -178,7 +177,9 @@ __global__ __launch_bounds__(WG) void seg_sum_kernel(const float* __restrict__ i
                                    __HIP_MEMORY_SCOPE_AGENT);
         out[s] = (float)sum;
       }
+      s_c = ticket_draw(&ctl->work);
     }
+    __syncthreads();
   }
 }
 
@@ -194,16 +195,14 @@ __global__ __launch_bounds__(WG) void seg_scan_kernel(const float* __restrict__ 
   __shared__ double red[WG / 64];
   __shared__ double s_thr[WG];
   __shared__ double s_carry;
-  const unsigned long long total = ctl->total;
+  const long long total = (long long)ctl->total;
   const unsigned epoch = ctl->epoch;
-  while (true) {
-    if (threadIdx.x == 0)
-      s_c = (long long)__hip_atomic_fetch_add(&ctl->work, 1ull, __ATOMIC_RELAXED,
-                                              __HIP_MEMORY_SCOPE_AGENT);
+  if (threadIdx.x == 0) s_c = ticket_draw(&ctl->work);
+  __syncthreads();
+  for (;;) {
+    const long long c = ticket_read(s_c);
     __syncthreads();
-    const long long c = s_c;
-    __syncthreads();
-    if (c >= (long long)total) break;
+    if (c >= total) break;
     const int s = find_segment(cstart, nseg, c);
     const long long k = c - cstart[s];
     const long long b0 = off[s], len = off[s + 1] - b0;
@@ -267,6 +266,7 @@ __global__ __launch_bounds__(WG) void seg_scan_kernel(const float* __restrict__ 
     }
     __syncthreads();
     for (int i = threadIdx.x; i < m; i += WG) out[lo + i] = tile[i];
+    if (threadIdx.x == 0) s_c = ticket_draw(&ctl->work);
     __syncthreads();
   }
 }

@@ -217,3 +217,26 @@ def test_graph_capture_zip_reduce(skl):
     _lib.check(L.skl_graph_destroy(g))
     ref = oracle.dot_f64(x, y)
     assert abs(float(eager) - ref) <= 2 * np.spacing(np.float32(abs(ref))) + 1e-12 * n
+
+
+def test_autotune_keeps_results_and_caches(skl, tmp_path, monkeypatch):
+    """The autotuner changes only launch shapes: results stay bit-identical,
+    and the chosen shape is cached and reused."""
+    import ctypes as C
+    from skelcl_amd import _lib
+    monkeypatch.setenv("SKELCL_CACHE_DIR", str(tmp_path))
+    L = _lib.lib()
+    s = skl.Scan("int(int)", "int func(int x, int y){ return x+y; }")
+    x = oracle.gen_i32_below(77, 1 << 20, 1000)
+    before = np.asarray(s(skl.Vector(x), inclusive=True))
+    _lib.check(L.skl_autotune(s.handle, 1 << 20, 1))
+    shape = (C.c_int * 4)()
+    _lib.check(L.skl_skeleton_launch_shape(s.handle, shape))
+    assert shape[2] in (2, 4, 8, 16)
+    after = np.asarray(s(skl.Vector(x), inclusive=True))
+    assert np.array_equal(before, after)
+    m = skl.Zip("float(float, float)", "float func(float x, float y){ return x*y; }")
+    _lib.check(L.skl_autotune(m.handle, 1 << 22, 1))
+    a, b = oracle.gen_f32_sym(1, 12345), oracle.gen_f32_sym(2, 12345)
+    assert np.array_equal(np.asarray(m(skl.Vector(a), skl.Vector(b))), a * b)
+    _lib.check(L.skl_autotune(m.handle, 0, 0))  # back to defaults
