@@ -262,9 +262,9 @@ def run_extras(skelcl, L, ck, stream, args):
     sums = skelcl.DeviceBuffer(4 * len(lens))
     scan = skelcl.Scan("float(float)", "float func(float x, float y){ return x+y; }", "0.0f")
     t_sr = _time(stream, lambda: ck(L.skl_reduce_segmented_run(add.handle, sums.ptr, segx.ptr,
-                                                               offb.ptr, len(lens), st)), reps=3)
+                                                               tot, offb.ptr, len(lens), st)), reps=3)
     t_ss = _time(stream, lambda: ck(L.skl_scan_segmented_run(scan.handle, segy.ptr, segx.ptr,
-                                                             offb.ptr, len(lens), 1, st)), reps=3)
+                                                             tot, offb.ptr, len(lens), 1, st)), reps=3)
     t_sb = _time(stream, lambda: ck(L.skl_scan_backward(scan.handle, segy.ptr, segx.ptr, tot, 1,
                                                         st)), reps=3)
     shapes = fx["gemm_shapes"]

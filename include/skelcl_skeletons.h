@@ -187,14 +187,18 @@ int skl_zip_reduce_run(skl_skeleton zip, skl_skeleton reduce, void* out,
                        const void* left, const void* right, size_t n,
                        skl_stream stream);
 
-/* Batched segmented Reduce / Scan: nseg segments described by device
- * offsets[0..nseg] (int64, offsets[s]..offsets[s+1]) processed in ONE
- * persistent launch.  Reduce writes out[s]; Scan writes out[offsets[s]+k].
- * An empty segment reduces to the identity. */
-int skl_reduce_segmented_run(skl_skeleton s, void* out, const void* in,
+/* Batched segmented Reduce / Scan (float (+) only): nseg segments of the n
+ * elements of `in`, described by device offsets[0..nseg] (int64, segment s
+ * is offsets[s]..offsets[s+1]), all processed by one launch after a
+ * one-work-group prologue.  Reduce writes out[s] (nseg floats; an empty
+ * segment gives 0); Scan writes out[offsets[s] + k] (n floats).  Offsets are
+ * validated on the device (non-decreasing, inside [0, n]); invalid offsets
+ * make Reduce write NaN to every out[s] and Scan write nothing.  No
+ * counterpart in the reference. */
+int skl_reduce_segmented_run(skl_skeleton s, void* out, const void* in, size_t n,
                              const int64_t* offsets, int nseg,
                              skl_stream stream);
-int skl_scan_segmented_run(skl_skeleton s, void* out, const void* in,
+int skl_scan_segmented_run(skl_skeleton s, void* out, const void* in, size_t n,
                            const int64_t* offsets, int nseg, int inclusive,
                            skl_stream stream);
 

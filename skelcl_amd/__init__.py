@@ -364,6 +364,22 @@ class IndexMatrix:
         self.shape = (int(shape[0]), int(shape[1]))
 
 
+def _segment_offsets(offsets, n: int) -> "Vector":
+    """Segment offsets as a device Vector of int64.  Host offsets are checked
+    here (the device check only flags them); a Vector is passed as it is."""
+    if isinstance(offsets, Vector):
+        if len(offsets) < 1:
+            raise SkelError(_lib.SKL_ERR_INVALID, "offsets need at least one entry")
+        return offsets
+    off = np.ascontiguousarray(offsets, dtype=np.int64).ravel()
+    if off.size < 1:
+        raise SkelError(_lib.SKL_ERR_INVALID, "offsets need at least one entry")
+    if off[0] < 0 or off[-1] > n or np.any(np.diff(off) < 0):
+        raise SkelError(_lib.SKL_ERR_INVALID,
+                        f"offsets must be non-decreasing and inside [0, {n}]")
+    return Vector(off)
+
+
 def _etype_for_array(host: np.ndarray, hint: Optional[str]) -> str:
     """Element type of a host array: the given name when its layout matches
     (a vector type such as float4 is an array whose last axis holds the
@@ -680,11 +696,11 @@ class Reduce(_Skeleton):
     def segmented(self, inp, offsets, stream=None):
         """Batched segmented Reduce(+) (extension): out[s] = sum of
         inp[offsets[s]:offsets[s+1]], all segments in one persistent launch."""
-        off = offsets if isinstance(offsets, Vector) else Vector(np.asarray(offsets, np.int64))
+        off = _segment_offsets(offsets, len(inp))
         nseg = len(off) - 1
         o = Vector(size=nseg, etype=self.types[0])
         check(lib().skl_reduce_segmented_run(self.handle, o.device_buffer_for_output(),
-                                             inp.device_ptr(), off.device_ptr(), nseg,
+                                             inp.device_ptr(), len(inp), off.device_ptr(), nseg,
                                              _stream(stream)))
         o.data_on_device_modified()
         return o
@@ -725,11 +741,11 @@ class Scan(_Skeleton):
     def segmented(self, inp, offsets, inclusive=False, stream=None):
         """Batched segmented Scan(+) (extension): every segment scanned on its
         own, all segments in one persistent launch."""
-        off = offsets if isinstance(offsets, Vector) else Vector(np.asarray(offsets, np.int64))
+        off = _segment_offsets(offsets, len(inp))
         nseg = len(off) - 1
         o = Vector(size=len(inp), etype=self.types[0])
         check(lib().skl_scan_segmented_run(self.handle, o.device_buffer_for_output(),
-                                           inp.device_ptr(), off.device_ptr(), nseg,
+                                           inp.device_ptr(), len(inp), off.device_ptr(), nseg,
                                            int(bool(inclusive)), _stream(stream)))
         o.data_on_device_modified()
         return o

@@ -130,8 +130,8 @@ SIGNATURES = {
     "skl_mapoverlap_run": (_i, [_vp, _vp, _vp, _sz, _sz, _argp, _i, _vp]),
     "skl_allpairs_run": (_i, [_vp, _vp, _vp, _vp, _sz, _sz, _sz, _argp, _i, _vp]),
     "skl_zip_reduce_run": (_i, [_vp, _vp, _vp, _vp, _vp, _sz, _vp]),
-    "skl_reduce_segmented_run": (_i, [_vp, _vp, _vp, _vp, _i, _vp]),
-    "skl_scan_segmented_run": (_i, [_vp, _vp, _vp, _vp, _i, _i, _vp]),
+    "skl_reduce_segmented_run": (_i, [_vp, _vp, _vp, _sz, _vp, _i, _vp]),
+    "skl_scan_segmented_run": (_i, [_vp, _vp, _vp, _sz, _vp, _i, _i, _vp]),
     "skl_map_backward": (_i, [_vp, _vp, _vp, _vp, _sz, _argp, _i, _vp]),
     "skl_zip_backward": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _sz, _argp, _i, _vp]),
     "skl_reduce_backward": (_i, [_vp, _vp, _vp, _sz, _vp]),

@@ -18,12 +18,13 @@ extern "C" int skl_internal_matmul(float* C, const float* A, const float* B, int
 extern "C" int skl_internal_stencil_transpose(float* din, const float* dout, const float* w,
                                               int rows, int cols, int R, int nearest,
                                               hipStream_t s);
-extern "C" int skl_internal_segmented_sum(float* out, const float* in, const int64_t* offsets,
-                                          int nseg, void* ws, hipStream_t s);
-extern "C" int skl_internal_segmented_scan(float* out, const float* in,
+extern "C" int skl_internal_segmented_sum(float* out, const float* in, unsigned long long n,
+                                          const int64_t* offsets, int nseg, void* ws,
+                                          hipStream_t s);
+extern "C" int skl_internal_segmented_scan(float* out, const float* in, unsigned long long n,
                                            const int64_t* offsets, int nseg, int inclusive,
-                                           void* ws, size_t ws_bytes, hipStream_t s);
-extern "C" size_t skl_internal_segmented_scan_workspace(int nseg);
+                                           void* ws, hipStream_t s);
+extern "C" size_t skl_internal_segmented_workspace(unsigned long long n, int nseg);
 extern "C" int skl_internal_broadcast(void* dst, const void* src, size_t elem, size_t count,
                                       hipStream_t s);
 extern "C" size_t skl_internal_batched_workspace(int count);
@@ -190,21 +191,22 @@ extern "C" int skl_mapoverlap_stencil_backward(skl_skeleton s, void* d_in, const
                                         resolve_stream(stream));
 }
 
-extern "C" int skl_reduce_segmented_run(skl_skeleton s, void* out, const void* in,
+extern "C" int skl_reduce_segmented_run(skl_skeleton s, void* out, const void* in, size_t n,
                                         const int64_t* offsets, int nseg, skl_stream stream)
 {
   SKL_TRY(ensure_initialized());
   if (!s || s->kind != SKL_REDUCE) return fail(SKL_ERR_INVALID, "not a Reduce skeleton");
   if (!(s->op == UserOp::Add && s->extras.empty() && s->types[0].name == "float"))
     return fail(SKL_ERR_UNSUPPORTED, "segmented Reduce is available for float Reduce(+)");
-  if (nseg <= 0) return SKL_OK;
-  if (!out || !in || !offsets) return fail(SKL_ERR_INVALID, "NULL buffer");
-  SKL_TRY(seg_scratch(s, skl_internal_segmented_scan_workspace(nseg)));
-  return skl_internal_segmented_sum((float*)out, (const float*)in, offsets, nseg,
+  if (nseg < 0) return fail(SKL_ERR_INVALID, "negative segment count");
+  if (nseg == 0) return SKL_OK;
+  if (!out || !offsets || (n && !in)) return fail(SKL_ERR_INVALID, "NULL buffer");
+  SKL_TRY(seg_scratch(s, skl_internal_segmented_workspace(n, nseg)));
+  return skl_internal_segmented_sum((float*)out, (const float*)in, n, offsets, nseg,
                                     s->seg_scratch, resolve_stream(stream));
 }
 
-extern "C" int skl_scan_segmented_run(skl_skeleton s, void* out, const void* in,
+extern "C" int skl_scan_segmented_run(skl_skeleton s, void* out, const void* in, size_t n,
                                       const int64_t* offsets, int nseg, int inclusive,
                                       skl_stream stream)
 {
@@ -212,12 +214,12 @@ extern "C" int skl_scan_segmented_run(skl_skeleton s, void* out, const void* in,
   if (!s || s->kind != SKL_SCAN) return fail(SKL_ERR_INVALID, "not a Scan skeleton");
   if (!(s->op == UserOp::Add && s->extras.empty() && s->types[0].name == "float"))
     return fail(SKL_ERR_UNSUPPORTED, "segmented Scan is available for float Scan(+)");
-  if (nseg <= 0) return SKL_OK;
+  if (nseg < 0) return fail(SKL_ERR_INVALID, "negative segment count");
+  if (nseg == 0 || n == 0) return SKL_OK;
   if (!out || !in || !offsets) return fail(SKL_ERR_INVALID, "NULL buffer");
-  size_t wsb = skl_internal_segmented_scan_workspace(nseg);
-  SKL_TRY(seg_scratch(s, wsb));
-  return skl_internal_segmented_scan((float*)out, (const float*)in, offsets, nseg, inclusive,
-                                     s->seg_scratch, wsb, resolve_stream(stream));
+  SKL_TRY(seg_scratch(s, skl_internal_segmented_workspace(n, nseg)));
+  return skl_internal_segmented_scan((float*)out, (const float*)in, n, offsets, nseg,
+                                     inclusive, s->seg_scratch, resolve_stream(stream));
 }
 
 extern "C" int skl_matmul_batched_f32(const skl_gemm_desc* descs_device, int count,

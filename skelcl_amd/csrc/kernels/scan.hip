@@ -85,19 +85,21 @@ extern "C" __global__ __launch_bounds__(SKL_WG) void skl_scan(
   const int lane = threadIdx.x & 63;
   const int wv = threadIdx.x >> 6;
 
-  // ctl[0]: tile ticket, ctl[1]: launch epoch (tags the look-back words, so
-  // words left by earlier launches never match).  The work-group drawing the
-  // last ticket re-arms both for the next launch; every other work-group has
-  // already read the epoch by then (the read precedes its acq_rel ticket).
+  // ctl[0] = launch epoch << 32 | tile ticket.  One relaxed RMW hands out the
+  // tile and the epoch together (the epoch tags the look-back words, so words
+  // left by earlier launches never match); the work-group drawing the last
+  // ticket re-arms the word for the next launch, after every other
+  // work-group's draw.  Nothing else needs ordering: every look-back value
+  // travels inside its tagged status word.  (An acquire-release RMW here
+  // costs an L2 write-back + invalidate per tile on a multi-XCD part.)
   if (threadIdx.x == 0) {
-    unsigned ep = (unsigned)__hip_atomic_load(&ctl[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    unsigned long long t =
-        __hip_atomic_fetch_add(&ctl[0], 1ull, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-    if (t == (unsigned long long)gridDim.x - 1) {
-      __hip_atomic_store(&ctl[0], 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      __hip_atomic_store(&ctl[1], (unsigned long long)((ep + 1u) & 0x3FFFFFFFu),
+    const unsigned long long w =
+        __hip_atomic_fetch_add(&ctl[0], 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const unsigned long long t = w & 0xFFFFFFFFull;
+    const unsigned ep = (unsigned)(w >> 32);
+    if (t == (unsigned long long)gridDim.x - 1)
+      __hip_atomic_store(&ctl[0], (unsigned long long)((ep + 1u) & 0x3FFFFFFFu) << 32,
                          __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
     s_tile = t;
     s_epoch = ep;
   }

@@ -1,21 +1,24 @@
 // Batched segmented Reduce(+) / Scan(+) on float (north-star extension):
-// many uneven segments, given by device offsets, in one persistent launch
-// after a one-work-group prologue.
+// many uneven segments, given by device offsets, in one launch after a
+// one-work-group prologue.
 //
-// The prologue cuts segment s (length L) into chunks of CH(s) =
-// max(4096, roundup(ceil(L / 1024), 1024)) elements (at most 1024 chunks,
-// CH <= 16384), writes the chunk prefix sums and re-arms the work counter;
-// empty segments get the identity.  The persistent work-groups then take
-// chunks from a counter (global chunk order), so a chunk only ever waits on
-// chunks handed out earlier.
-//   Reduce: each chunk's compensated float sum is stored as an FP64
-//           partial; the last chunk of a segment to finish (per-segment
-//           counter) adds the segment's partials in chunk order (FP64) and
-//           writes out[s].
-//   Scan:   a chunk is staged in LDS (64 KB max), its total published with
-//           the launch epoch; it then adds the published totals of the
-//           segment's earlier chunks in order (FP64 carry) and writes
-//           carry + local prefix.
+// The prologue validates the offsets (non-decreasing, inside [0, n]), cuts
+// segment s (length L) into ceil(L / CH) chunks and writes the chunk prefix
+// counts cstart[] (global chunk ids; the chunks of a segment are
+// consecutive).
+//   Reduce (CH = 16384): a static grid-stride loop over chunks -- chunks do
+//     not wait on each other -- stores each chunk's sum as an FP64 partial
+//     (compensated float accumulation, FP64 work-group combine); a second
+//     kernel, one wave per segment, adds the segment's partials in a fixed
+//     order and writes out[s].  No ticket, no acquire/release.
+//   Scan (CH = 4096 = one register tile of 256 lanes x 16 elements):
+//     chunks are handed out by a relaxed ticket, so a chunk only waits on
+//     chunks handed out earlier; tile-local prefixes in FP64; chunk
+//     aggregates and inclusive prefixes are published in epoch-tagged
+//     status words (AGG / INCL, 32 bits of the FP64 value per word) and
+//     wave 0 looks back over windows of 64 predecessors within the segment.
+//     Every output is float(carry + local prefix), all in FP64.
+// Invalid offsets set an error flag: no chunk runs, Reduce writes NaN.
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -25,37 +28,30 @@
 
 namespace {
 
-constexpr int MAXC = 1024;
-constexpr long long MINCH = 4096, MAXCH = 16384;
 constexpr int WG = 256;
+constexpr int ROWS = 16;                              // scan rows of 64 per wave
+constexpr long long SCAN_CH = (long long)WG * ROWS;  // 4096 elements per scan chunk
+constexpr long long SUM_CH = 16384;                  // elements per reduce chunk
+constexpr unsigned ST_AGG = 1u, ST_INCL = 2u;
 
 struct SegCtl {
-  unsigned long long work;   // chunk ticket
-  unsigned long long total;  // number of chunks
-  unsigned int epoch;
-  unsigned int error;
+  unsigned long long work;   // scan: chunk ticket
+  unsigned long long total;  // number of chunks (0 after an error)
+  unsigned int epoch;        // scan: tags this launch's status words
+  unsigned int error;        // 1: negative length, 2: offsets outside [0, n]
 };
-
-__device__ __forceinline__ long long chunk_size(long long len)
-{
-  long long ch = (len + MAXC - 1) / MAXC;
-  ch = (ch + 1023) / 1024 * 1024;
-  return ch < MINCH ? MINCH : ch;
-}
 
 // one work-group of 1024 threads
 __global__ __launch_bounds__(1024) void seg_prologue(const int64_t* __restrict__ off, int nseg,
-                                                     long long* __restrict__ cstart,
-                                                     unsigned* __restrict__ done,
-                                                     float* __restrict__ out_empty,
-                                                     SegCtl* ctl, int is_scan)
+                                                     unsigned long long n, long long ch,
+                                                     long long* __restrict__ cstart, SegCtl* ctl)
 {
   __shared__ long long part[1024];
   __shared__ long long carry;
   __shared__ unsigned err;
   if (threadIdx.x == 0) {
     carry = 0;
-    err = 0;
+    err = (off[0] < 0 || (unsigned long long)off[nseg] > n) ? 2u : 0u;
   }
   __syncthreads();
   for (int base = 0; base < nseg; base += 1024) {
@@ -63,16 +59,8 @@ __global__ __launch_bounds__(1024) void seg_prologue(const int64_t* __restrict__
     long long cnt = 0;
     if (s < nseg) {
       const long long len = off[s + 1] - off[s];
-      if (len < 0) {
-        atomicOr(&err, 1u);
-      } else if (len > 0) {
-        const long long ch = chunk_size(len);
-        if (ch > MAXCH) atomicOr(&err, 2u);
-        cnt = (len + ch - 1) / ch;
-      } else if (!is_scan) {
-        out_empty[s] = 0.0f;  // identity of (+)
-      }
-      done[s] = 0;
+      if (len < 0) atomicOr(&err, 1u);
+      else cnt = (len + ch - 1) / ch;
     }
     part[threadIdx.x] = cnt;
     __syncthreads();
@@ -97,9 +85,10 @@ __global__ __launch_bounds__(1024) void seg_prologue(const int64_t* __restrict__
   }
 }
 
+// segment of global chunk c: cstart[lo] <= c < cstart[lo + 1]
 __device__ __forceinline__ int find_segment(const long long* cstart, int nseg, long long c)
 {
-  int lo = 0, hi = nseg;  // cstart[lo] <= c < cstart[hi]
+  int lo = 0, hi = nseg;
   while (hi - lo > 1) {
     int mid = (lo + hi) >> 1;
     if (cstart[mid] <= c) lo = mid;
@@ -108,7 +97,7 @@ __device__ __forceinline__ int find_segment(const long long* cstart, int nseg, l
   return lo;
 }
 
-// compensated (two-sum) accumulation in float, returned as hi + lo in double
+// compensated (two-sum) accumulation in float
 struct DF {
   float hi, lo;
 };
@@ -135,29 +124,21 @@ __device__ __forceinline__ double block_sum_double(double v, double* red)
 __global__ __launch_bounds__(WG) void seg_sum_kernel(const float* __restrict__ in,
                                                      const int64_t* __restrict__ off, int nseg,
                                                      const long long* __restrict__ cstart,
-                                                     unsigned* __restrict__ done,
                                                      double* __restrict__ partials,
-                                                     float* __restrict__ out, SegCtl* ctl)
+                                                     const SegCtl* __restrict__ ctl)
 {
-  __shared__ long long s_c;
   __shared__ double red[WG / 64];
   const long long total = (long long)ctl->total;
-  if (threadIdx.x == 0) s_c = ticket_draw(&ctl->work);
-  __syncthreads();
-  for (;;) {
-    const long long c = ticket_read(s_c);
-    __syncthreads();
-    if (c >= total) break;
+  for (long long c = blockIdx.x; c < total; c += gridDim.x) {
     const int s = find_segment(cstart, nseg, c);
     const long long k = c - cstart[s];
-    const long long b0 = off[s], len = off[s + 1] - b0;
-    const long long ch = chunk_size(len);
-    const long long lo = b0 + k * ch;
-    const long long hi = lo + ch < b0 + len ? lo + ch : b0 + len;
+    const long long lo = off[s] + k * SUM_CH;
+    const long long end = off[s + 1];
+    const long long hi = lo + SUM_CH < end ? lo + SUM_CH : end;
     DF acc{0.0f, 0.0f};
     long long i = lo + threadIdx.x;
     for (; i + 3 * WG < hi; i += 4 * WG) {
-      float x0 = in[i], x1 = in[i + WG], x2 = in[i + 2 * WG], x3 = in[i + 3 * WG];
+      const float x0 = in[i], x1 = in[i + WG], x2 = in[i + 2 * WG], x3 = in[i + 3 * WG];
       acc = df_add(acc, x0);
       acc = df_add(acc, x1);
       acc = df_add(acc, x2);
@@ -165,22 +146,55 @@ __global__ __launch_bounds__(WG) void seg_sum_kernel(const float* __restrict__ i
     }
     for (; i < hi; i += WG) acc = df_add(acc, in[i]);
     const double tot = block_sum_double((double)acc.hi + (double)acc.lo, red);
-    if (threadIdx.x == 0) {
-      const long long cnt = cstart[s + 1] - cstart[s];
-      partials[(size_t)s * MAXC + k] = tot;
-      unsigned d = __hip_atomic_fetch_add(&done[s], 1u, __ATOMIC_ACQ_REL,
-                                          __HIP_MEMORY_SCOPE_AGENT);
-      if ((long long)d == cnt - 1) {  // last chunk of the segment: ordered FP64 combine
-        double sum = 0.0;
-        for (long long j = 0; j < cnt; ++j)
-          sum += __hip_atomic_load(&partials[(size_t)s * MAXC + j], __ATOMIC_RELAXED,
-                                   __HIP_MEMORY_SCOPE_AGENT);
-        out[s] = (float)sum;
-      }
-      s_c = ticket_draw(&ctl->work);
-    }
-    __syncthreads();
+    if (threadIdx.x == 0) partials[c] = tot;
   }
+}
+
+// one wave per segment: partials in a fixed order (lane-strided FP64 sums,
+// then a fixed shuffle tree)
+__global__ __launch_bounds__(WG) void seg_combine_kernel(const long long* __restrict__ cstart,
+                                                         int nseg,
+                                                         const double* __restrict__ partials,
+                                                         float* __restrict__ out,
+                                                         const SegCtl* __restrict__ ctl)
+{
+  const int s = blockIdx.x * (WG / 64) + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (s >= nseg) return;
+  if (ctl->error) {
+    if (lane == 0) out[s] = __builtin_nanf("");
+    return;
+  }
+  const long long c0 = cstart[s], c1 = cstart[s + 1];
+  double v = 0.0;
+  for (long long c = c0 + lane; c < c1; c += 64) v += partials[c];
+  for (int o = 32; o >= 1; o >>= 1) v += __shfl_down(v, o, 64);
+  if (lane == 0) out[s] = (float)v;
+}
+
+__device__ __forceinline__ void publish(unsigned long long* pub, long long c, unsigned state,
+                                        double v, unsigned epoch)
+{
+  const unsigned long long bits = (unsigned long long)__double_as_longlong(v);
+  const unsigned long long tag = (unsigned long long)((epoch << 2) | state) << 32;
+  __hip_atomic_store(&pub[2 * c], tag | (bits & 0xFFFFFFFFull), __ATOMIC_RELAXED,
+                     __HIP_MEMORY_SCOPE_AGENT);
+  __hip_atomic_store(&pub[2 * c + 1], tag | (bits >> 32), __ATOMIC_RELAXED,
+                     __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// state (AGG / INCL) once both words carry the same tag of this launch, else 0
+__device__ __forceinline__ unsigned read_status(const unsigned long long* pub, long long c,
+                                                unsigned epoch, double* v)
+{
+  const unsigned long long w0 =
+      __hip_atomic_load(&pub[2 * c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  const unsigned long long w1 =
+      __hip_atomic_load(&pub[2 * c + 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  const unsigned t0 = (unsigned)(w0 >> 32), t1 = (unsigned)(w1 >> 32);
+  if (t0 != t1 || (t0 >> 2) != epoch) return 0;
+  *v = __longlong_as_double((long long)((w1 << 32) | (w0 & 0xFFFFFFFFull)));
+  return t0 & 3u;
 }
 
 __global__ __launch_bounds__(WG) void seg_scan_kernel(const float* __restrict__ in,
@@ -190,13 +204,12 @@ __global__ __launch_bounds__(WG) void seg_scan_kernel(const float* __restrict__ 
                                                       float* __restrict__ out, int inclusive,
                                                       SegCtl* ctl)
 {
-  __shared__ float tile[MAXCH];
   __shared__ long long s_c;
-  __shared__ double red[WG / 64];
-  __shared__ double s_thr[WG];
+  __shared__ double s_wtot[WG / 64];
   __shared__ double s_carry;
   const long long total = (long long)ctl->total;
   const unsigned epoch = ctl->epoch;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   if (threadIdx.x == 0) s_c = ticket_draw(&ctl->work);
   __syncthreads();
   for (;;) {
@@ -204,130 +217,168 @@ __global__ __launch_bounds__(WG) void seg_scan_kernel(const float* __restrict__ 
     __syncthreads();
     if (c >= total) break;
     const int s = find_segment(cstart, nseg, c);
-    const long long k = c - cstart[s];
-    const long long b0 = off[s], len = off[s + 1] - b0;
-    const long long ch = chunk_size(len);
-    const long long lo = b0 + k * ch;
-    const int m = (int)((lo + ch < b0 + len ? lo + ch : b0 + len) - lo);
-    for (int i = threadIdx.x; i < m; i += WG) tile[i] = in[lo + i];
-    __syncthreads();
-    // each thread: a contiguous run of per = ceil(m / WG) elements
-    const int per = (m + WG - 1) / WG;
-    const int t0 = threadIdx.x * per;
-    const int t1 = t0 + per < m ? t0 + per : m;
-    DF run{0.0f, 0.0f};
-    for (int i = t0; i < t1; ++i) run = df_add(run, tile[i]);
-    const double mine = (double)run.hi + (double)run.lo;
-    s_thr[threadIdx.x] = mine;
-    const double tot = block_sum_double(mine, red);
-    // publish the chunk total (value bits + epoch tag in two words)
-    if (threadIdx.x == 0) {
-      const unsigned long long bits = __double_as_longlong(tot);
-      unsigned long long* p = pub + ((size_t)s * MAXC + k) * 2;
-      const unsigned long long tag = (unsigned long long)epoch << 32;
-      __hip_atomic_store(p, tag | (bits & 0xFFFFFFFFull), __ATOMIC_RELAXED,
-                         __HIP_MEMORY_SCOPE_AGENT);
-      __hip_atomic_store(p + 1, tag | (bits >> 32), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      // carry: the segment's earlier chunks, in order
-      double carry = 0.0;
-      for (long long j = 0; j < k; ++j) {
-        unsigned long long* q = pub + ((size_t)s * MAXC + j) * 2;
-        unsigned long long w0, w1;
-        do {
-          w0 = __hip_atomic_load(q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-          w1 = __hip_atomic_load(q + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        } while ((unsigned)(w0 >> 32) != epoch || (unsigned)(w1 >> 32) != epoch);
-        carry += __longlong_as_double((long long)((w1 << 32) | (w0 & 0xFFFFFFFFull)));
-      }
-      s_carry = carry;
+    const long long cfirst = cstart[s];
+    const long long k = c - cfirst;
+    const long long lo = off[s] + k * SCAN_CH;
+    const long long end = off[s + 1];
+    const long long hi = lo + SCAN_CH < end ? lo + SCAN_CH : end;
+    const long long wbase = lo + (long long)wv * (64 * ROWS);
+
+    // row j of this wave = 64 consecutive elements, one per lane
+    double x[ROWS], inc[ROWS];
+#pragma unroll
+    for (int j = 0; j < ROWS; ++j) {
+      const long long e = wbase + j * 64 + lane;
+      x[j] = e < hi ? (double)in[e] : 0.0;
     }
-    __syncthreads();
-    // exclusive scan of the per-thread totals (serial in thread 0: 256 adds)
-    if (threadIdx.x == 0) {
-      double acc = s_carry;
-      for (int t = 0; t < WG; ++t) {
-        double v = s_thr[t];
-        s_thr[t] = acc;
-        acc += v;
+    double wsum = 0.0;  // sum of the rows before row j
+#pragma unroll
+    for (int j = 0; j < ROWS; ++j) {
+      double v = x[j];
+#pragma unroll
+      for (int o = 1; o < 64; o <<= 1) {
+        const double t = __shfl_up(v, o, 64);
+        if (lane >= o) v += t;
       }
+      const double rt = __shfl(v, 63, 64);
+      inc[j] = wsum + v;  // wave-local inclusive prefix
+      wsum += rt;
     }
+    if (lane == 0) s_wtot[wv] = wsum;
     __syncthreads();
-    const double base = s_thr[threadIdx.x];
-    DF p{0.0f, 0.0f};
-    for (int i = t0; i < t1; ++i) {
-      const float x = tile[i];
-      if (inclusive) {
-        p = df_add(p, x);
-        tile[i] = (float)(base + ((double)p.hi + (double)p.lo));
+    double wpre = 0.0, agg = 0.0;
+#pragma unroll
+    for (int w = 0; w < WG / 64; ++w) {
+      const double t = s_wtot[w];
+      if (w < wv) wpre += t;
+      agg += t;
+    }
+
+    // decoupled look-back over this segment's earlier chunks (wave 0)
+    if (wv == 0) {
+      if (k == 0) {
+        if (lane == 0) {
+          publish(pub, c, ST_INCL, agg, epoch);
+          s_carry = 0.0;
+        }
       } else {
-        tile[i] = (float)(base + ((double)p.hi + (double)p.lo));
-        p = df_add(p, x);
+        if (lane == 0) publish(pub, c, ST_AGG, agg, epoch);
+        double excl = 0.0;
+        long long wend = c;  // window [wend - 64, wend)
+        for (;;) {
+          const long long idx = wend - 64 + lane;
+          unsigned st;
+          double v = 0.0;
+          if (idx < cfirst) {
+            st = ST_INCL;  // before the segment: identity
+          } else {
+            do {
+              st = read_status(pub, idx, epoch, &v);
+            } while (st == 0);
+          }
+          const unsigned long long im = __ballot(st == ST_INCL);
+          const int h = im ? 63 - __builtin_clzll(im) : -1;
+          // lanes [max(h, 0), 63] in order
+          double r = lane >= h ? v : 0.0;
+#pragma unroll
+          for (int o = 32; o >= 1; o >>= 1) r += __shfl_xor(r, o, 64);
+          excl += r;
+          if (h >= 0) break;
+          wend -= 64;
+        }
+        if (lane == 0) {
+          publish(pub, c, ST_INCL, excl + agg, epoch);
+          s_carry = excl;
+        }
       }
     }
     __syncthreads();
-    for (int i = threadIdx.x; i < m; i += WG) out[lo + i] = tile[i];
+    const double base = s_carry + wpre;
+#pragma unroll
+    for (int j = 0; j < ROWS; ++j) {
+      const long long e = wbase + j * 64 + lane;
+      const double p = inclusive ? inc[j] : inc[j] - x[j];
+      if (e < hi) out[e] = (float)(base + p);
+    }
     if (threadIdx.x == 0) s_c = ticket_draw(&ctl->work);
     __syncthreads();
   }
 }
 
-int grid_size()
+int device_cus()
 {
   int dev = 0, cus = 256;
   if (hipGetDevice(&dev) == hipSuccess) {
-    hipDeviceProp_t p;
-    if (hipGetDeviceProperties(&p, dev) == hipSuccess) cus = p.multiProcessorCount;
+    int v = 0;
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
+        v > 0)
+      cus = v;
   }
-  return cus * 4;
+  return cus;
 }
 
-}  // namespace
-
-// workspace: ctl | cstart (nseg+1) | done (nseg) | partials / publish words
-extern "C" size_t skl_internal_segmented_scan_workspace(int nseg)
+size_t max_chunks(unsigned long long n, int nseg, long long ch)
 {
-  return 256 + (size_t)(nseg + 1) * 8 + (size_t)nseg * 4 + 256 + (size_t)nseg * MAXC * 16;
+  return (size_t)(n / (unsigned long long)ch) + (size_t)nseg + 1;
 }
 
-static void carve(void* ws, int nseg, SegCtl** ctl, long long** cstart, unsigned** done,
-                  void** data)
+// workspace: ctl (256 B) | cstart (nseg + 1) | pad to 256 | per-chunk data
+void carve(void* ws, int nseg, SegCtl** ctl, long long** cstart, void** data)
 {
   char* p = static_cast<char*>(ws);
   *ctl = reinterpret_cast<SegCtl*>(p);
   *cstart = reinterpret_cast<long long*>(p + 256);
-  *done = reinterpret_cast<unsigned*>(p + 256 + (size_t)(nseg + 1) * 8);
-  size_t o = 256 + (size_t)(nseg + 1) * 8 + (size_t)nseg * 4;
+  size_t o = 256 + (size_t)(nseg + 1) * 8;
   o = (o + 255) & ~(size_t)255;
   *data = p + o;
 }
 
-extern "C" int skl_internal_segmented_sum(float* out, const float* in, const int64_t* offsets,
-                                          int nseg, void* ws, hipStream_t s)
+size_t head_bytes(int nseg)
+{
+  return ((256 + (size_t)(nseg + 1) * 8) + 255) & ~(size_t)255;
+}
+
+}  // namespace
+
+extern "C" size_t skl_internal_segmented_workspace(unsigned long long n, int nseg)
+{
+  const size_t sum = max_chunks(n, nseg, SUM_CH) * 8;
+  const size_t scan = max_chunks(n, nseg, SCAN_CH) * 16;
+  return head_bytes(nseg) + (sum > scan ? sum : scan);
+}
+
+extern "C" int skl_internal_segmented_sum(float* out, const float* in, unsigned long long n,
+                                          const int64_t* offsets, int nseg, void* ws,
+                                          hipStream_t s)
 {
   SegCtl* ctl;
   long long* cstart;
-  unsigned* done;
   void* data;
-  carve(ws, nseg, &ctl, &cstart, &done, &data);
-  seg_prologue<<<1, 1024, 0, s>>>(offsets, nseg, cstart, done, out, ctl, 0);
-  seg_sum_kernel<<<grid_size(), WG, 0, s>>>(in, offsets, nseg, cstart, done,
-                                            static_cast<double*>(data), out, ctl);
+  carve(ws, nseg, &ctl, &cstart, &data);
+  seg_prologue<<<1, 1024, 0, s>>>(offsets, nseg, n, SUM_CH, cstart, ctl);
+  const size_t chunks = max_chunks(n, nseg, SUM_CH);
+  const size_t cap = (size_t)device_cus() * 8;
+  const unsigned grid = (unsigned)(chunks < cap ? chunks : cap);
+  seg_sum_kernel<<<grid, WG, 0, s>>>(in, offsets, nseg, cstart, static_cast<double*>(data), ctl);
+  seg_combine_kernel<<<(unsigned)((nseg + WG / 64 - 1) / (WG / 64)), WG, 0, s>>>(
+      cstart, nseg, static_cast<const double*>(data), out, ctl);
   return hipGetLastError() == hipSuccess ? SKL_OK : SKL_ERR_HIP;
 }
 
-extern "C" int skl_internal_segmented_scan(float* out, const float* in, const int64_t* offsets,
-                                           int nseg, int inclusive, void* ws, size_t ws_bytes,
-                                           hipStream_t s)
+extern "C" int skl_internal_segmented_scan(float* out, const float* in, unsigned long long n,
+                                           const int64_t* offsets, int nseg, int inclusive,
+                                           void* ws, hipStream_t s)
 {
-  (void)ws_bytes;
   SegCtl* ctl;
   long long* cstart;
-  unsigned* done;
   void* data;
-  carve(ws, nseg, &ctl, &cstart, &done, &data);
-  seg_prologue<<<1, 1024, 0, s>>>(offsets, nseg, cstart, done, nullptr, ctl, 1);
-  seg_scan_kernel<<<grid_size(), WG, 0, s>>>(in, offsets, nseg, cstart,
-                                             static_cast<unsigned long long*>(data), out,
-                                             inclusive ? 1 : 0, ctl);
+  carve(ws, nseg, &ctl, &cstart, &data);
+  seg_prologue<<<1, 1024, 0, s>>>(offsets, nseg, n, SCAN_CH, cstart, ctl);
+  const size_t chunks = max_chunks(n, nseg, SCAN_CH);
+  const size_t cap = (size_t)device_cus() * 8;
+  const unsigned grid = (unsigned)(chunks < cap ? chunks : cap);
+  seg_scan_kernel<<<grid, WG, 0, s>>>(in, offsets, nseg, cstart,
+                                      static_cast<unsigned long long*>(data), out,
+                                      inclusive ? 1 : 0, ctl);
   return hipGetLastError() == hipSuccess ? SKL_OK : SKL_ERR_HIP;
 }

@@ -83,6 +83,7 @@ public:
     int nseg = (int)offsets.size() - 1;
     o.prepareOutput((size_t)(nseg > 0 ? nseg : 0));
     detail::check(skl_reduce_segmented_run(_h, o.deviceOutputPointer(), in.devicePointer(),
+                                           in.size(),
                                            static_cast<const int64_t*>(offsets.devicePointer()),
                                            nseg, nullptr),
                   "segmented Reduce");

@@ -67,6 +67,7 @@ public:
     o.prepareOutput(in.size());
     int nseg = (int)offsets.size() - 1;
     detail::check(skl_scan_segmented_run(_h, o.deviceOutputPointer(), in.devicePointer(),
+                                         in.size(),
                                          static_cast<const int64_t*>(offsets.devicePointer()),
                                          nseg, inclusive ? 1 : 0, nullptr),
                   "segmented Scan");

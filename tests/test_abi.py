@@ -121,3 +121,14 @@ def test_cpp_api_headers_compile():
                         "-I", os.path.join(ROOT, "skelcl_amd", "include"), src],
                        capture_output=True, text=True)
     assert r.returncode == 0, r.stderr[-4000:]
+
+
+def test_segment_offsets_checked_on_host():
+    """Host offsets are validated before anything reaches the device."""
+    import skelcl_amd as skl
+    from skelcl_amd._lib import SkelError
+    v = skl._segment_offsets([0, 3, 3, 10], 10)
+    assert len(v) == 4
+    for bad, n in (([0, 5, 4], 10), ([0, 11], 10), ([-1, 2], 10), ([], 10)):
+        with pytest.raises(SkelError):
+            skl._segment_offsets(bad, n)

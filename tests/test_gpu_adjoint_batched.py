@@ -157,6 +157,52 @@ def test_segmented_reduce_scan_cfg5_subset(skl):
         assert np.all(np.abs(got - ref) <= tol), incl
 
 
+def test_segmented_long_segment(skl):
+    """A segment of 5e6 elements (1221 scan chunks, 306 reduce chunks) between
+    short and empty ones: multi-window look-back inside one segment."""
+    lens = np.array([3, 5_000_000, 0, 4096, 1], np.int64)
+    off = np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)
+    x = oracle.gen_f32_sym(0x5006, int(off[-1]))
+    X = skl.Vector(x)
+    r = skl.Reduce("float(float)", "float func(float x, float y){ return x+y; }")
+    got = np.asarray(r.segmented(X, off)).astype(np.float64)
+    ref = oracle.segmented_sum_f64(x, off)
+    assert np.all(np.abs(got - ref) <= np.spacing(np.abs(ref).astype(np.float32)) + 1e-12)
+    s = skl.Scan("float(float)", "float func(float x, float y){ return x+y; }", "0.0f")
+    for incl in (False, True):
+        got = np.asarray(s.segmented(X, off, inclusive=incl)).astype(np.float64)
+        ref = oracle.segmented_scan_f64(x, off, incl)
+        tol = np.spacing(np.abs(ref).astype(np.float32)).astype(np.float64) + 1e-9
+        assert np.all(np.abs(got - ref) <= tol), incl
+
+
+def test_segmented_invalid_offsets_flagged_on_device(skl):
+    """Offsets checked by the prologue kernel (the host check is bypassed by
+    passing device offsets): Reduce writes NaN, Scan writes nothing."""
+    from skelcl_amd import _lib
+    n = 64
+    x = oracle.gen_f32_sym(0x5007, n)
+    X = skl.DeviceBuffer.from_array(x)
+    r = skl.Reduce("float(float)", "float func(float x, float y){ return x+y; }")
+    s = skl.Scan("float(float)", "float func(float x, float y){ return x+y; }", "0.0f")
+    for bad in ([0, 40, 20, 64], [0, 10, 65], [-1, 10, 20]):
+        off = skl.DeviceBuffer.from_array(np.array(bad, np.int64))
+        nseg = len(bad) - 1
+        o = skl.DeviceBuffer.from_array(np.full(nseg, 7.0, np.float32))
+        _lib.check(_lib.lib().skl_reduce_segmented_run(r.handle, o.ptr, X.ptr, n, off.ptr, nseg,
+                                                       None))
+        assert np.all(np.isnan(o.to_numpy(np.float32, nseg))), bad
+        so = skl.DeviceBuffer.from_array(np.full(n, 7.0, np.float32))
+        _lib.check(_lib.lib().skl_scan_segmented_run(s.handle, so.ptr, X.ptr, n, off.ptr, nseg,
+                                                     1, None))
+        assert np.all(so.to_numpy(np.float32, n) == 7.0), bad
+    # a valid call on the same objects afterwards is unaffected
+    off = np.array([0, 30, 64], np.int64)
+    got = np.asarray(r.segmented(skl.Vector(x), off)).astype(np.float64)
+    ref = oracle.segmented_sum_f64(x, off)
+    assert np.all(np.abs(got - ref) <= np.spacing(np.abs(ref).astype(np.float32)) + 1e-12)
+
+
 @pytest.mark.parametrize("prec", [1, 2])
 def test_batched_matmul(skl, prec):
     from skelcl_amd import _lib
