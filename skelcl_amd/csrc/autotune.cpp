@@ -206,7 +206,7 @@ extern "C" int skl_autotune(skl_skeleton s, size_t n, int enable)
   if (!s) return fail(SKL_ERR_INVALID, "skeleton is NULL");
   if (!enable) {
     int cus = device_props().multiProcessorCount > 0 ? device_props().multiProcessorCount : 256;
-    apply(s, Shape{4, 8, 4, 8});
+    apply(s, Shape{4, 8, s->kind == SKL_SCAN ? s->default_scan_j() : s->scan_j, 8});
     s->max_blocks = cus * 8;
     return SKL_OK;
   }

@@ -106,53 +106,53 @@ extern "C" __global__ __launch_bounds__(SKL_WG) void skl_reduce(
       acc = skl_lift(SKL_ELEM(e0));
       for (unsigned long long e = e0 + 1; e < n; ++e) acc = comb(acc, skl_lift(SKL_ELEM(e)));
     }
-    // remaining packs, SKL_UNROLL loads in flight per thread
-    for (unsigned long long p0 = g + G; p0 < npk; p0 += G * SKL_UNROLL) {
+    // remaining packs.  Main loop: all SKL_UNROLL packs are full, so the
+    // loads are unconditional and all issued before the first fold (a
+    // guarded load makes the compiler wait for the previous one).
 #ifdef SKL_FUSED_ZIP
-      typedef skl_pack<SKL_TL, SKL_V> pl_t;
-      typedef skl_pack<SKL_TR, SKL_V> pr_t;
-      pl_t a[SKL_UNROLL];
-      pr_t b[SKL_UNROLL];
-#pragma unroll
-      for (int u = 0; u < SKL_UNROLL; ++u) {
-        unsigned long long p = p0 + (unsigned long long)u * G;
-        if (p < nfull) {
-          a[u] = reinterpret_cast<const pl_t*>(l_)[p];
-          b[u] = reinterpret_cast<const pr_t*>(r_)[p];
-        }
-      }
-#pragma unroll
-      for (int u = 0; u < SKL_UNROLL; ++u) {
-        unsigned long long p = p0 + (unsigned long long)u * G;
-        if (p < nfull) {
-#pragma unroll
-          for (int k = 0; k < SKL_V; ++k)
-            acc = comb(acc, skl_lift(SKL_ZCALL(a[u].v[k], b[u].v[k])));
-        } else if (p < npk) {
-          for (unsigned long long e = p * SKL_V; e < n; ++e)
-            acc = comb(acc, skl_lift(SKL_ELEM(e)));
-        }
-      }
+    typedef skl_pack<SKL_TL, SKL_V> pl_t;
+    typedef skl_pack<SKL_TR, SKL_V> pr_t;
+#define SKL_LOAD_PACK(u, p)                                   \
+  pl_t a##u = reinterpret_cast<const pl_t*>(l_)[p];           \
+  pr_t b##u = reinterpret_cast<const pr_t*>(r_)[p];
+#define SKL_FOLD_PACK(u)                                      \
+  _Pragma("unroll") for (int k = 0; k < SKL_V; ++k)           \
+      acc = comb(acc, skl_lift(SKL_ZCALL(a##u.v[k], b##u.v[k])));
 #else
-      typedef skl_pack<SKL_T, SKL_V> pin_t;
-      pin_t a[SKL_UNROLL];
-#pragma unroll
-      for (int u = 0; u < SKL_UNROLL; ++u) {
-        unsigned long long p = p0 + (unsigned long long)u * G;
-        if (p < nfull) a[u] = reinterpret_cast<const pin_t*>(in_)[p];
-      }
-#pragma unroll
-      for (int u = 0; u < SKL_UNROLL; ++u) {
-        unsigned long long p = p0 + (unsigned long long)u * G;
-        if (p < nfull) {
-#pragma unroll
-          for (int k = 0; k < SKL_V; ++k) acc = comb(acc, skl_lift(a[u].v[k]));
-        } else if (p < npk) {
-          for (unsigned long long e = p * SKL_V; e < n; ++e) acc = comb(acc, skl_lift(in_[e]));
-        }
-      }
+    typedef skl_pack<SKL_T, SKL_V> pin_t;
+#define SKL_LOAD_PACK(u, p) pin_t a##u = reinterpret_cast<const pin_t*>(in_)[p];
+#define SKL_FOLD_PACK(u)                                      \
+  _Pragma("unroll") for (int k = 0; k < SKL_V; ++k) acc = comb(acc, skl_lift(a##u.v[k]));
+#endif
+    unsigned long long p = g + G;
+    for (; p + (SKL_UNROLL - 1) * G < nfull; p += SKL_UNROLL * G) {
+#if SKL_UNROLL >= 8
+      SKL_LOAD_PACK(0, p) SKL_LOAD_PACK(1, p + G) SKL_LOAD_PACK(2, p + 2 * G)
+      SKL_LOAD_PACK(3, p + 3 * G) SKL_LOAD_PACK(4, p + 4 * G) SKL_LOAD_PACK(5, p + 5 * G)
+      SKL_LOAD_PACK(6, p + 6 * G) SKL_LOAD_PACK(7, p + 7 * G)
+      SKL_FOLD_PACK(0) SKL_FOLD_PACK(1) SKL_FOLD_PACK(2) SKL_FOLD_PACK(3)
+      SKL_FOLD_PACK(4) SKL_FOLD_PACK(5) SKL_FOLD_PACK(6) SKL_FOLD_PACK(7)
+#elif SKL_UNROLL >= 4
+      SKL_LOAD_PACK(0, p) SKL_LOAD_PACK(1, p + G) SKL_LOAD_PACK(2, p + 2 * G)
+      SKL_LOAD_PACK(3, p + 3 * G)
+      SKL_FOLD_PACK(0) SKL_FOLD_PACK(1) SKL_FOLD_PACK(2) SKL_FOLD_PACK(3)
+#elif SKL_UNROLL >= 2
+      SKL_LOAD_PACK(0, p) SKL_LOAD_PACK(1, p + G)
+      SKL_FOLD_PACK(0) SKL_FOLD_PACK(1)
+#else
+      SKL_LOAD_PACK(0, p)
+      SKL_FOLD_PACK(0)
 #endif
     }
+    for (; p < nfull; p += G) {
+      SKL_LOAD_PACK(0, p)
+      SKL_FOLD_PACK(0)
+    }
+#undef SKL_LOAD_PACK
+#undef SKL_FOLD_PACK
+    // the partial last pack belongs to the thread whose stride lands on it
+    if (p < npk)
+      for (unsigned long long e = p * SKL_V; e < n; ++e) acc = comb(acc, skl_lift(SKL_ELEM(e)));
   }
   // valid threads of this work-group: a prefix of it
   const unsigned long long first = (unsigned long long)blockIdx.x * SKL_WG;

@@ -10,8 +10,9 @@
 //     coalesced 16-byte accesses: pack j of lane l sits at wave_base +
 //     (j*64 + l)*SKL_V; every lane scans its packs, then one wave64 scan per
 //     pack row and an LDS step across waves give the tile-local prefixes;
-//   * wave 0 publishes the tile aggregate, looks back over windows of 64
-//     predecessor tiles (one status word per 32-bit chunk of the carry,
+//   * wave 0 publishes the tile aggregate, looks back over windows of
+//     64 x SKL_LB_K predecessor tiles, SKL_LB_K consecutive ones per lane
+//     (one status word per 32-bit chunk of the carry,
 //     tagged with the launch epoch, read and written with agent-scope
 //     relaxed atomics) and publishes the inclusive prefix;
 //   * outputs = carry (+) local prefix, exclusive (the reference's form,
@@ -28,6 +29,16 @@
 #define SKL_SCAN_AGG 1u
 #define SKL_SCAN_INCL 2u
 #define SKL_CARRY_WORDS ((int)((sizeof(SKL_CARRY_T) + 3) / 4))
+// predecessor tiles read per lane and look-back step (window = 64 x SKL_LB_K)
+// optional: waves per SIMD the register allocation must leave room for
+#if defined(SKL_SCAN_WAVES) && SKL_SCAN_WAVES > 0
+#define SKL_SCAN_ATTR __attribute__((amdgpu_waves_per_eu(SKL_SCAN_WAVES)))
+#else
+#define SKL_SCAN_ATTR
+#endif
+#ifndef SKL_LB_K
+#define SKL_LB_K 4
+#endif
 
 __device__ __forceinline__ void skl_scan_publish(unsigned long long* lb,
                                                  unsigned long long tile,
@@ -67,7 +78,7 @@ __device__ __forceinline__ unsigned skl_scan_read(unsigned long long* lb,
   return tag0 & 3u;
 }
 
-extern "C" __global__ __launch_bounds__(SKL_WG) void skl_scan(
+extern "C" __global__ __launch_bounds__(SKL_WG) SKL_SCAN_ATTR void skl_scan(
     const SKL_T* in_, SKL_T* out_, unsigned long long n, int inclusive,
     unsigned long long* lb, unsigned long long* ctl SKL_EXTRA_PARAMS)
 {
@@ -185,21 +196,41 @@ extern "C" __global__ __launch_bounds__(SKL_WG) void skl_scan(
       bool have = false;
       long long end = (long long)tile;
       while (true) {
-        const long long idx = end - 64 + lane;
-        unsigned st;
-        SKL_CARRY_T v;
-        if (idx < 0) {
-          st = SKL_SCAN_INCL;
-          v = SKL_CARRY_ID;
-        } else {
-          do {
-            st = skl_scan_read(lb, (unsigned long long)idx, epoch, &v);
-          } while (st == 0);
+        // window [end - 64K, end): lane l reads the K consecutive tiles
+        // starting at end - 64K + lK, all loads issued before any wait
+        const long long base = end - 64 * SKL_LB_K + (long long)lane * SKL_LB_K;
+        unsigned st[SKL_LB_K];
+        SKL_CARRY_T v[SKL_LB_K];
+#pragma unroll
+        for (int k = 0; k < SKL_LB_K; ++k) {
+          st[k] = base + k < 0 ? SKL_SCAN_INCL : 0u;
+          v[k] = SKL_CARRY_ID;
         }
-        const unsigned long long im = __ballot(st == SKL_SCAN_INCL);
+        bool pending = true;
+        while (pending) {
+          pending = false;
+#pragma unroll
+          for (int k = 0; k < SKL_LB_K; ++k)
+            if (st[k] == 0) {
+              st[k] = skl_scan_read(lb, (unsigned long long)(base + k), epoch, &v[k]);
+              pending |= st[k] == 0;
+            }
+        }
+        // this lane's tiles folded in order from its last inclusive one
+        SKL_CARRY_T r = v[0];
+        bool incl = st[0] == SKL_SCAN_INCL;
+#pragma unroll
+        for (int k = 1; k < SKL_LB_K; ++k) {
+          if (st[k] == SKL_SCAN_INCL) {
+            r = v[k];
+            incl = true;
+          } else {
+            r = ccomb(r, v[k]);
+          }
+        }
+        const unsigned long long im = __ballot(incl);
         const int h = im ? 63 - __builtin_clzll(im) : -1;
-        // ordered fold of lanes [h, 63]: lane i ends with v_i (+) ... (+) v_63
-        SKL_CARRY_T r = v;
+        // ordered fold of lanes [h, 63]: lane i ends with r_i (+) ... (+) r_63
 #pragma unroll
         for (int off = 1; off < 64; off <<= 1) {
           SKL_CARRY_T o = skl_shfl_down(r, off);
@@ -209,7 +240,7 @@ extern "C" __global__ __launch_bounds__(SKL_WG) void skl_scan(
         excl = have ? ccomb(ws, excl) : ws;
         have = true;
         if (h >= 0) break;
-        end -= 64;
+        end -= 64 * SKL_LB_K;
       }
       if (lane == 0) {
         skl_scan_publish(lb, tile, SKL_SCAN_INCL, ccomb(excl, cagg), epoch);

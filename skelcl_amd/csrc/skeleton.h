@@ -52,4 +52,6 @@ struct skl_skeleton_s {
   size_t seg_scratch_bytes = 0;
   // the float (+) forms with compensated / FP64 combine
   bool precise_add() const;
+  // default Scan tile depth: about 256 bytes of accumulators per lane
+  int default_scan_j() const;
 };

@@ -28,6 +28,14 @@ bool skl_skeleton_s::precise_add() const
          !types.empty() && types[0].name == "float";
 }
 
+int skl_skeleton_s::default_scan_j() const
+{
+  const size_t acc = precise_add() ? 2 * sizeof(float) : (types.empty() ? 4 : types[0].size);
+  const size_t per_pack = (size_t)(V > 0 ? V : 1) * (acc > 0 ? acc : 1);
+  size_t j = 256 / per_pack;
+  return (int)(j < 2 ? 2 : (j > 16 ? 16 : j));
+}
+
 namespace {
 
 std::string str_or(const char* s, const char* dflt) { return s && *s ? s : dflt; }
@@ -559,7 +567,7 @@ static int setup_skeleton(const skl_skeleton_desc* d, skl_skeleton_s* s)
     default:
       s->V = 1;
   }
-  if (s->kind == SKL_SCAN) s->scan_j = s->types[0].size >= 8 ? 4 : 4;
+  if (s->kind == SKL_SCAN) s->scan_j = s->default_scan_j();
   return SKL_OK;
 }
 
@@ -631,8 +639,11 @@ extern "C" int skl_skeleton_set_precision(skl_skeleton s, int precision)
   if (precision < SKL_PREC_DEFAULT || precision > SKL_PREC_FP8)
     return fail(SKL_ERR_INVALID, "unknown precision %d", precision);
   if (precision != s->precision) {
+    const bool default_j = s->kind == SKL_SCAN && s->scan_j == s->default_scan_j();
     s->precision = precision;
+    for (auto& kv : s->progs) skl_program_release(kv.second.program);
     s->progs.clear();
+    if (default_j) s->scan_j = s->default_scan_j();
   }
   return SKL_OK;
 }
