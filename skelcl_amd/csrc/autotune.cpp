@@ -61,7 +61,7 @@ std::string tune_key(const skl_skeleton_s* s, size_t n)
 {
   std::ostringstream o;
   o << s->kind << "|" << s->src << "|" << s->src2 << "|" << s->identity << "|" << s->precision
-    << "|" << s->overlap << "|" << s->padding;
+    << "|" << s->overlap << "|" << s->padding << "|" << s->options;
   for (auto& t : s->types) o << "|" << t.name;
   int cls = n ? (int)std::floor(std::log2((double)n)) : 0;
   return sha1_hex(o.str()) + ":" + std::to_string(cls) + ":" + device_props().gcnArchName;

@@ -9,8 +9,9 @@
 //      SKL_UNROLL 16-byte loads before folding them;
 //   2. a wave64 shuffle tree, then an LDS step across the waves, gives the
 //      work-group partial;
-//   3. the last work-group to finish (agent-scope ticket counter, re-armed
-//      to zero by that work-group) folds all partials and writes out[0].
+//   3. the last work-group to finish (a two-level agent-scope completion
+//      count, re-armed by the work-groups that complete it) folds all
+//      partials and writes out[0].
 // Only threads that own at least one element take part: the valid threads
 // are a prefix of the grid, so every wave knows how many lanes are valid.
 //
@@ -160,11 +161,32 @@ extern "C" __global__ __launch_bounds__(SKL_WG) void skl_reduce(
   const int nvalid = first >= nvt ? 0 : (int)((nvt - first) < SKL_WG ? (nvt - first) : SKL_WG);
   SKL_ACC_T bacc = skl_block_reduce(acc, nvalid, s_acc, comb);
 
+  // Completion count in two levels, so no single address takes one RMW per
+  // work-group (same-address RMWs serialise at ~12 ns each): work-group b
+  // counts itself in sub-counter b % 64 (256 bytes apart); the one that
+  // completes its class counts the class in the top counter; the one that
+  // completes the top counter folds.  Acquire-release RMWs carry every
+  // partial to the folding work-group; the completing ones re-arm the
+  // counters for the next launch.
   if (threadIdx.x == 0) {
     partials[blockIdx.x] = skl_to_carry(bacc);
-    unsigned long long t =
-        __hip_atomic_fetch_add(ticket, 1ull, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-    s_last = (t == (unsigned long long)gridDim.x - 1) ? 1 : 0;
+    const unsigned nb = gridDim.x, q = blockIdx.x & 63u;
+    const unsigned long long members = (nb - 1 - q) / 64 + 1;
+    unsigned long long* sub = ticket + 32 * (1 + q);
+    const unsigned long long t =
+        __hip_atomic_fetch_add(sub, 1ull, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    int last = 0;
+    if (t == members - 1) {
+      __hip_atomic_store(sub, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      const unsigned long long classes = nb < 64 ? nb : 64;
+      const unsigned long long u =
+          __hip_atomic_fetch_add(ticket, 1ull, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+      if (u == classes - 1) {
+        __hip_atomic_store(ticket, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        last = 1;
+      }
+    }
+    s_last = last;
   }
   __syncthreads();
   if (!s_last) return;
@@ -179,10 +201,5 @@ extern "C" __global__ __launch_bounds__(SKL_WG) void skl_reduce(
   }
   const int cvalid = np < SKL_WG ? np : SKL_WG;
   SKL_CARRY_T total = skl_block_reduce(c, cvalid, s_car, ccomb);
-  if (threadIdx.x == 0) {
-    out_[0] = skl_carry_out(total);
-    // every work-group has counted itself: re-arm the counter for the next
-    // launch (also across graph replays)
-    __hip_atomic_store(ticket, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  }
+  if (threadIdx.x == 0) out_[0] = skl_carry_out(total);
 }

@@ -3,21 +3,27 @@
 // one-work-group prologue.
 //
 // The prologue validates the offsets (non-decreasing, inside [0, n]), cuts
-// segment s (length L) into ceil(L / CH) chunks and writes the chunk prefix
-// counts cstart[] (global chunk ids; the chunks of a segment are
-// consecutive).
+// segment s (length L) into ceil(L / CH) chunks and numbers the chunks
+// class-major: segment s belongs to class s % 64, and position
+// p = (s % 64) * M + s / 64 (M = ceil(nseg / 64)) holds its chunk count;
+// cstart[] is the prefix over positions, so a class's chunks form one range
+// of global chunk ids and a segment's chunks are consecutive.
 //   Reduce (CH = 16384): a static grid-stride loop over chunks -- chunks do
 //     not wait on each other -- stores each chunk's sum as an FP64 partial
 //     (compensated float accumulation, FP64 work-group combine); a second
 //     kernel, one wave per segment, adds the segment's partials in a fixed
 //     order and writes out[s].  No ticket, no acquire/release.
-//   Scan (CH = 4096 = one register tile of 256 lanes x 16 elements):
-//     chunks are handed out by a relaxed ticket, so a chunk only waits on
-//     chunks handed out earlier; tile-local prefixes in FP64; chunk
-//     aggregates and inclusive prefixes are published in epoch-tagged
-//     status words (AGG / INCL, 32 bits of the FP64 value per word) and
-//     wave 0 looks back over windows of 64 predecessors within the segment.
-//     Every output is float(carry + local prefix), all in FP64.
+//   Scan (CH = 4096 = one register tile of 256 lanes x 16 elements): each
+//     class has its own relaxed ticket counter (64 counters 256 bytes apart:
+//     one hot counter would serialise every draw at ~12 ns), handing out the
+//     class's chunks in order; a work-group starts on class blockIdx % 64 and
+//     moves on when that class runs dry.  A chunk only waits on earlier
+//     chunks of its own segment, drawn earlier from the same counter.
+//     Tile-local prefixes in FP64; chunk aggregates and inclusive prefixes
+//     are published in epoch-tagged status words (AGG / INCL, 32 bits of the
+//     FP64 value per word) and wave 0 looks back over windows of 64
+//     predecessors within the segment.  Every output is
+//     float(carry + local prefix), all in FP64.
 // Invalid offsets set an error flag: no chunk runs, Reduce writes NaN.
 #include <hip/hip_runtime.h>
 
@@ -34,33 +40,45 @@ constexpr long long SCAN_CH = (long long)WG * ROWS;  // 4096 elements per scan c
 constexpr long long SUM_CH = 16384;                  // elements per reduce chunk
 constexpr unsigned ST_AGG = 1u, ST_INCL = 2u;
 
+constexpr int NCLASS = 64;
+constexpr int CTR_STRIDE = 32;  // counters 256 bytes apart (in 64-bit words)
+
 struct SegCtl {
-  unsigned long long work;   // scan: chunk ticket
   unsigned long long total;  // number of chunks (0 after an error)
   unsigned int epoch;        // scan: tags this launch's status words
   unsigned int error;        // 1: negative length, 2: offsets outside [0, n]
 };
 
-// one work-group of 1024 threads
+__device__ __forceinline__ int seg_of_pos(int p, int m) { return (p / m) + NCLASS * (p % m); }
+__device__ __forceinline__ int pos_of_seg(int s, int m) { return (s % NCLASS) * m + s / NCLASS; }
+
+// one work-group of 1024 threads; npos = NCLASS * M positions
 __global__ __launch_bounds__(1024) void seg_prologue(const int64_t* __restrict__ off, int nseg,
-                                                     unsigned long long n, long long ch,
-                                                     long long* __restrict__ cstart, SegCtl* ctl)
+                                                     int m, unsigned long long n, long long ch,
+                                                     long long* __restrict__ cstart,
+                                                     unsigned long long* __restrict__ ctr,
+                                                     SegCtl* ctl)
 {
   __shared__ long long part[1024];
   __shared__ long long carry;
   __shared__ unsigned err;
+  const int npos = NCLASS * m;
   if (threadIdx.x == 0) {
     carry = 0;
     err = (off[0] < 0 || (unsigned long long)off[nseg] > n) ? 2u : 0u;
   }
+  if (threadIdx.x < NCLASS) ctr[threadIdx.x * CTR_STRIDE] = 0;
   __syncthreads();
-  for (int base = 0; base < nseg; base += 1024) {
-    const int s = base + threadIdx.x;
+  for (int base = 0; base < npos; base += 1024) {
+    const int p = base + threadIdx.x;
     long long cnt = 0;
-    if (s < nseg) {
-      const long long len = off[s + 1] - off[s];
-      if (len < 0) atomicOr(&err, 1u);
-      else cnt = (len + ch - 1) / ch;
+    if (p < npos) {
+      const int s = seg_of_pos(p, m);
+      if (s < nseg) {
+        const long long len = off[s + 1] - off[s];
+        if (len < 0) atomicOr(&err, 1u);
+        else cnt = (len + ch - 1) / ch;
+      }
     }
     part[threadIdx.x] = cnt;
     __syncthreads();
@@ -71,24 +89,24 @@ __global__ __launch_bounds__(1024) void seg_prologue(const int64_t* __restrict__
       part[threadIdx.x] += v;
       __syncthreads();
     }
-    if (s < nseg) cstart[s] = carry + part[threadIdx.x] - cnt;
+    if (p < npos) cstart[p] = carry + part[threadIdx.x] - cnt;
     __syncthreads();
     if (threadIdx.x == 1023) carry += part[1023];
     __syncthreads();
   }
   if (threadIdx.x == 0) {
-    cstart[nseg] = carry;
-    ctl->work = 0;
+    cstart[npos] = carry;
     ctl->total = err ? 0 : (unsigned long long)carry;
     ctl->error = err;
     ctl->epoch = (ctl->epoch + 1u) & 0x3FFFFFFFu;
   }
 }
 
-// segment of global chunk c: cstart[lo] <= c < cstart[lo + 1]
-__device__ __forceinline__ int find_segment(const long long* cstart, int nseg, long long c)
+// position of global chunk c: cstart[lo] <= c < cstart[lo + 1] (positions
+// with no chunks are skipped because cstart[lo + 1] > c)
+__device__ __forceinline__ int find_position(const long long* cstart, int npos, long long c)
 {
-  int lo = 0, hi = nseg;
+  int lo = 0, hi = npos;
   while (hi - lo > 1) {
     int mid = (lo + hi) >> 1;
     if (cstart[mid] <= c) lo = mid;
@@ -122,7 +140,7 @@ __device__ __forceinline__ double block_sum_double(double v, double* red)
 }
 
 __global__ __launch_bounds__(WG) void seg_sum_kernel(const float* __restrict__ in,
-                                                     const int64_t* __restrict__ off, int nseg,
+                                                     const int64_t* __restrict__ off, int m,
                                                      const long long* __restrict__ cstart,
                                                      double* __restrict__ partials,
                                                      const SegCtl* __restrict__ ctl)
@@ -130,8 +148,9 @@ __global__ __launch_bounds__(WG) void seg_sum_kernel(const float* __restrict__ i
   __shared__ double red[WG / 64];
   const long long total = (long long)ctl->total;
   for (long long c = blockIdx.x; c < total; c += gridDim.x) {
-    const int s = find_segment(cstart, nseg, c);
-    const long long k = c - cstart[s];
+    const int pos = find_position(cstart, NCLASS * m, c);
+    const int s = seg_of_pos(pos, m);
+    const long long k = c - cstart[pos];
     const long long lo = off[s] + k * SUM_CH;
     const long long end = off[s + 1];
     const long long hi = lo + SUM_CH < end ? lo + SUM_CH : end;
@@ -153,7 +172,7 @@ __global__ __launch_bounds__(WG) void seg_sum_kernel(const float* __restrict__ i
 // one wave per segment: partials in a fixed order (lane-strided FP64 sums,
 // then a fixed shuffle tree)
 __global__ __launch_bounds__(WG) void seg_combine_kernel(const long long* __restrict__ cstart,
-                                                         int nseg,
+                                                         int nseg, int m,
                                                          const double* __restrict__ partials,
                                                          float* __restrict__ out,
                                                          const SegCtl* __restrict__ ctl)
@@ -165,7 +184,8 @@ __global__ __launch_bounds__(WG) void seg_combine_kernel(const long long* __rest
     if (lane == 0) out[s] = __builtin_nanf("");
     return;
   }
-  const long long c0 = cstart[s], c1 = cstart[s + 1];
+  const int pos = pos_of_seg(s, m);
+  const long long c0 = cstart[pos], c1 = cstart[pos + 1];
   double v = 0.0;
   for (long long c = c0 + lane; c < c1; c += 64) v += partials[c];
   for (int o = 32; o >= 1; o >>= 1) v += __shfl_down(v, o, 64);
@@ -197,12 +217,33 @@ __device__ __forceinline__ unsigned read_status(const unsigned long long* pub, l
   return t0 & 3u;
 }
 
+// thread 0: the next chunk of class q or, once q runs dry, of the following
+// classes; -1 when every class is done.  A dry class is recognised by a
+// plain read first, so its counter takes no further RMWs.
+__device__ __forceinline__ long long draw_chunk(unsigned long long* ctr,
+                                                const long long* cstart, int m, long long total,
+                                                int& q)
+{
+  if (total == 0) return -1;
+  for (int tries = 0; tries < NCLASS; ++tries) {
+    const long long lo = cstart[q * m], hi = cstart[(q + 1) * m];
+    unsigned long long* c = ctr + q * CTR_STRIDE;
+    if (lo + (long long)__hip_atomic_load(c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < hi) {
+      const long long j = ticket_draw(c);
+      if (lo + j < hi) return lo + j;
+    }
+    q = (q + 1) & (NCLASS - 1);
+  }
+  return -1;
+}
+
 __global__ __launch_bounds__(WG) void seg_scan_kernel(const float* __restrict__ in,
-                                                      const int64_t* __restrict__ off, int nseg,
+                                                      const int64_t* __restrict__ off, int m,
                                                       const long long* __restrict__ cstart,
+                                                      unsigned long long* __restrict__ ctr,
                                                       unsigned long long* __restrict__ pub,
                                                       float* __restrict__ out, int inclusive,
-                                                      SegCtl* ctl)
+                                                      const SegCtl* __restrict__ ctl)
 {
   __shared__ long long s_c;
   __shared__ double s_wtot[WG / 64];
@@ -210,14 +251,16 @@ __global__ __launch_bounds__(WG) void seg_scan_kernel(const float* __restrict__ 
   const long long total = (long long)ctl->total;
   const unsigned epoch = ctl->epoch;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  if (threadIdx.x == 0) s_c = ticket_draw(&ctl->work);
+  int q = blockIdx.x & (NCLASS - 1);  // thread 0's current class
+  if (threadIdx.x == 0) s_c = draw_chunk(ctr, cstart, m, total, q);
   __syncthreads();
   for (;;) {
     const long long c = ticket_read(s_c);
     __syncthreads();
-    if (c >= total) break;
-    const int s = find_segment(cstart, nseg, c);
-    const long long cfirst = cstart[s];
+    if (c < 0) break;
+    const int pos = find_position(cstart, NCLASS * m, c);
+    const int s = seg_of_pos(pos, m);
+    const long long cfirst = cstart[pos];
     const long long k = c - cfirst;
     const long long lo = off[s] + k * SCAN_CH;
     const long long end = off[s + 1];
@@ -300,7 +343,7 @@ __global__ __launch_bounds__(WG) void seg_scan_kernel(const float* __restrict__ 
       const double p = inclusive ? inc[j] : inc[j] - x[j];
       if (e < hi) out[e] = (float)(base + p);
     }
-    if (threadIdx.x == 0) s_c = ticket_draw(&ctl->work);
+    if (threadIdx.x == 0) s_c = draw_chunk(ctr, cstart, m, total, q);
     __syncthreads();
   }
 }
@@ -322,20 +365,24 @@ size_t max_chunks(unsigned long long n, int nseg, long long ch)
   return (size_t)(n / (unsigned long long)ch) + (size_t)nseg + 1;
 }
 
-// workspace: ctl (256 B) | cstart (nseg + 1) | pad to 256 | per-chunk data
-void carve(void* ws, int nseg, SegCtl** ctl, long long** cstart, void** data)
+int classes_m(int nseg) { return (nseg + NCLASS - 1) / NCLASS; }
+
+// workspace: SegCtl (256 B) | 64 ticket counters, 256 B apart | cstart over
+// the 64 * M positions (+1) | pad to 256 | per-chunk data
+size_t head_bytes(int nseg)
+{
+  const size_t npos = (size_t)NCLASS * classes_m(nseg);
+  return (256 + (size_t)NCLASS * CTR_STRIDE * 8 + (npos + 1) * 8 + 255) & ~(size_t)255;
+}
+
+void carve(void* ws, SegCtl** ctl, unsigned long long** ctr, long long** cstart, void** data,
+           int nseg)
 {
   char* p = static_cast<char*>(ws);
   *ctl = reinterpret_cast<SegCtl*>(p);
-  *cstart = reinterpret_cast<long long*>(p + 256);
-  size_t o = 256 + (size_t)(nseg + 1) * 8;
-  o = (o + 255) & ~(size_t)255;
-  *data = p + o;
-}
-
-size_t head_bytes(int nseg)
-{
-  return ((256 + (size_t)(nseg + 1) * 8) + 255) & ~(size_t)255;
+  *ctr = reinterpret_cast<unsigned long long*>(p + 256);
+  *cstart = reinterpret_cast<long long*>(p + 256 + (size_t)NCLASS * CTR_STRIDE * 8);
+  *data = p + head_bytes(nseg);
 }
 
 }  // namespace
@@ -352,16 +399,18 @@ extern "C" int skl_internal_segmented_sum(float* out, const float* in, unsigned 
                                           hipStream_t s)
 {
   SegCtl* ctl;
+  unsigned long long* ctr;
   long long* cstart;
   void* data;
-  carve(ws, nseg, &ctl, &cstart, &data);
-  seg_prologue<<<1, 1024, 0, s>>>(offsets, nseg, n, SUM_CH, cstart, ctl);
+  carve(ws, &ctl, &ctr, &cstart, &data, nseg);
+  const int m = classes_m(nseg);
+  seg_prologue<<<1, 1024, 0, s>>>(offsets, nseg, m, n, SUM_CH, cstart, ctr, ctl);
   const size_t chunks = max_chunks(n, nseg, SUM_CH);
   const size_t cap = (size_t)device_cus() * 8;
   const unsigned grid = (unsigned)(chunks < cap ? chunks : cap);
-  seg_sum_kernel<<<grid, WG, 0, s>>>(in, offsets, nseg, cstart, static_cast<double*>(data), ctl);
+  seg_sum_kernel<<<grid, WG, 0, s>>>(in, offsets, m, cstart, static_cast<double*>(data), ctl);
   seg_combine_kernel<<<(unsigned)((nseg + WG / 64 - 1) / (WG / 64)), WG, 0, s>>>(
-      cstart, nseg, static_cast<const double*>(data), out, ctl);
+      cstart, nseg, m, static_cast<const double*>(data), out, ctl);
   return hipGetLastError() == hipSuccess ? SKL_OK : SKL_ERR_HIP;
 }
 
@@ -370,14 +419,16 @@ extern "C" int skl_internal_segmented_scan(float* out, const float* in, unsigned
                                            void* ws, hipStream_t s)
 {
   SegCtl* ctl;
+  unsigned long long* ctr;
   long long* cstart;
   void* data;
-  carve(ws, nseg, &ctl, &cstart, &data);
-  seg_prologue<<<1, 1024, 0, s>>>(offsets, nseg, n, SCAN_CH, cstart, ctl);
+  carve(ws, &ctl, &ctr, &cstart, &data, nseg);
+  const int m = classes_m(nseg);
+  seg_prologue<<<1, 1024, 0, s>>>(offsets, nseg, m, n, SCAN_CH, cstart, ctr, ctl);
   const size_t chunks = max_chunks(n, nseg, SCAN_CH);
   const size_t cap = (size_t)device_cus() * 8;
   const unsigned grid = (unsigned)(chunks < cap ? chunks : cap);
-  seg_scan_kernel<<<grid, WG, 0, s>>>(in, offsets, nseg, cstart,
+  seg_scan_kernel<<<grid, WG, 0, s>>>(in, offsets, m, cstart, ctr,
                                       static_cast<unsigned long long*>(data), out,
                                       inclusive ? 1 : 0, ctl);
   return hipGetLastError() == hipSuccess ? SKL_OK : SKL_ERR_HIP;

@@ -856,8 +856,10 @@ static int reduce_launch(skl_skeleton_s* s, const std::string& key, void* out,
   size_t csz = (carry_size(s) + 7) & ~(size_t)7;
   unsigned long long npk = (n + s->V - 1) / s->V;
   unsigned blocks = blocks_for(npk, 256, (unsigned long long)s->max_blocks);
-  SKL_TRY(ensure_scratch(s, 256 + (size_t)s->max_blocks * csz));
-  void* partials = static_cast<char*>(s->scratch) + 256;
+  // scratch: top counter at 0, 64 sub-counters 256 bytes apart, partials
+  constexpr size_t kCounters = 256 * 65;
+  SKL_TRY(ensure_scratch(s, kCounters + (size_t)s->max_blocks * csz));
+  void* partials = static_cast<char*>(s->scratch) + kCounters;
   void* ticket = s->scratch;
   unsigned long long nn = n;
   skl_kernel k;
