@@ -166,6 +166,8 @@ def run_extras(skelcl, L, ck, stream, args):
   return sum;
 }""", R, skelcl.NEUTRAL, 0)
     warg, nw, keep, _ = gauss._marshal((wv,))
+    if args.autotune:
+        ck(L.skl_autotune_args(gauss.handle, S * S, warg, nw))
     t_mo = _time(stream, lambda: ck(L.skl_mapoverlap_run(gauss.handle, outb.ptr, img.ptr, S, S,
                                                          warg, nw, st)), reps=5)
     t_adj = _time(stream, lambda: ck(L.skl_mapoverlap_stencil_backward(
@@ -315,6 +317,8 @@ def main() -> int:
                          "FP32 / BF16X3 / FP8 with errors vs FP64, MapOverlap 8192^2 R=5 "
                          "forward + adjoint, batched segmented Reduce/Scan + 512 GEMMs, "
                          "fused dot product)")
+    ap.add_argument("--autotune", type=int, default=1,
+                    help="1: autotune the launch shapes of the timed skeletons first")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also time an identity Map (copy) leg, for counter calibration")
     args = ap.parse_args()
@@ -350,6 +354,18 @@ def main() -> int:
     sax = skelcl.Zip("float(float, float)",
                      "float func(float x, float y, float a){ return a*x+y; }")
     a_arg, n_a, keep, _ = sax._marshal((np.float32(A),))
+    shapes = {}
+    if args.autotune:
+        # launch shapes measured once per (skeleton, size class, device) and
+        # cached; outside the timed region
+        import ctypes as C
+        ck(L.skl_autotune(scan.handle, N, 1))
+        ck(L.skl_autotune(red.handle, N, 1))
+        ck(L.skl_autotune_args(sax.handle, N, a_arg, n_a))
+        for name, sk in (("scan", scan), ("reduce", red), ("saxpy", sax)):
+            sh = (C.c_int * 4)()
+            ck(L.skl_skeleton_launch_shape(sk.handle, sh))
+            shapes[name] = dict(zip(("unroll", "blocks_per_cu", "scan_j", "mo_rb"), list(sh)))
 
     def run_scan():
         ck(L.skl_scan_run(scan.handle, sout.ptr, xs.ptr, N, 1, None, 0, st))
@@ -439,6 +455,7 @@ def main() -> int:
                      "frac": round(dl["bytes"] / (dl["avg_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
                      "traffic": measured_traffic(dom_kernel),
                      "algorithmic_bytes": dl["bytes"]},
+        "launch_shapes": shapes,
         "device": {"name": info["name"], "arch": info["arch"], "cus": info["compute_units"]},
         "wall_s": round(wall, 3),
         "reduce_result": got_red,

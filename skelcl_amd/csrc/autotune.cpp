@@ -100,7 +100,7 @@ std::vector<Shape> candidates(const skl_skeleton_s* s)
       for (int j : {2, 4, 8, 16}) c.push_back(Shape{d.unroll, d.blocks_per_cu, j, d.mo_rb});
       break;
     case SKL_MAPOVERLAP:
-      for (int rb : {2, 4, 8, 16}) c.push_back(Shape{d.unroll, d.blocks_per_cu, d.scan_j, rb});
+      for (int rb : {1, 2, 4, 8}) c.push_back(Shape{d.unroll, d.blocks_per_cu, d.scan_j, rb});
       break;
     default:
       c.push_back(d);
@@ -206,8 +206,15 @@ extern "C" int skl_autotune(skl_skeleton s, size_t n, int enable)
   if (!s) return fail(SKL_ERR_INVALID, "skeleton is NULL");
   if (!enable) {
     int cus = device_props().multiProcessorCount > 0 ? device_props().multiProcessorCount : 256;
-    apply(s, Shape{4, 8, s->kind == SKL_SCAN ? s->default_scan_j() : s->scan_j, 8});
-    s->max_blocks = cus * 8;
+    skl_skeleton_s d;
+    d.kind = s->kind;
+    d.types = s->types;
+    d.op = s->op;
+    d.precision = s->precision;
+    d.V = s->V;
+    d.extras = s->extras;
+    d.apply_default_shape(cus);
+    apply(s, Shape{d.unroll, d.max_blocks / cus, d.scan_j, d.mo_rb});
     return SKL_OK;
   }
   if (!s->extras.empty())

@@ -42,20 +42,18 @@ extern "C" __global__ __launch_bounds__(256) void skl_allpairs(
     for (int e = tid; e < SKL_AP_BM * SKL_AP_BK; e += 256) {
       const int r = e / SKL_AP_BK, kk = e - r * SKL_AP_BK;
       const int gr = m0 + r, gk = k0 + kk;
-      SKL_TL v;
-      __builtin_memset(&v, 0, sizeof(v));
-      if (gr < M && gk < K) v = A[(size_t)gr * K + gk];
-      As[kk][r] = v;
+      SKL_TL z;
+      __builtin_memset(&z, 0, sizeof(z));
+      As[kk][r] = skl_ld_or(A, (size_t)gr * K + gk, 0, gr < M && gk < K, z);
     }
     // B: 16 k x 64 columns, read along columns
 #pragma unroll
     for (int e = tid; e < SKL_AP_BK * SKL_AP_BN; e += 256) {
       const int kk = e / SKL_AP_BN, c = e - kk * SKL_AP_BN;
       const int gk = k0 + kk, gc = n0 + c;
-      SKL_TR v;
-      __builtin_memset(&v, 0, sizeof(v));
-      if (gk < K && gc < N) v = B[(size_t)gk * N + gc];
-      Bs[kk][c] = v;
+      SKL_TR z;
+      __builtin_memset(&z, 0, sizeof(z));
+      Bs[kk][c] = skl_ld_or(B, (size_t)gk * N + gc, 0, gk < K && gc < N, z);
     }
     __syncthreads();
     const int kmax = K - k0 < SKL_AP_BK ? K - k0 : SKL_AP_BK;
@@ -138,17 +136,15 @@ extern "C" __global__ __launch_bounds__(256) void skl_allpairs_general_lds(
   const int m0 = blockIdx.y * SKL_APG_B, n0 = blockIdx.x * SKL_APG_B;
   for (int e = tid; e < SKL_APG_B * K; e += 256) {
     const int r = e / K, k = e - r * K;
-    SKL_TL v;
-    __builtin_memset(&v, 0, sizeof(v));
-    if (m0 + r < M) v = A[(size_t)(m0 + r) * K + k];
-    As[e] = v;
+    SKL_TL z;
+    __builtin_memset(&z, 0, sizeof(z));
+    As[e] = skl_ld_or(A, (size_t)(m0 + r) * K + k, 0, m0 + r < M, z);
   }
   for (int e = tid; e < SKL_APG_B * K; e += 256) {
     const int k = e / SKL_APG_B, c = e - k * SKL_APG_B;
-    SKL_TR v;
-    __builtin_memset(&v, 0, sizeof(v));
-    if (n0 + c < N) v = B[(size_t)k * N + n0 + c];
-    Bs[e] = v;
+    SKL_TR z;
+    __builtin_memset(&z, 0, sizeof(z));
+    Bs[e] = skl_ld_or(B, (size_t)k * N + n0 + c, 0, n0 + c < N, z);
   }
   __syncthreads();
   const int tx = tid & 31, ty = tid >> 5;

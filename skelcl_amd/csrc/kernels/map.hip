@@ -3,9 +3,9 @@
 // reference's one-element-per-work-item forms).
 //
 // Fast path: HBM streaming.  Each thread moves SKL_UNROLL packs of SKL_V
-// elements (16-byte global_load/store_dwordx4 for 4-byte types), all loads
-// of a thread issued before any user-function evaluation, grid-stride over
-// the whole array.  Used when the user function does not ask for its
+// elements (16-byte global_load/store_dwordx4 for 4-byte types) per batch,
+// grid-stride over the whole array; the next batch's loads are issued
+// before the current batch is evaluated and stored.  Used when the user function does not ask for its
 // work-item id, barriers or __local memory.
 // Simple path: one element per work-item with the configured work-group
 // size -- get_global_id(0) is the element index, as in the reference.
@@ -23,26 +23,38 @@ extern "C" __global__ __launch_bounds__(SKL_WG) void skl_map(
   typedef skl_pack<SKL_TIN, SKL_V> pin_t;
   typedef skl_pack<SKL_TOUT, SKL_V> pout_t;
   const unsigned long long npk = n / SKL_V;
-  const unsigned long long step = (unsigned long long)gridDim.x * SKL_WG * SKL_UNROLL;
-  for (unsigned long long p0 = (unsigned long long)blockIdx.x * SKL_WG * SKL_UNROLL + threadIdx.x;
-       p0 < npk; p0 += step) {
-    pin_t a[SKL_UNROLL];
+  constexpr unsigned long long U = SKL_UNROLL, W = SKL_WG;
+  const unsigned long long step = (unsigned long long)gridDim.x * W * U;
+  auto ld = [&](unsigned long long p) { return reinterpret_cast<const pin_t*>(in_)[p]; };
+  auto emit = [&](unsigned long long p, const pin_t& x) {
+    pout_t r;
 #pragma unroll
-    for (int u = 0; u < SKL_UNROLL; ++u) {
-      unsigned long long p = p0 + (unsigned long long)u * SKL_WG;
-      if (p < npk) a[u] = reinterpret_cast<const pin_t*>(in_)[p];
+    for (int k = 0; k < SKL_V; ++k) r.v[k] = SKL_CALL1(x.v[k]);
+    reinterpret_cast<pout_t*>(out_)[p] = r;
+  };
+  unsigned long long p0 = (unsigned long long)blockIdx.x * W * U + threadIdx.x;
+  // full batches: unconditional loads, the next batch's loads issued before
+  // this batch's stores (one counter tracks loads and stores alike, so a
+  // load issued after a store would also wait for the store)
+  if (p0 + (U - 1) * W < npk) {
+    pin_t a[SKL_UNROLL], b[SKL_UNROLL];
+#pragma unroll
+    for (int u = 0; u < SKL_UNROLL; ++u) a[u] = ld(p0 + u * W);
+    for (; p0 + step + (U - 1) * W < npk; p0 += step) {
+#pragma unroll
+      for (int u = 0; u < SKL_UNROLL; ++u) b[u] = ld(p0 + step + u * W);
+#pragma unroll
+      for (int u = 0; u < SKL_UNROLL; ++u) emit(p0 + u * W, a[u]);
+#pragma unroll
+      for (int u = 0; u < SKL_UNROLL; ++u) a[u] = b[u];
     }
 #pragma unroll
-    for (int u = 0; u < SKL_UNROLL; ++u) {
-      unsigned long long p = p0 + (unsigned long long)u * SKL_WG;
-      if (p < npk) {
-        pout_t r;
-#pragma unroll
-        for (int k = 0; k < SKL_V; ++k) r.v[k] = SKL_CALL1(a[u].v[k]);
-        reinterpret_cast<pout_t*>(out_)[p] = r;
-      }
-    }
+    for (int u = 0; u < SKL_UNROLL; ++u) emit(p0 + u * W, a[u]);
+    p0 += step;
   }
+  if (p0 < npk)  // a partial batch
+    for (int u = 0; u < SKL_UNROLL; ++u)
+      if (p0 + u * W < npk) emit(p0 + u * W, ld(p0 + u * W));
   if (blockIdx.x == gridDim.x - 1)
     for (unsigned long long i = npk * SKL_V + threadIdx.x; i < n; i += SKL_WG)
       out_[i] = SKL_CALL1(in_[i]);
@@ -160,30 +172,48 @@ extern "C" __global__ __launch_bounds__(SKL_WG) void skl_zip(
   typedef skl_pack<SKL_TR, SKL_V> pr_t;
   typedef skl_pack<SKL_TOUT, SKL_V> pout_t;
   const unsigned long long npk = n / SKL_V;
-  const unsigned long long step = (unsigned long long)gridDim.x * SKL_WG * SKL_UNROLL;
-  for (unsigned long long p0 = (unsigned long long)blockIdx.x * SKL_WG * SKL_UNROLL + threadIdx.x;
-       p0 < npk; p0 += step) {
-    pl_t a[SKL_UNROLL];
-    pr_t b[SKL_UNROLL];
+  constexpr unsigned long long U = SKL_UNROLL, W = SKL_WG;
+  const unsigned long long step = (unsigned long long)gridDim.x * W * U;
+  auto emit = [&](unsigned long long p, const pl_t& x, const pr_t& y) {
+    pout_t r;
+#pragma unroll
+    for (int k = 0; k < SKL_V; ++k) r.v[k] = SKL_CALL2(x.v[k], y.v[k]);
+    reinterpret_cast<pout_t*>(out_)[p] = r;
+  };
+  unsigned long long p0 = (unsigned long long)blockIdx.x * W * U + threadIdx.x;
+  // full batches: unconditional loads, the next batch's loads issued before
+  // this batch's stores
+  if (p0 + (U - 1) * W < npk) {
+    pl_t a[SKL_UNROLL], na[SKL_UNROLL];
+    pr_t b[SKL_UNROLL], nb[SKL_UNROLL];
 #pragma unroll
     for (int u = 0; u < SKL_UNROLL; ++u) {
-      unsigned long long p = p0 + (unsigned long long)u * SKL_WG;
-      if (p < npk) {
-        a[u] = reinterpret_cast<const pl_t*>(l_)[p];
-        b[u] = reinterpret_cast<const pr_t*>(r_)[p];
+      a[u] = reinterpret_cast<const pl_t*>(l_)[p0 + u * W];
+      b[u] = reinterpret_cast<const pr_t*>(r_)[p0 + u * W];
+    }
+    for (; p0 + step + (U - 1) * W < npk; p0 += step) {
+#pragma unroll
+      for (int u = 0; u < SKL_UNROLL; ++u) {
+        na[u] = reinterpret_cast<const pl_t*>(l_)[p0 + step + u * W];
+        nb[u] = reinterpret_cast<const pr_t*>(r_)[p0 + step + u * W];
+      }
+#pragma unroll
+      for (int u = 0; u < SKL_UNROLL; ++u) emit(p0 + u * W, a[u], b[u]);
+#pragma unroll
+      for (int u = 0; u < SKL_UNROLL; ++u) {
+        a[u] = na[u];
+        b[u] = nb[u];
       }
     }
 #pragma unroll
-    for (int u = 0; u < SKL_UNROLL; ++u) {
-      unsigned long long p = p0 + (unsigned long long)u * SKL_WG;
-      if (p < npk) {
-        pout_t r;
-#pragma unroll
-        for (int k = 0; k < SKL_V; ++k) r.v[k] = SKL_CALL2(a[u].v[k], b[u].v[k]);
-        reinterpret_cast<pout_t*>(out_)[p] = r;
-      }
-    }
+    for (int u = 0; u < SKL_UNROLL; ++u) emit(p0 + u * W, a[u], b[u]);
+    p0 += step;
   }
+  if (p0 < npk)  // a partial batch
+    for (int u = 0; u < SKL_UNROLL; ++u)
+      if (p0 + u * W < npk)
+        emit(p0 + u * W, reinterpret_cast<const pl_t*>(l_)[p0 + u * W],
+             reinterpret_cast<const pr_t*>(r_)[p0 + u * W]);
   if (blockIdx.x == gridDim.x - 1)
     for (unsigned long long i = npk * SKL_V + threadIdx.x; i < n; i += SKL_WG)
       out_[i] = SKL_CALL2(l_[i], r_[i]);

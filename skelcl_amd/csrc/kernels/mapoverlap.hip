@@ -20,6 +20,13 @@
 #define SKL_MO_TY 4
 #define SKL_MO_TH (SKL_MO_TY * SKL_MO_RB)
 #define SCL_TILE_WIDTH (SKL_MO_TW + 2 * SCL_OVERLAP_RANGE)
+// optional: waves per SIMD the register allocation must leave room for (the
+// unrolled rows otherwise keep the whole input window in registers)
+#if defined(SKL_MO_WAVES) && SKL_MO_WAVES > 0
+#define SKL_MO_ATTR __attribute__((amdgpu_waves_per_eu(SKL_MO_WAVES)))
+#else
+#define SKL_MO_ATTR
+#endif
 
 typedef struct {
   const SKL_TIN* data;  // LDS tile
@@ -32,7 +39,7 @@ __device__ __forceinline__ SKL_TIN getData(input_matrix_t m, int x, int y)
   return m.data[(m.local_row + y) * SCL_TILE_WIDTH + m.local_column + x];
 }
 
-extern "C" __global__ __launch_bounds__(256) void skl_mapoverlap(
+extern "C" __global__ __launch_bounds__(256) SKL_MO_ATTR void skl_mapoverlap(
     const SKL_TIN* in_, SKL_TOUT* out_, int rows, int cols SKL_EXTRA_PARAMS)
 {
   SKL_EXTRA_SETUP
@@ -42,24 +49,28 @@ extern "C" __global__ __launch_bounds__(256) void skl_mapoverlap(
   __shared__ SKL_TIN tile[THP * TWP];
   const int c0 = blockIdx.x * SKL_MO_TW;
   const int r0 = blockIdx.y * SKL_MO_TH;
-  for (int idx = threadIdx.x; idx < THP * TWP; idx += 256) {
+  // every load issues unconditionally from clamped coordinates (the NEAREST
+  // rule), so a thread's loads are all in flight before the LDS writes
+  constexpr int NLD = (THP * TWP + 255) / 256;
+  SKL_TIN v[NLD];
+#pragma unroll
+  for (int u = 0; u < NLD; ++u) {
+    const int idx = (int)threadIdx.x + 256 * u < THP * TWP ? (int)threadIdx.x + 256 * u : 0;
     const int tr = idx / TWP;
     const int tc = idx - tr * TWP;
-    int gr = r0 - R + tr;
-    int gc = c0 - R + tc;
-    SKL_TIN v;
-    if (gr >= 0 && gr < rows && gc >= 0 && gc < cols) {
-      v = in_[(size_t)gr * cols + gc];
-    } else {
+    const int gr = r0 - R + tr;
+    const int gc = c0 - R + tc;
+    const int cr = gr < 0 ? 0 : (gr >= rows ? rows - 1 : gr);
+    const int cc = gc < 0 ? 0 : (gc >= cols ? cols - 1 : gc);
+    v[u] = in_[(size_t)cr * cols + cc];
 #ifdef SKL_PAD_NEUTRAL
-      v = SKL_NEUTRAL;
-#else
-      gr = gr < 0 ? 0 : (gr >= rows ? rows - 1 : gr);
-      gc = gc < 0 ? 0 : (gc >= cols ? cols - 1 : gc);
-      v = in_[(size_t)gr * cols + gc];
+    if (gr != cr || gc != cc) v[u] = SKL_NEUTRAL;
 #endif
-    }
-    tile[idx] = v;
+  }
+#pragma unroll
+  for (int u = 0; u < NLD; ++u) {
+    const int idx = (int)threadIdx.x + 256 * u;
+    if (idx < THP * TWP) tile[idx] = v[u];
   }
   __syncthreads();
   const int tx = threadIdx.x & 63;

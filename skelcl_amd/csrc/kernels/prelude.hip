@@ -60,6 +60,17 @@ __device__ __forceinline__ uint get_work_dim() { return 1; }
 __device__ __forceinline__ void barrier(int) { __syncthreads(); }
 __device__ __forceinline__ void mem_fence(int) { __threadfence(); }
 __device__ __forceinline__ int mad24(int a, int b, int c) { return a * b + c; }
+
+// p[i] when ok, else `fill`, with the load outside any branch: the index
+// falls back to `safe` (any valid index) and the load always issues.  A load
+// under a branch is waited for before the next one is issued; these keep
+// several in flight.
+template <typename T>
+__device__ __forceinline__ T skl_ld_or(const T* p, size_t i, size_t safe, bool ok, T fill)
+{
+  const T v = p[ok ? i : safe];
+  return ok ? v : fill;
+}
 __device__ __forceinline__ uint mad24(uint a, uint b, uint c) { return a * b + c; }
 __device__ __forceinline__ int mul24(int a, int b) { return a * b; }
 __device__ __forceinline__ uint mul24(uint a, uint b) { return a * b; }

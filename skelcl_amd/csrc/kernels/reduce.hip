@@ -107,50 +107,54 @@ extern "C" __global__ __launch_bounds__(SKL_WG) void skl_reduce(
       acc = skl_lift(SKL_ELEM(e0));
       for (unsigned long long e = e0 + 1; e < n; ++e) acc = comb(acc, skl_lift(SKL_ELEM(e)));
     }
-    // remaining packs.  Main loop: all SKL_UNROLL packs are full, so the
-    // loads are unconditional and all issued before the first fold (a
-    // guarded load makes the compiler wait for the previous one).
+    // remaining packs.  Main loop over strides whose SKL_UNROLL packs are
+    // all full: the loads are unconditional (a guarded load makes the
+    // compiler wait for the previous one) and software-pipelined -- the next
+    // stride's loads are issued before the current stride is folded.
 #ifdef SKL_FUSED_ZIP
     typedef skl_pack<SKL_TL, SKL_V> pl_t;
     typedef skl_pack<SKL_TR, SKL_V> pr_t;
-#define SKL_LOAD_PACK(u, p)                                   \
-  pl_t a##u = reinterpret_cast<const pl_t*>(l_)[p];           \
-  pr_t b##u = reinterpret_cast<const pr_t*>(r_)[p];
-#define SKL_FOLD_PACK(u)                                      \
-  _Pragma("unroll") for (int k = 0; k < SKL_V; ++k)           \
-      acc = comb(acc, skl_lift(SKL_ZCALL(a##u.v[k], b##u.v[k])));
+    struct pk_t {
+      pl_t l;
+      pr_t r;
+    };
+    auto ldp = [&](unsigned long long q) {
+      pk_t x;
+      x.l = reinterpret_cast<const pl_t*>(l_)[q];
+      x.r = reinterpret_cast<const pr_t*>(r_)[q];
+      return x;
+    };
+    auto fold = [&](const pk_t& x) {
+#pragma unroll
+      for (int k = 0; k < SKL_V; ++k) acc = comb(acc, skl_lift(SKL_ZCALL(x.l.v[k], x.r.v[k])));
+    };
 #else
-    typedef skl_pack<SKL_T, SKL_V> pin_t;
-#define SKL_LOAD_PACK(u, p) pin_t a##u = reinterpret_cast<const pin_t*>(in_)[p];
-#define SKL_FOLD_PACK(u)                                      \
-  _Pragma("unroll") for (int k = 0; k < SKL_V; ++k) acc = comb(acc, skl_lift(a##u.v[k]));
+    typedef skl_pack<SKL_T, SKL_V> pk_t;
+    auto ldp = [&](unsigned long long q) { return reinterpret_cast<const pk_t*>(in_)[q]; };
+    auto fold = [&](const pk_t& x) {
+#pragma unroll
+      for (int k = 0; k < SKL_V; ++k) acc = comb(acc, skl_lift(x.v[k]));
+    };
 #endif
+    constexpr unsigned long long U = SKL_UNROLL;
     unsigned long long p = g + G;
-    for (; p + (SKL_UNROLL - 1) * G < nfull; p += SKL_UNROLL * G) {
-#if SKL_UNROLL >= 8
-      SKL_LOAD_PACK(0, p) SKL_LOAD_PACK(1, p + G) SKL_LOAD_PACK(2, p + 2 * G)
-      SKL_LOAD_PACK(3, p + 3 * G) SKL_LOAD_PACK(4, p + 4 * G) SKL_LOAD_PACK(5, p + 5 * G)
-      SKL_LOAD_PACK(6, p + 6 * G) SKL_LOAD_PACK(7, p + 7 * G)
-      SKL_FOLD_PACK(0) SKL_FOLD_PACK(1) SKL_FOLD_PACK(2) SKL_FOLD_PACK(3)
-      SKL_FOLD_PACK(4) SKL_FOLD_PACK(5) SKL_FOLD_PACK(6) SKL_FOLD_PACK(7)
-#elif SKL_UNROLL >= 4
-      SKL_LOAD_PACK(0, p) SKL_LOAD_PACK(1, p + G) SKL_LOAD_PACK(2, p + 2 * G)
-      SKL_LOAD_PACK(3, p + 3 * G)
-      SKL_FOLD_PACK(0) SKL_FOLD_PACK(1) SKL_FOLD_PACK(2) SKL_FOLD_PACK(3)
-#elif SKL_UNROLL >= 2
-      SKL_LOAD_PACK(0, p) SKL_LOAD_PACK(1, p + G)
-      SKL_FOLD_PACK(0) SKL_FOLD_PACK(1)
-#else
-      SKL_LOAD_PACK(0, p)
-      SKL_FOLD_PACK(0)
-#endif
+    if (p + (U - 1) * G < nfull) {
+      pk_t a[SKL_UNROLL], b[SKL_UNROLL];
+#pragma unroll
+      for (int u = 0; u < SKL_UNROLL; ++u) a[u] = ldp(p + u * G);
+      for (; p + (2 * U - 1) * G < nfull; p += U * G) {
+#pragma unroll
+        for (int u = 0; u < SKL_UNROLL; ++u) b[u] = ldp(p + (U + u) * G);
+#pragma unroll
+        for (int u = 0; u < SKL_UNROLL; ++u) fold(a[u]);
+#pragma unroll
+        for (int u = 0; u < SKL_UNROLL; ++u) a[u] = b[u];
+      }
+#pragma unroll
+      for (int u = 0; u < SKL_UNROLL; ++u) fold(a[u]);
+      p += U * G;
     }
-    for (; p < nfull; p += G) {
-      SKL_LOAD_PACK(0, p)
-      SKL_FOLD_PACK(0)
-    }
-#undef SKL_LOAD_PACK
-#undef SKL_FOLD_PACK
+    for (; p < nfull; p += G) fold(ldp(p));
     // the partial last pack belongs to the thread whose stride lands on it
     if (p < npk)
       for (unsigned long long e = p * SKL_V; e < n; ++e) acc = comb(acc, skl_lift(SKL_ELEM(e)));

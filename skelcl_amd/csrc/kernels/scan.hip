@@ -37,7 +37,7 @@
 #define SKL_SCAN_ATTR
 #endif
 #ifndef SKL_LB_K
-#define SKL_LB_K 4
+#define SKL_LB_K 1
 #endif
 
 __device__ __forceinline__ void skl_scan_publish(unsigned long long* lb,
@@ -141,12 +141,21 @@ extern "C" __global__ __launch_bounds__(SKL_WG) SKL_SCAN_ATTR void skl_scan(
 #pragma unroll
       for (int k = 0; k < SKL_V; ++k) q[j][k] = skl_lift(pk[j].v[k]);
   } else {
+    SKL_T v[SKL_J][SKL_V];
 #pragma unroll
     for (int j = 0; j < SKL_J; ++j) {
       const unsigned long long e0 = wbase + ((unsigned long long)j * 64 + lane) * SKL_V;
 #pragma unroll
-      for (int k = 0; k < SKL_V; ++k)
-        q[j][k] = (e0 + k < n) ? skl_lift(in_[SKL_RIDX(e0 + k)]) : SKL_ACC_ID;
+      for (int k = 0; k < SKL_V; ++k) {
+        const unsigned long long e = e0 + k < n ? e0 + k : n - 1;
+        v[j][k] = in_[SKL_RIDX(e)];  // unconditional: all loads in flight together
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < SKL_J; ++j) {
+      const unsigned long long e0 = wbase + ((unsigned long long)j * 64 + lane) * SKL_V;
+#pragma unroll
+      for (int k = 0; k < SKL_V; ++k) q[j][k] = e0 + k < n ? skl_lift(v[j][k]) : SKL_ACC_ID;
     }
   }
 #pragma unroll

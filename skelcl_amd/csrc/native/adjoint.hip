@@ -9,6 +9,7 @@
 #include <cstdint>
 
 #include "skelcl_rt.h"
+#include "ld_util.h"
 
 namespace {
 
@@ -43,7 +44,8 @@ __global__ __launch_bounds__(256) void stencil_transpose_kernel(
   for (int idx = threadIdx.x; idx < THP * TWP; idx += 256) {
     const int tr = idx / TWP, tc = idx - tr * TWP;
     const int gr = r0 - R + tr, gc = c0 - R + tc;
-    tile[idx] = (gr >= 0 && gr < rows && gc >= 0 && gc < cols) ? g[(size_t)gr * cols + gc] : 0.0f;
+    tile[idx] = ld_or(g, (size_t)gr * cols + gc, 0, gr >= 0 && gr < rows && gc >= 0 && gc < cols,
+                      0.0f);
   }
   __syncthreads();
   const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;

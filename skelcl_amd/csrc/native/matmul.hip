@@ -22,6 +22,7 @@
 #include <cstdint>
 
 #include "skelcl_skeletons.h"
+#include "ld_util.h"
 #include "work_ticket.h"
 
 namespace {
@@ -91,7 +92,7 @@ __global__ __launch_bounds__(256) void split3_transpose_kernel(const float* __re
   const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
   for (int k = ty; k < 32; k += 8) {
     const int c = c0 + k, r = r0 + tx;
-    t[k][tx] = (c < Cc && r < R) ? X[(size_t)c * R + r] : 0.0f;
+    t[k][tx] = ld_or(X, (size_t)c * R + r, 0, c < Cc && r < R, 0.0f);
   }
   __syncthreads();
   for (int k = ty; k < 32; k += 8) {
@@ -381,9 +382,8 @@ __global__ __launch_bounds__(256) void gemm_f32_kernel(const float* __restrict__
         kk = e & 7;
       }
       const int gr = m0 + r, gk = k0 + kk;
-      float v = 0.0f;
-      if (gr < M && gk < K) v = ta ? A[(size_t)gk * M + gr] : A[(size_t)gr * K + gk];
-      As[kk][r] = v;
+      As[kk][r] = ld_or(A, ta ? (size_t)gk * M + gr : (size_t)gr * K + gk, 0, gr < M && gk < K,
+                        0.0f);
     }
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
@@ -397,9 +397,8 @@ __global__ __launch_bounds__(256) void gemm_f32_kernel(const float* __restrict__
         c = e & 127;
       }
       const int gc = n0 + c, gk = k0 + kk;
-      float v = 0.0f;
-      if (gc < N && gk < K) v = tb ? B[(size_t)gc * K + gk] : B[(size_t)gk * N + gc];
-      Bs[kk][c] = v;
+      Bs[kk][c] = ld_or(B, tb ? (size_t)gc * K + gk : (size_t)gk * N + gc, 0, gc < N && gk < K,
+                        0.0f);
     }
     __syncthreads();
 #pragma unroll
@@ -603,10 +602,10 @@ __global__ __launch_bounds__(256) void batch_gemm_kernel(const skl_gemm_desc* __
         for (int e = tid; e < T * KS; e += 256) {
           const int r = e / KS, kk = e % KS;
           const int gr = m0 + r, gk = k0 + kk;
-          Af[kk][r] = (gr < g.M && gk < g.K) ? g.A[(size_t)gr * g.K + gk] : 0.0f;
+          Af[kk][r] = ld_or(g.A, (size_t)gr * g.K + gk, 0, gr < g.M && gk < g.K, 0.0f);
           const int kb = e / T, c = e % T;
           const int gkb = k0 + kb, gc = n0 + c;
-          Bf[kb][c] = (gkb < g.K && gc < g.N) ? g.B[(size_t)gkb * g.N + gc] : 0.0f;
+          Bf[kb][c] = ld_or(g.B, (size_t)gkb * g.N + gc, 0, gkb < g.K && gc < g.N, 0.0f);
         }
         __syncthreads();
 #pragma unroll 8
@@ -648,7 +647,7 @@ __global__ __launch_bounds__(256) void batch_gemm_kernel(const skl_gemm_desc* __
         for (int e = tid; e < T * KS; e += 256) {
           const int r = e / KS, kk = e % KS;  // A row r, k kk (reads along k)
           const int gr = m0 + r, gk = k0 + kk;
-          const float a = (gr < g.M && gk < g.K) ? g.A[(size_t)gr * g.K + gk] : 0.0f;
+          const float a = ld_or(g.A, (size_t)gr * g.K + gk, 0, gr < g.M && gk < g.K, 0.0f);
           uint16_t h0, h1, h2;
           split3(a, h0, h1, h2);
           Ah[0][r][kk] = h0;
@@ -656,7 +655,7 @@ __global__ __launch_bounds__(256) void batch_gemm_kernel(const skl_gemm_desc* __
           Ah[2][r][kk] = h2;
           const int kb = e / T, c = e % T;    // B row kb, column c (reads along n)
           const int gkb = k0 + kb, gc = n0 + c;
-          const float b = (gkb < g.K && gc < g.N) ? g.B[(size_t)gkb * g.N + gc] : 0.0f;
+          const float b = ld_or(g.B, (size_t)gkb * g.N + gc, 0, gkb < g.K && gc < g.N, 0.0f);
           split3(b, h0, h1, h2);
           Bh[0][c][kb] = h0;
           Bh[1][c][kb] = h1;

@@ -30,6 +30,7 @@
 #include <cstdint>
 
 #include "skelcl_rt.h"
+#include "ld_util.h"
 #include "work_ticket.h"
 
 namespace {
@@ -272,7 +273,7 @@ __global__ __launch_bounds__(WG) void seg_scan_kernel(const float* __restrict__ 
 #pragma unroll
     for (int j = 0; j < ROWS; ++j) {
       const long long e = wbase + j * 64 + lane;
-      x[j] = e < hi ? (double)in[e] : 0.0;
+      x[j] = (double)ld_or(in, (size_t)e, (size_t)lo, e < hi, 0.0f);
     }
     double wsum = 0.0;  // sum of the rows before row j
 #pragma unroll

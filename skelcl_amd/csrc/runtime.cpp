@@ -244,6 +244,13 @@ extern "C" int skl_device_get_info(int device, skl_device_info* info)
   if (!info) return fail(SKL_ERR_INVALID, "info is NULL");
   memset(info, 0, sizeof(*info));
   snprintf(info->name, sizeof(info->name), "%s", g_props.name);
+  if (!info->name[0]) {  // some runtimes leave the marketing name empty
+    char nm[256] = {0};
+    if (hipDeviceGetName(nm, (int)sizeof(nm) - 1, g_device) == hipSuccess && nm[0])
+      snprintf(info->name, sizeof(info->name), "%s", nm);
+    else
+      snprintf(info->name, sizeof(info->name), "AMD %s", g_props.gcnArchName);
+  }
   snprintf(info->arch, sizeof(info->arch), "%s", g_props.gcnArchName);
   info->compute_units = g_props.multiProcessorCount;
   info->wavefront_size = g_props.warpSize;

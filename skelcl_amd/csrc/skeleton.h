@@ -41,7 +41,7 @@ struct skl_skeleton_s {
   int V = 1;        // elements per 16-byte pack
   int unroll = 4;   // packs in flight per thread (Map/Zip/Reduce)
   int scan_j = 4;   // packs per lane (Scan tile = 256*J*V)
-  int mo_rb = 8;    // MapOverlap rows per thread
+  int mo_rb = 4;    // MapOverlap rows per thread
   int max_blocks = 0;
 
   // scratch: [0,8) ticket, [8,12) epoch, partials / look-back words at 256
@@ -54,4 +54,6 @@ struct skl_skeleton_s {
   bool precise_add() const;
   // default Scan tile depth: about 256 bytes of accumulators per lane
   int default_scan_j() const;
+  // launch shape defaults per kind (unroll, max_blocks, scan_j, mo_rb)
+  void apply_default_shape(int cus);
 };

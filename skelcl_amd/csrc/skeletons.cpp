@@ -28,6 +28,22 @@ bool skl_skeleton_s::precise_add() const
          !types.empty() && types[0].name == "float";
 }
 
+void skl_skeleton_s::apply_default_shape(int cus)
+{
+  // measured on MI355X at 2^28 elements: the streaming Reduce is fastest
+  // with 4 work-groups per CU and 2 packs per thread per stride; Map/Zip
+  // with 8 per CU and 4 packs
+  if (kind == SKL_REDUCE) {
+    unroll = 2;
+    max_blocks = cus * 4;
+  } else {
+    unroll = 4;
+    max_blocks = cus * 8;
+  }
+  if (kind == SKL_SCAN) scan_j = default_scan_j();
+  mo_rb = 4;
+}
+
 int skl_skeleton_s::default_scan_j() const
 {
   const size_t acc = precise_add() ? 2 * sizeof(float) : (types.empty() ? 4 : types[0].size);
@@ -593,7 +609,7 @@ extern "C" int skl_skeleton_create(const skl_skeleton_desc* desc, skl_skeleton* 
     delete s;
     return st;
   }
-  s->max_blocks = cu_count() * 8;
+  s->apply_default_shape(cu_count());
   *out = s;
   return SKL_OK;
 }
