@@ -121,12 +121,17 @@ extern "C" __global__ __launch_bounds__(SKL_WG) SKL_SCAN_ATTR void skl_scan(
   const unsigned long long wbase = tbase + (unsigned long long)wv * WAVE_ELEMS;
 #ifdef SKL_SCAN_REVERSE
   // scan read from the end (the adjoint of a scan): element e of the scan
-  // is element n-1-e of the arrays; element-wise accesses only
-  const bool full = false;
+  // is element n-1-e of the arrays.  When n % V == 0 the V elements of a
+  // pack, e0 .. e0+V-1, are the aligned array pack at n-e0-V read backwards.
+  const bool full = tbase + TILE <= n && n % SKL_V == 0;
 #define SKL_RIDX(e) (n - 1 - (e))
+#define SKL_PIDX(e0) ((n - (e0) - SKL_V) / SKL_V)
+#define SKL_PCOMP(k) (SKL_V - 1 - (k))
 #else
   const bool full = tbase + TILE <= n;
 #define SKL_RIDX(e) (e)
+#define SKL_PIDX(e0) ((e0) / SKL_V)
+#define SKL_PCOMP(k) (k)
 #endif
 
   // ---- load + per-lane inclusive scan of each pack
@@ -135,11 +140,11 @@ extern "C" __global__ __launch_bounds__(SKL_WG) SKL_SCAN_ATTR void skl_scan(
     pack_t pk[SKL_J];
 #pragma unroll
     for (int j = 0; j < SKL_J; ++j)
-      pk[j] = reinterpret_cast<const pack_t*>(in_)[(wbase + ((unsigned long long)j * 64 + lane) * SKL_V) / SKL_V];
+      pk[j] = reinterpret_cast<const pack_t*>(in_)[SKL_PIDX(wbase + ((unsigned long long)j * 64 + lane) * SKL_V)];
 #pragma unroll
     for (int j = 0; j < SKL_J; ++j)
 #pragma unroll
-      for (int k = 0; k < SKL_V; ++k) q[j][k] = skl_lift(pk[j].v[k]);
+      for (int k = 0; k < SKL_V; ++k) q[j][k] = skl_lift(pk[j].v[SKL_PCOMP(k)]);
   } else {
     SKL_T v[SKL_J][SKL_V];
 #pragma unroll
@@ -275,7 +280,10 @@ extern "C" __global__ __launch_bounds__(SKL_WG) SKL_SCAN_ATTR void skl_scan(
     }
     const unsigned long long e0 = wbase + ((unsigned long long)j * 64 + lane) * SKL_V;
     if (full) {
-      reinterpret_cast<pack_t*>(out_)[e0 / SKL_V] = o;
+      pack_t w;
+#pragma unroll
+      for (int k = 0; k < SKL_V; ++k) w.v[SKL_PCOMP(k)] = o.v[k];
+      reinterpret_cast<pack_t*>(out_)[SKL_PIDX(e0)] = w;
     } else {
 #pragma unroll
       for (int k = 0; k < SKL_V; ++k)

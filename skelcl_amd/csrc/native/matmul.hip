@@ -353,16 +353,16 @@ __global__ __launch_bounds__(256, 1) void gemm_fp8_kernel(
 // FP32 VALU GEMM: 128x128 block tile, 8x8 per thread (rows ty*4+{0..3} and
 // 64+ty*4+{0..3}, same split for columns), k-slices of 8.
 constexpr int FK = 8;
-__global__ __launch_bounds__(256) void gemm_f32_kernel(const float* __restrict__ A,
-                                                       const float* __restrict__ B,
-                                                       float* __restrict__ C, int M, int N,
-                                                       int K, int ta, int tb, int ldc)
+// One 128x128 output tile at (m0, n0); shared by the single product and the
+// batched products.  Ends with a barrier, so the LDS can be reused at once.
+__device__ __forceinline__ void f32_tile(const float* __restrict__ A, const float* __restrict__ B,
+                                         float* __restrict__ C, int M, int N, int K, int ta,
+                                         int tb, int ldc, int m0, int n0,
+                                         float (*__restrict__ As)[BM + 4],
+                                         float (*__restrict__ Bs)[BN + 4])
 {
-  __shared__ float As[FK][BM + 4];
-  __shared__ float Bs[FK][BN + 4];
   const int tid = threadIdx.x;
   const int tx = tid & 15, ty = tid >> 4;
-  const int m0 = blockIdx.y * BM, n0 = blockIdx.x * BN;
   float acc[8][8];
 #pragma unroll
   for (int i = 0; i < 8; ++i)
@@ -428,6 +428,16 @@ __global__ __launch_bounds__(256) void gemm_f32_kernel(const float* __restrict__
       if (col < N) C[(size_t)row * ldc + col] = acc[i][j];
     }
   }
+}
+
+__global__ __launch_bounds__(256) void gemm_f32_kernel(const float* __restrict__ A,
+                                                       const float* __restrict__ B,
+                                                       float* __restrict__ C, int M, int N,
+                                                       int K, int ta, int tb, int ldc)
+{
+  __shared__ float As[FK][BM + 4];
+  __shared__ float Bs[FK][BN + 4];
+  f32_tile(A, B, C, M, N, K, ta, tb, ldc, blockIdx.y * BM, blockIdx.x * BN, As, Bs);
 }
 
 inline size_t round_up(size_t x, size_t m) { return (x + m - 1) / m * m; }
@@ -508,16 +518,18 @@ extern "C" int skl_internal_matmul(float* C, const float* A, const float* B, int
 }
 
 // ---------------------------------------------------------------------------
-// Batched products (many small AllPairs(+,*) on float) in one persistent
-// launch after a one-work-group prologue.  Work item = one 64x64 tile of one
-// product; persistent work-groups take items from a counter until none is
-// left.  FP32: VALU 4x4 per thread.  BF16X3: operands are split into three
-// bf16 planes while they are staged in LDS (no pre-pass), 4 waves of 32x32,
-// six MFMA products per 16x16x32 step as in gemm_bf16x3_kernel.
+// Batched products (many small AllPairs(+,*) on float) in one launch after a
+// one-work-group prologue.  Work item = one 128x128 tile of one product;
+// tiles are independent, so the work-groups take them round-robin (no
+// ticket: a shared counter would serialise one RMW per tile).
+//   FP32    the single product's tile body (f32_tile).
+//   BF16X3  the FP32 operand slices are split into three bf16 terms with the
+//           hardware round-to-nearest conversion while they are staged in
+//           LDS (no pre-pass; the next slice is loaded during the MFMAs),
+//           then the six MFMA products of gemm_bf16x3_kernel.
 namespace {
 
 struct BatchCtl {
-  unsigned long long work;
   unsigned long long total;
 };
 
@@ -533,7 +545,7 @@ __global__ __launch_bounds__(1024) void batch_prologue(const skl_gemm_desc* __re
     const int p = base + threadIdx.x;
     long long t = 0;
     if (p < count && d[p].M > 0 && d[p].N > 0)
-      t = (long long)((d[p].M + 63) / 64) * ((d[p].N + 63) / 64);
+      t = (long long)((d[p].M + BM - 1) / BM) * ((d[p].N + BN - 1) / BN);
     part[threadIdx.x] = t;
     __syncthreads();
     for (int s = 1; s < 1024; s <<= 1) {
@@ -549,7 +561,6 @@ __global__ __launch_bounds__(1024) void batch_prologue(const skl_gemm_desc* __re
   }
   if (threadIdx.x == 0) {
     tstart[count] = carry;
-    ctl->work = 0;
     ctl->total = (unsigned long long)carry;
   }
 }
@@ -565,140 +576,132 @@ __device__ __forceinline__ int find_item(const long long* ts, int count, long lo
   return lo;
 }
 
-template <bool BF16X3>
-__global__ __launch_bounds__(256) void batch_gemm_kernel(const skl_gemm_desc* __restrict__ d,
-                                                         int count,
-                                                         const long long* __restrict__ tstart,
-                                                         BatchCtl* ctl)
+__global__ __launch_bounds__(256) void batch_f32_kernel(const skl_gemm_desc* __restrict__ d,
+                                                        int count,
+                                                        const long long* __restrict__ tstart,
+                                                        const BatchCtl* __restrict__ ctl)
 {
-  constexpr int T = 64, KS = 32;
-  // FP32: As[KS][T], Bs[KS][T]; BF16X3: 3 planes x 64 rows x (32+8) bf16
-  __shared__ __attribute__((aligned(16))) float Af[KS][T + 4];
-  __shared__ __attribute__((aligned(16))) float Bf[KS][T + 4];
-  __shared__ __attribute__((aligned(16))) uint16_t Ah[3][T][KS + 8];
-  __shared__ __attribute__((aligned(16))) uint16_t Bh[3][T][KS + 8];
-  __shared__ long long s_t;
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  __shared__ float As[FK][BM + 4];
+  __shared__ float Bs[FK][BN + 4];
   const long long total = (long long)ctl->total;
-  if (tid == 0) s_t = ticket_draw(&ctl->work);
-  __syncthreads();
-  for (;;) {
-    const long long t = ticket_read(s_t);
-    __syncthreads();
-    if (t >= total) break;
+  for (long long t = blockIdx.x; t < total; t += gridDim.x) {
     const int p = find_item(tstart, count, t);
     const skl_gemm_desc g = d[p];
     const long long lt = t - tstart[p];
-    const int ntn = (g.N + T - 1) / T;
-    const int m0 = (int)(lt / ntn) * T, n0 = (int)(lt % ntn) * T;
-    if (!BF16X3) {
-      const int tx = tid & 15, ty = tid >> 4;
-      float acc[4][4];
+    const int ntn = (g.N + BN - 1) / BN;
+    f32_tile(g.A, g.B, g.C, g.M, g.N, g.K, 0, 0, g.N, (int)(lt / ntn) * BM, (int)(lt % ntn) * BN,
+             As, Bs);
+  }
+}
+
+// x = h0 + h1 + h2 (+ a remainder below 2^-24 |x|), each term the
+// round-to-nearest bf16 of what is left (v_cvt_pk_bf16_f32 on gfx950)
+__device__ __forceinline__ void split3_hw(float x, __bf16& h0, __bf16& h1, __bf16& h2)
+{
+  h0 = (__bf16)x;
+  const float r1 = x - (float)h0;
+  h1 = (__bf16)r1;
+  h2 = (__bf16)(r1 - (float)h1);
+}
+
+constexpr int BKS = 32;               // k per slice
+constexpr int BLD = BKS + 8;          // bf16 per LDS row (padding vs bank conflicts)
+
+__global__ __launch_bounds__(256) void batch_bf16x3_kernel(const skl_gemm_desc* __restrict__ d,
+                                                           int count,
+                                                           const long long* __restrict__ tstart,
+                                                           const BatchCtl* __restrict__ ctl)
+{
+  __shared__ __attribute__((aligned(16))) __bf16 As[3][BM][BLD];
+  __shared__ __attribute__((aligned(16))) __bf16 Bs[3][BN][BLD];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave >> 1, wn = wave & 1;
+  const long long total = (long long)ctl->total;
+  for (long long t = blockIdx.x; t < total; t += gridDim.x) {
+    const int p = find_item(tstart, count, t);
+    const skl_gemm_desc g = d[p];
+    const long long lt = t - tstart[p];
+    const int ntn = (g.N + BN - 1) / BN;
+    const int m0 = (int)(lt / ntn) * BM, n0 = (int)(lt % ntn) * BN;
+    f32x4 acc0[4][4], acc1[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        acc0[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+        acc1[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+      }
+    // per thread and slice: 16 elements of A (128 x 32, along k) and 16 of
+    // B (32 x 128, along n), loaded unconditionally from clamped indices
+    float ra[16], rb[16];
+    auto gload = [&](int k0) {
+#pragma unroll
+      for (int j = 0; j < 16; ++j) {
+        const int e = tid + 256 * j;
+        const int r = e >> 5, kk = e & 31;
+        const int gr = m0 + r, gk = k0 + kk;
+        ra[j] = ld_or(g.A, (size_t)gr * g.K + gk, 0, gr < g.M && gk < g.K, 0.0f);
+        const int kb = e >> 7, c = e & 127;
+        const int gkb = k0 + kb, gc = n0 + c;
+        rb[j] = ld_or(g.B, (size_t)gkb * g.N + gc, 0, gkb < g.K && gc < g.N, 0.0f);
+      }
+    };
+    auto lstore = [&]() {
+#pragma unroll
+      for (int j = 0; j < 16; ++j) {
+        const int e = tid + 256 * j;
+        const int r = e >> 5, kk = e & 31;
+        __bf16 h0, h1, h2;
+        split3_hw(ra[j], h0, h1, h2);
+        As[0][r][kk] = h0;
+        As[1][r][kk] = h1;
+        As[2][r][kk] = h2;
+        const int kb = e >> 7, c = e & 127;
+        split3_hw(rb[j], h0, h1, h2);
+        Bs[0][c][kb] = h0;
+        Bs[1][c][kb] = h1;
+        Bs[2][c][kb] = h2;
+      }
+    };
+    gload(0);
+    for (int k0 = 0; k0 < g.K; k0 += BKS) {
+      lstore();
+      __syncthreads();
+      if (k0 + BKS < g.K) gload(k0 + BKS);  // next slice in flight during the MFMAs
+      const int fr = lane & 15, fk = (lane >> 4) * 8;
+      bf16x8 a[4][3], b[4][3];
 #pragma unroll
       for (int i = 0; i < 4; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) acc[i][j] = 0.0f;
-      for (int k0 = 0; k0 < g.K; k0 += KS) {
-        for (int e = tid; e < T * KS; e += 256) {
-          const int r = e / KS, kk = e % KS;
-          const int gr = m0 + r, gk = k0 + kk;
-          Af[kk][r] = ld_or(g.A, (size_t)gr * g.K + gk, 0, gr < g.M && gk < g.K, 0.0f);
-          const int kb = e / T, c = e % T;
-          const int gkb = k0 + kb, gc = n0 + c;
-          Bf[kb][c] = ld_or(g.B, (size_t)gkb * g.N + gc, 0, gkb < g.K && gc < g.N, 0.0f);
+        for (int q = 0; q < 3; ++q) {
+          a[i][q] = *reinterpret_cast<const bf16x8*>(&As[q][wm * 64 + i * 16 + fr][fk]);
+          b[i][q] = *reinterpret_cast<const bf16x8*>(&Bs[q][wn * 64 + i * 16 + fr][fk]);
         }
-        __syncthreads();
-#pragma unroll 8
-        for (int kk = 0; kk < KS; ++kk) {
-          float a[4], b[4];
 #pragma unroll
-          for (int i = 0; i < 4; ++i) {
-            a[i] = Af[kk][ty * 4 + i];
-            b[i] = Bf[kk][tx * 4 + i];
-          }
-#pragma unroll
-          for (int i = 0; i < 4; ++i)
-#pragma unroll
-            for (int j = 0; j < 4; ++j) acc[i][j] = fmaf(a[i], b[j], acc[i][j]);
-        }
-        __syncthreads();
-      }
-#pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        const int r = m0 + ty * 4 + i;
-        if (r >= g.M) continue;
+      for (int i = 0; i < 4; ++i)
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
-          const int c = n0 + tx * 4 + j;
-          if (c < g.N) g.C[(size_t)r * g.N + c] = acc[i][j];
+          acc0[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i][0], b[j][0], acc0[i][j], 0, 0, 0);
+          acc1[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i][1], b[j][1], acc1[i][j], 0, 0, 0);
+          acc1[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i][0], b[j][2], acc1[i][j], 0, 0, 0);
+          acc1[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i][2], b[j][0], acc1[i][j], 0, 0, 0);
+          acc1[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i][0], b[j][1], acc1[i][j], 0, 0, 0);
+          acc1[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i][1], b[j][0], acc1[i][j], 0, 0, 0);
         }
-      }
-    } else {
-      const int wm = wave >> 1, wn = wave & 1;  // each wave 32x32 = 2x2 tiles
-      f32x4 acc0[2][2], acc1[2][2];
-#pragma unroll
-      for (int i = 0; i < 2; ++i)
-#pragma unroll
-        for (int j = 0; j < 2; ++j) {
-          acc0[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
-          acc1[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
-        }
-      for (int k0 = 0; k0 < g.K; k0 += KS) {
-        for (int e = tid; e < T * KS; e += 256) {
-          const int r = e / KS, kk = e % KS;  // A row r, k kk (reads along k)
-          const int gr = m0 + r, gk = k0 + kk;
-          const float a = ld_or(g.A, (size_t)gr * g.K + gk, 0, gr < g.M && gk < g.K, 0.0f);
-          uint16_t h0, h1, h2;
-          split3(a, h0, h1, h2);
-          Ah[0][r][kk] = h0;
-          Ah[1][r][kk] = h1;
-          Ah[2][r][kk] = h2;
-          const int kb = e / T, c = e % T;    // B row kb, column c (reads along n)
-          const int gkb = k0 + kb, gc = n0 + c;
-          const float b = ld_or(g.B, (size_t)gkb * g.N + gc, 0, gkb < g.K && gc < g.N, 0.0f);
-          split3(b, h0, h1, h2);
-          Bh[0][c][kb] = h0;
-          Bh[1][c][kb] = h1;
-          Bh[2][c][kb] = h2;
-        }
-        __syncthreads();
-        const int fr = lane & 15, fk = (lane >> 4) * 8;
-        bf16x8 a[2][3], b[2][3];
-#pragma unroll
-        for (int i = 0; i < 2; ++i)
-#pragma unroll
-          for (int q = 0; q < 3; ++q) {
-            a[i][q] = *reinterpret_cast<const bf16x8*>(&Ah[q][wm * 32 + i * 16 + fr][fk]);
-            b[i][q] = *reinterpret_cast<const bf16x8*>(&Bh[q][wn * 32 + i * 16 + fr][fk]);
-          }
-#pragma unroll
-        for (int i = 0; i < 2; ++i)
-#pragma unroll
-          for (int j = 0; j < 2; ++j) {
-            acc0[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i][0], b[j][0], acc0[i][j], 0, 0, 0);
-            acc1[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i][1], b[j][1], acc1[i][j], 0, 0, 0);
-            acc1[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i][0], b[j][2], acc1[i][j], 0, 0, 0);
-            acc1[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i][2], b[j][0], acc1[i][j], 0, 0, 0);
-            acc1[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i][0], b[j][1], acc1[i][j], 0, 0, 0);
-            acc1[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i][1], b[j][0], acc1[i][j], 0, 0, 0);
-          }
-        __syncthreads();
-      }
-      const int cc = lane & 15, cr = (lane >> 4) * 4;
-#pragma unroll
-      for (int i = 0; i < 2; ++i)
-#pragma unroll
-        for (int j = 0; j < 2; ++j) {
-          const int col = n0 + wn * 32 + j * 16 + cc;
-#pragma unroll
-          for (int r = 0; r < 4; ++r) {
-            const int row = m0 + wm * 32 + i * 16 + cr + r;
-            if (row < g.M && col < g.N) g.C[(size_t)row * g.N + col] = acc0[i][j][r] + acc1[i][j][r];
-          }
-        }
+      __syncthreads();
     }
-    if (tid == 0) s_t = ticket_draw(&ctl->work);
-    __syncthreads();
+    const int cc = lane & 15, cr = (lane >> 4) * 4;
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int col = n0 + wn * 64 + j * 16 + cc;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int row = m0 + wm * 64 + i * 16 + cr + r;
+          if (row < g.M && col < g.N) g.C[(size_t)row * g.N + col] = acc0[i][j][r] + acc1[i][j][r];
+        }
+      }
   }
 }
 
@@ -715,13 +718,15 @@ extern "C" int skl_internal_batched_gemm_ws(const skl_gemm_desc* descs, int coun
   BatchCtl* ctl = static_cast<BatchCtl*>(ws);
   long long* tstart = reinterpret_cast<long long*>(static_cast<char*>(ws) + 256);
   int dev = 0, cus = 256;
-  hipDeviceProp_t prop;
-  if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess)
-    cus = prop.multiProcessorCount;
+  if (hipGetDevice(&dev) == hipSuccess) {
+    int v = 0;
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && v > 0)
+      cus = v;
+  }
   batch_prologue<<<1, 1024, 0, s>>>(descs, count, tstart, ctl);
   if (precision == SKL_PREC_BF16X3)
-    batch_gemm_kernel<true><<<cus * 2, 256, 0, s>>>(descs, count, tstart, ctl);
+    batch_bf16x3_kernel<<<cus * 2, 256, 0, s>>>(descs, count, tstart, ctl);
   else
-    batch_gemm_kernel<false><<<cus * 2, 256, 0, s>>>(descs, count, tstart, ctl);
+    batch_f32_kernel<<<cus * 2, 256, 0, s>>>(descs, count, tstart, ctl);
   return st(hipGetLastError());
 }

@@ -238,6 +238,39 @@ __device__ __forceinline__ long long draw_chunk(unsigned long long* ctr,
   return -1;
 }
 
+// wave 0: lane 0 draws the next chunk (draw_chunk), then the wave finds its
+// position with a 64-way search inside the chunk's class -- one coalesced
+// load of 64 prefix values per step, one step for classes of <= 64
+// positions -- instead of a chain of dependent loads.  Writes *s_c, *s_pos.
+__device__ __forceinline__ void next_chunk(unsigned long long* ctr, const long long* cstart,
+                                           int m, long long total, int& q, long long* s_c,
+                                           int* s_pos)
+{
+  const int lane = threadIdx.x & 63;
+  long long c = -1;
+  if (lane == 0) c = draw_chunk(ctr, cstart, m, total, q);
+  c = __shfl(c, 0, 64);
+  const int cq = __shfl(q, 0, 64);
+  int pos = 0;
+  if (c >= 0) {
+    int lo = cq * m, hi = cq * m + m;  // cstart[lo] <= c < cstart[hi]
+    while (hi - lo > 1) {
+      const int step = (hi - lo + 63) / 64;
+      const int idx = lo + lane * step;
+      const bool in = idx < hi;
+      const long long v = cstart[in ? idx : lo];
+      const unsigned long long mk = __ballot(in && v <= c);  // bit 0 is always set
+      lo += (63 - __builtin_clzll(mk)) * step;
+      hi = lo + step < hi ? lo + step : hi;
+    }
+    pos = lo;
+  }
+  if (lane == 0) {
+    *s_c = c;
+    *s_pos = pos;
+  }
+}
+
 __global__ __launch_bounds__(WG) void seg_scan_kernel(const float* __restrict__ in,
                                                       const int64_t* __restrict__ off, int m,
                                                       const long long* __restrict__ cstart,
@@ -247,19 +280,20 @@ __global__ __launch_bounds__(WG) void seg_scan_kernel(const float* __restrict__ 
                                                       const SegCtl* __restrict__ ctl)
 {
   __shared__ long long s_c;
+  __shared__ int s_pos;
   __shared__ double s_wtot[WG / 64];
   __shared__ double s_carry;
   const long long total = (long long)ctl->total;
   const unsigned epoch = ctl->epoch;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  int q = blockIdx.x & (NCLASS - 1);  // thread 0's current class
-  if (threadIdx.x == 0) s_c = draw_chunk(ctr, cstart, m, total, q);
+  int q = blockIdx.x & (NCLASS - 1);  // lane 0 of wave 0: current class
+  if (wv == 0) next_chunk(ctr, cstart, m, total, q, &s_c, &s_pos);
   __syncthreads();
   for (;;) {
     const long long c = ticket_read(s_c);
+    const int pos = (int)__builtin_amdgcn_readfirstlane((unsigned)s_pos);
     __syncthreads();
     if (c < 0) break;
-    const int pos = find_position(cstart, NCLASS * m, c);
     const int s = seg_of_pos(pos, m);
     const long long cfirst = cstart[pos];
     const long long k = c - cfirst;
@@ -344,7 +378,7 @@ __global__ __launch_bounds__(WG) void seg_scan_kernel(const float* __restrict__ 
       const double p = inclusive ? inc[j] : inc[j] - x[j];
       if (e < hi) out[e] = (float)(base + p);
     }
-    if (threadIdx.x == 0) s_c = draw_chunk(ctr, cstart, m, total, q);
+    if (wv == 0) next_chunk(ctr, cstart, m, total, q, &s_c, &s_pos);
     __syncthreads();
   }
 }

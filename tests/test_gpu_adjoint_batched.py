@@ -49,7 +49,7 @@ def test_reduce_backward_broadcast(skl):
     assert d.shape == (12345,) and np.all(d == np.float32(0.75))
 
 
-@pytest.mark.parametrize("n", [1, 1000, 4097, 300001])
+@pytest.mark.parametrize("n", [1, 1000, 4097, 300001, 1 << 20, 65540])
 @pytest.mark.parametrize("inclusive", [False, True])
 def test_scan_backward_reversed_scan(skl, n, inclusive):
     s = skl.Scan("float(float)", "float func(float x, float y){ return x+y; }", "0.0f")
@@ -226,6 +226,32 @@ def test_batched_matmul(skl, prec):
     for dC, M, N, ref in refs:
         c = dC.to_numpy(np.float32, M * N).reshape(M, N)
         assert np.linalg.norm(c - ref) / np.linalg.norm(ref) < 1e-6
+
+
+@pytest.mark.parametrize("prec", [1, 2])
+def test_batched_matmul_ragged_shapes(skl, prec):
+    """Partial 128x128 tiles, K not a multiple of the 32-deep slice, and
+    degenerate 1x1x1 products in one batch."""
+    from skelcl_amd import _lib
+    shapes = [(1, 1, 1), (3, 130, 7), (200, 1, 65), (129, 257, 33), (64, 64, 1), (17, 300, 95)]
+    descs = (_lib.skl_gemm_desc * len(shapes))()
+    keep, refs = [], []
+    for i, (M, N, K) in enumerate(shapes):
+        A = oracle.gen_f32_sym(0x5100 + 2 * i, M * K).reshape(M, K)
+        B = oracle.gen_f32_sym(0x5101 + 2 * i, K * N).reshape(K, N)
+        dA, dB = skl.DeviceBuffer.from_array(A), skl.DeviceBuffer.from_array(B)
+        dC = skl.DeviceBuffer(4 * M * N)
+        keep += [dA, dB, dC]
+        descs[i].A, descs[i].B, descs[i].C = dA.value, dB.value, dC.value
+        descs[i].M, descs[i].N, descs[i].K = M, N, K
+        refs.append((dC, M, N, A.astype(np.float64) @ B.astype(np.float64)))
+    dd = skl.DeviceBuffer(C.sizeof(descs))
+    _lib.check(_lib.lib().skl_memcpy_h2d(dd.ptr, 0, C.cast(descs, C.c_void_p), C.sizeof(descs),
+                                         None, 1))
+    _lib.check(_lib.lib().skl_matmul_batched_f32(dd.ptr, len(shapes), prec, None))
+    for dC, M, N, ref in refs:
+        c = dC.to_numpy(np.float32, M * N).reshape(M, N)
+        assert np.linalg.norm(c - ref) / max(np.linalg.norm(ref), 1e-30) < 1e-6, (M, N)
 
 
 def test_graph_capture_zip_reduce(skl):
