@@ -31,7 +31,7 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 N = 1 << 28
 SEED_SCAN, SEED_RED, SEED_X, SEED_Y = 0x2001, 0x2002, 0x2003, 0x2004
-A = 2.5
+SAXPY_A = 2.5
 
 
 def cpu_baseline(nthreads: int):
@@ -53,7 +53,7 @@ def cpu_baseline(nthreads: int):
     red = oracle.reduce_add_f32_reforder(xr, 256, nthreads)
     t["reduce"] = time.perf_counter() - t0
     t0 = time.perf_counter()
-    sax = oracle.saxpy_f32(A, x, y, nthreads)
+    sax = oracle.saxpy_f32(SAXPY_A, x, y, nthreads)
     t["saxpy"] = time.perf_counter() - t0
     ref64 = oracle.sum_f64(xr)
     total = sum(t.values())
@@ -148,6 +148,25 @@ def run_extras(skelcl, L, ck, stream, args):
         "int_reduce_exact": bool(np.uint32(got_int & 0xFFFFFFFF)
                                  == oracle.reduce_add_u32_reforder(k1)),
     }
+
+    # ---- host buffers handed over: saxpy 2^26 through the Vector API
+    # (upload, kernel, download; pageable numpy memory), wall clock
+    n2 = 1 << 26
+    hx, hy = oracle.gen_f32_sym(SEED_X, n2), oracle.gen_f32_sym(SEED_Y, n2)
+    sax_v = skelcl.Zip("float(float, float)",
+                       "float func(float x, float y, float a){ return a*x+y; }")
+    np.asarray(sax_v(skelcl.Vector(hx[:1024]), skelcl.Vector(hy[:1024]), np.float32(SAXPY_A)))
+    walls = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        res = np.asarray(sax_v(skelcl.Vector(hx), skelcl.Vector(hy), np.float32(SAXPY_A)))
+        walls.append(time.perf_counter() - t0)
+    w = min(walls)
+    ex["saxpy_2^26_host_buffers"] = {
+        "wall_ms": round(w * 1e3, 3), "GBps_algorithmic": round(12 * n2 / w / 1e9, 1),
+        "bit_exact_vs_fma": bool(np.array_equal(res, oracle.saxpy_f32_fma(SAXPY_A, hx, hy))),
+    }
+    del hx, hy, res
 
     # ---- configuration 3: MapOverlap 8192^2 Gaussian R=5 NEUTRAL + adjoint
     R, S = 5, 8192
@@ -353,7 +372,7 @@ def main() -> int:
     red = skelcl.Reduce("float(float)", "float func(float x, float y){ return x+y; }", "0")
     sax = skelcl.Zip("float(float, float)",
                      "float func(float x, float y, float a){ return a*x+y; }")
-    a_arg, n_a, keep, _ = sax._marshal((np.float32(A),))
+    a_arg, n_a, keep, _ = sax._marshal((np.float32(SAXPY_A),))
     shapes = {}
     if args.autotune:
         # launch shapes measured once per (skeleton, size class, device) and
