@@ -25,12 +25,12 @@ extern "C" __global__ __launch_bounds__(SKL_WG) void skl_map(
   const unsigned long long npk = n / SKL_V;
   constexpr unsigned long long U = SKL_UNROLL, W = SKL_WG;
   const unsigned long long step = (unsigned long long)gridDim.x * W * U;
-  auto ld = [&](unsigned long long p) { return reinterpret_cast<const pin_t*>(in_)[p]; };
+  auto ld = [&](unsigned long long p) { return skl_load_pack(reinterpret_cast<const pin_t*>(in_) + p); };
   auto emit = [&](unsigned long long p, const pin_t& x) {
     pout_t r;
 #pragma unroll
     for (int k = 0; k < SKL_V; ++k) r.v[k] = SKL_CALL1(x.v[k]);
-    reinterpret_cast<pout_t*>(out_)[p] = r;
+    skl_store_pack(reinterpret_cast<pout_t*>(out_) + p, r);
   };
   unsigned long long p0 = (unsigned long long)blockIdx.x * W * U + threadIdx.x;
   // full batches: unconditional loads, the next batch's loads issued before
@@ -178,8 +178,10 @@ extern "C" __global__ __launch_bounds__(SKL_WG) void skl_zip(
     pout_t r;
 #pragma unroll
     for (int k = 0; k < SKL_V; ++k) r.v[k] = SKL_CALL2(x.v[k], y.v[k]);
-    reinterpret_cast<pout_t*>(out_)[p] = r;
+    skl_store_pack(reinterpret_cast<pout_t*>(out_) + p, r);
   };
+  auto ldl = [&](unsigned long long p) { return skl_load_pack(reinterpret_cast<const pl_t*>(l_) + p); };
+  auto ldr = [&](unsigned long long p) { return skl_load_pack(reinterpret_cast<const pr_t*>(r_) + p); };
   unsigned long long p0 = (unsigned long long)blockIdx.x * W * U + threadIdx.x;
   // full batches: unconditional loads, the next batch's loads issued before
   // this batch's stores
@@ -188,14 +190,14 @@ extern "C" __global__ __launch_bounds__(SKL_WG) void skl_zip(
     pr_t b[SKL_UNROLL], nb[SKL_UNROLL];
 #pragma unroll
     for (int u = 0; u < SKL_UNROLL; ++u) {
-      a[u] = reinterpret_cast<const pl_t*>(l_)[p0 + u * W];
-      b[u] = reinterpret_cast<const pr_t*>(r_)[p0 + u * W];
+      a[u] = ldl(p0 + u * W);
+      b[u] = ldr(p0 + u * W);
     }
     for (; p0 + step + (U - 1) * W < npk; p0 += step) {
 #pragma unroll
       for (int u = 0; u < SKL_UNROLL; ++u) {
-        na[u] = reinterpret_cast<const pl_t*>(l_)[p0 + step + u * W];
-        nb[u] = reinterpret_cast<const pr_t*>(r_)[p0 + step + u * W];
+        na[u] = ldl(p0 + step + u * W);
+        nb[u] = ldr(p0 + step + u * W);
       }
 #pragma unroll
       for (int u = 0; u < SKL_UNROLL; ++u) emit(p0 + u * W, a[u], b[u]);

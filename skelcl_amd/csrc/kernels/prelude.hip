@@ -296,6 +296,50 @@ __device__ __forceinline__ T skl_shfl(const T& v, int src)
 template <typename T, int N> struct alignas(N * sizeof(T)) skl_pack { T v[N]; };
 template <typename T> struct skl_pack<T, 1> { T v[1]; };
 
+// Streaming pack load / store.  With SKL_NT_LOAD / SKL_NT_STORE defined (a
+// build option) 16- and 8-byte packs go through the non-temporal forms.
+template <typename P>
+__device__ __forceinline__ P skl_load_pack(const P* src)
+{
+#ifdef SKL_NT_LOAD
+  if constexpr (sizeof(P) == 16) {
+    typedef unsigned int u4 __attribute__((ext_vector_type(4)));
+    const u4 t = __builtin_nontemporal_load(reinterpret_cast<const u4*>(src));
+    P v;
+    __builtin_memcpy(&v, &t, 16);
+    return v;
+  } else if constexpr (sizeof(P) == 8) {
+    typedef unsigned int u2 __attribute__((ext_vector_type(2)));
+    const u2 t = __builtin_nontemporal_load(reinterpret_cast<const u2*>(src));
+    P v;
+    __builtin_memcpy(&v, &t, 8);
+    return v;
+  }
+#endif
+  return *src;
+}
+
+template <typename P>
+__device__ __forceinline__ void skl_store_pack(P* dst, const P& v)
+{
+#ifdef SKL_NT_STORE
+  if constexpr (sizeof(P) == 16) {
+    typedef unsigned int u4 __attribute__((ext_vector_type(4)));
+    u4 t;
+    __builtin_memcpy(&t, &v, 16);
+    __builtin_nontemporal_store(t, reinterpret_cast<u4*>(dst));
+    return;
+  } else if constexpr (sizeof(P) == 8) {
+    typedef unsigned int u2 __attribute__((ext_vector_type(2)));
+    u2 t;
+    __builtin_memcpy(&t, &v, 8);
+    __builtin_nontemporal_store(t, reinterpret_cast<u2*>(dst));
+    return;
+  }
+#endif
+  *dst = v;
+}
+
 // Double-float pair used by the compensated float paths.
 struct skl_df {
   float hi, lo;

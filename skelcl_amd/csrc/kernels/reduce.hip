@@ -120,8 +120,8 @@ extern "C" __global__ __launch_bounds__(SKL_WG) void skl_reduce(
     };
     auto ldp = [&](unsigned long long q) {
       pk_t x;
-      x.l = reinterpret_cast<const pl_t*>(l_)[q];
-      x.r = reinterpret_cast<const pr_t*>(r_)[q];
+      x.l = skl_load_pack(reinterpret_cast<const pl_t*>(l_) + q);
+      x.r = skl_load_pack(reinterpret_cast<const pr_t*>(r_) + q);
       return x;
     };
     auto fold = [&](const pk_t& x) {
@@ -130,7 +130,7 @@ extern "C" __global__ __launch_bounds__(SKL_WG) void skl_reduce(
     };
 #else
     typedef skl_pack<SKL_T, SKL_V> pk_t;
-    auto ldp = [&](unsigned long long q) { return reinterpret_cast<const pk_t*>(in_)[q]; };
+    auto ldp = [&](unsigned long long q) { return skl_load_pack(reinterpret_cast<const pk_t*>(in_) + q); };
     auto fold = [&](const pk_t& x) {
 #pragma unroll
       for (int k = 0; k < SKL_V; ++k) acc = comb(acc, skl_lift(x.v[k]));

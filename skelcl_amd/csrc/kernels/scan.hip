@@ -140,7 +140,7 @@ extern "C" __global__ __launch_bounds__(SKL_WG) SKL_SCAN_ATTR void skl_scan(
     pack_t pk[SKL_J];
 #pragma unroll
     for (int j = 0; j < SKL_J; ++j)
-      pk[j] = reinterpret_cast<const pack_t*>(in_)[SKL_PIDX(wbase + ((unsigned long long)j * 64 + lane) * SKL_V)];
+      pk[j] = skl_load_pack(reinterpret_cast<const pack_t*>(in_) + SKL_PIDX(wbase + ((unsigned long long)j * 64 + lane) * SKL_V));
 #pragma unroll
     for (int j = 0; j < SKL_J; ++j)
 #pragma unroll
@@ -283,7 +283,7 @@ extern "C" __global__ __launch_bounds__(SKL_WG) SKL_SCAN_ATTR void skl_scan(
       pack_t w;
 #pragma unroll
       for (int k = 0; k < SKL_V; ++k) w.v[SKL_PCOMP(k)] = o.v[k];
-      reinterpret_cast<pack_t*>(out_)[SKL_PIDX(e0)] = w;
+      skl_store_pack(reinterpret_cast<pack_t*>(out_) + SKL_PIDX(e0), w);
     } else {
 #pragma unroll
       for (int k = 0; k < SKL_V; ++k)
