@@ -382,8 +382,8 @@ __device__ __forceinline__ void f32_tile(const float* __restrict__ A, const floa
         kk = e & 7;
       }
       const int gr = m0 + r, gk = k0 + kk;
-      As[kk][r] = ld_or(A, ta ? (size_t)gk * M + gr : (size_t)gr * K + gk, 0, gr < M && gk < K,
-                        0.0f);
+      As[kk][r] = ld_or_g(A, ta ? (size_t)gk * M + gr : (size_t)gr * K + gk, 0, gr < M && gk < K,
+                          0.0f);
     }
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
@@ -397,8 +397,8 @@ __device__ __forceinline__ void f32_tile(const float* __restrict__ A, const floa
         c = e & 127;
       }
       const int gc = n0 + c, gk = k0 + kk;
-      Bs[kk][c] = ld_or(B, tb ? (size_t)gc * K + gk : (size_t)gk * N + gc, 0, gc < N && gk < K,
-                        0.0f);
+      Bs[kk][c] = ld_or_g(B, tb ? (size_t)gc * K + gk : (size_t)gk * N + gc, 0, gc < N && gk < K,
+                          0.0f);
     }
     __syncthreads();
 #pragma unroll
@@ -425,7 +425,7 @@ __device__ __forceinline__ void f32_tile(const float* __restrict__ A, const floa
 #pragma unroll
     for (int j = 0; j < 8; ++j) {
       const int col = n0 + (j < 4 ? tx * 4 + j : 64 + tx * 4 + j - 4);
-      if (col < N) C[(size_t)row * ldc + col] = acc[i][j];
+      if (col < N) as_global(C)[(size_t)row * ldc + col] = acc[i][j];
     }
   }
 }
@@ -631,36 +631,48 @@ __global__ __launch_bounds__(256) void batch_bf16x3_kernel(const skl_gemm_desc* 
         acc0[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
         acc1[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
       }
-    // per thread and slice: 16 elements of A (128 x 32, along k) and 16 of
-    // B (32 x 128, along n), loaded unconditionally from clamped indices
-    float ra[16], rb[16];
+    // per thread and slice: 4 groups of 4 consecutive k of A (128 x 32) and
+    // of B (32 x 128), loaded unconditionally from clamped indices; each
+    // group's three bf16 terms go to LDS as one 8-byte store per plane
+    float ra[4][4], rb[4][4];
     auto gload = [&](int k0) {
 #pragma unroll
-      for (int j = 0; j < 16; ++j) {
+      for (int j = 0; j < 4; ++j) {
         const int e = tid + 256 * j;
-        const int r = e >> 5, kk = e & 31;
-        const int gr = m0 + r, gk = k0 + kk;
-        ra[j] = ld_or(g.A, (size_t)gr * g.K + gk, 0, gr < g.M && gk < g.K, 0.0f);
-        const int kb = e >> 7, c = e & 127;
-        const int gkb = k0 + kb, gc = n0 + c;
-        rb[j] = ld_or(g.B, (size_t)gkb * g.N + gc, 0, gkb < g.K && gc < g.N, 0.0f);
+        const int r = e >> 3, kq = (e & 7) * 4;  // A: row r, k kq..kq+3
+        const int n = e & 127, kb = (e >> 7) * 4;  // B: column n, k kb..kb+3
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const int gr = m0 + r, gk = k0 + kq + i;
+          ra[j][i] = ld_or_g(g.A, (size_t)gr * g.K + gk, 0, gr < g.M && gk < g.K, 0.0f);
+          const int gkb = k0 + kb + i, gc = n0 + n;
+          rb[j][i] = ld_or_g(g.B, (size_t)gkb * g.N + gc, 0, gkb < g.K && gc < g.N, 0.0f);
+        }
       }
+    };
+    auto put4 = [&](__bf16 (*plane)[BLD], int row, int k, const float* v, int part) {
+      __bf16 h[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        __bf16 h0, h1, h2;
+        split3_hw(v[i], h0, h1, h2);
+        h[i] = part == 0 ? h0 : (part == 1 ? h1 : h2);
+      }
+      unsigned long long w;
+      __builtin_memcpy(&w, h, 8);
+      *reinterpret_cast<unsigned long long*>(&plane[row][k]) = w;
     };
     auto lstore = [&]() {
 #pragma unroll
-      for (int j = 0; j < 16; ++j) {
+      for (int j = 0; j < 4; ++j) {
         const int e = tid + 256 * j;
-        const int r = e >> 5, kk = e & 31;
-        __bf16 h0, h1, h2;
-        split3_hw(ra[j], h0, h1, h2);
-        As[0][r][kk] = h0;
-        As[1][r][kk] = h1;
-        As[2][r][kk] = h2;
-        const int kb = e >> 7, c = e & 127;
-        split3_hw(rb[j], h0, h1, h2);
-        Bs[0][c][kb] = h0;
-        Bs[1][c][kb] = h1;
-        Bs[2][c][kb] = h2;
+        const int r = e >> 3, kq = (e & 7) * 4;
+        const int n = e & 127, kb = (e >> 7) * 4;
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+          put4(As[q], r, kq, ra[j], q);
+          put4(Bs[q], n, kb, rb[j], q);
+        }
       }
     };
     gload(0);
@@ -699,7 +711,7 @@ __global__ __launch_bounds__(256) void batch_bf16x3_kernel(const skl_gemm_desc* 
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
           const int row = m0 + wm * 64 + i * 16 + cr + r;
-          if (row < g.M && col < g.N) g.C[(size_t)row * g.N + col] = acc0[i][j][r] + acc1[i][j][r];
+          if (row < g.M && col < g.N) as_global(g.C)[(size_t)row * g.N + col] = acc0[i][j][r] + acc1[i][j][r];
         }
       }
   }

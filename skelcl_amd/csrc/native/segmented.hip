@@ -303,16 +303,17 @@ __global__ __launch_bounds__(WG) void seg_scan_kernel(const float* __restrict__ 
     const long long wbase = lo + (long long)wv * (64 * ROWS);
 
     // row j of this wave = 64 consecutive elements, one per lane
-    double x[ROWS], inc[ROWS];
+    float x[ROWS];
+    double inc[ROWS];
 #pragma unroll
     for (int j = 0; j < ROWS; ++j) {
       const long long e = wbase + j * 64 + lane;
-      x[j] = (double)ld_or(in, (size_t)e, (size_t)lo, e < hi, 0.0f);
+      x[j] = ld_or(in, (size_t)e, (size_t)lo, e < hi, 0.0f);
     }
     double wsum = 0.0;  // sum of the rows before row j
 #pragma unroll
     for (int j = 0; j < ROWS; ++j) {
-      double v = x[j];
+      double v = (double)x[j];
 #pragma unroll
       for (int o = 1; o < 64; o <<= 1) {
         const double t = __shfl_up(v, o, 64);
@@ -375,7 +376,7 @@ __global__ __launch_bounds__(WG) void seg_scan_kernel(const float* __restrict__ 
 #pragma unroll
     for (int j = 0; j < ROWS; ++j) {
       const long long e = wbase + j * 64 + lane;
-      const double p = inclusive ? inc[j] : inc[j] - x[j];
+      const double p = inclusive ? inc[j] : inc[j] - (double)x[j];
       if (e < hi) out[e] = (float)(base + p);
     }
     if (wv == 0) next_chunk(ctr, cstart, m, total, q, &s_c, &s_pos);
