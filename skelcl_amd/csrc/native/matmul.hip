@@ -109,6 +109,29 @@ __global__ __launch_bounds__(256) void split3_transpose_kernel(const float* __re
 }
 
 // ---------------------------------------------------------------------------
+// XCD-aware tile order.  Work-groups are dispatched round-robin over the 8
+// XCDs (work-group b runs on XCD b % 8), each with its own L2.  Linear
+// work-group b is given tile xcd * (T / 8) + b / 8, so each XCD walks a
+// contiguous range of tiles; inside a range tiles go 8 M-tiles at a time
+// (column-major inside the group), so the ~32 tiles an XCD runs at once share
+// their A row-panels and B column-panels in its L2.  T % 8 != 0 keeps the
+// plain order (the map must be a bijection).
+__device__ __forceinline__ void tile_coords(int& tm, int& tn)
+{
+  const int ntn = (int)gridDim.x, ntm = (int)gridDim.y;
+  const int T = ntm * ntn;
+  int t = (int)(blockIdx.y * gridDim.x + blockIdx.x);
+  if ((T & 7) == 0) t = (t & 7) * (T >> 3) + (t >> 3);
+  constexpr int GM = 8;
+  const int group = t / (GM * ntn);
+  const int first = group * GM;
+  const int gsize = ntm - first < GM ? ntm - first : GM;
+  const int r = t - group * GM * ntn;
+  tm = first + r % gsize;
+  tn = r / gsize;
+}
+
+// ---------------------------------------------------------------------------
 // bf16x3 GEMM.  Ap: 3 planes Mp x Kp, Bp: 3 planes Np x Kp (B^T), Kp % 32 == 0,
 // Mp % 128 == 0, Np % 128 == 0.
 constexpr int BK = 32;
@@ -122,7 +145,9 @@ __global__ __launch_bounds__(256, 1) void gemm_bf16x3_kernel(
   __shared__ __attribute__((aligned(16))) uint16_t Bs[3][BN][LDS_STRIDE];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = wave >> 1, wn = wave & 1;
-  const int m0 = blockIdx.y * BM, n0 = blockIdx.x * BN;
+  int tm, tn;
+  tile_coords(tm, tn);
+  const int m0 = tm * BM, n0 = tn * BN;
   const size_t pa = (size_t)Mp * Kp, pb = (size_t)Np * Kp;
 
   f32x4 acc0[4][4], acc1[4][4];
@@ -285,7 +310,9 @@ __global__ __launch_bounds__(256, 1) void gemm_fp8_kernel(
   __shared__ __attribute__((aligned(16))) uint8_t Bs[BN][STR];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = wave >> 1, wn = wave & 1;
-  const int m0 = blockIdx.y * BM, n0 = blockIdx.x * BN;
+  int tm, tn;
+  tile_coords(tm, tn);
+  const int m0 = tm * BM, n0 = tn * BN;
   const int nkb = Kp / KB;
   f32x4 acc[4][4];
 #pragma unroll
@@ -437,7 +464,9 @@ __global__ __launch_bounds__(256) void gemm_f32_kernel(const float* __restrict__
 {
   __shared__ float As[FK][BM + 4];
   __shared__ float Bs[FK][BN + 4];
-  f32_tile(A, B, C, M, N, K, ta, tb, ldc, blockIdx.y * BM, blockIdx.x * BN, As, Bs);
+  int tm, tn;
+  tile_coords(tm, tn);
+  f32_tile(A, B, C, M, N, K, ta, tb, ldc, tm * BM, tn * BN, As, Bs);
 }
 
 inline size_t round_up(size_t x, size_t m) { return (x + m - 1) / m * m; }
