@@ -395,39 +395,63 @@ __device__ __forceinline__ void f32_tile(const float* __restrict__ A, const floa
   for (int i = 0; i < 8; ++i)
 #pragma unroll
     for (int j = 0; j < 8; ++j) acc[i][j] = 0.0f;
+  // slice staging: A 128 rows x 8 k and B 8 k x 128 columns, 4 + 4 elements
+  // per thread.  The next slice is loaded into registers (raw, from clamped
+  // indices) while the current one is multiplied; masks apply at the store.
+  auto a_at = [&](int j, int k0, int& r, int& kk, bool& ok) -> size_t {
+    const int e = tid + 256 * j;
+    if (ta) {  // A stored K x M: read along m
+      kk = e >> 7;
+      r = e & 127;
+    } else {
+      r = e >> 3;
+      kk = e & 7;
+    }
+    const int gr = m0 + r, gk = k0 + kk;
+    ok = gr < M && gk < K;
+    return ta ? (size_t)gk * M + gr : (size_t)gr * K + gk;
+  };
+  auto b_at = [&](int j, int k0, int& c, int& kk, bool& ok) -> size_t {
+    const int e = tid + 256 * j;
+    if (tb) {  // B stored N x K: read along k
+      c = e >> 3;
+      kk = e & 7;
+    } else {
+      kk = e >> 7;
+      c = e & 127;
+    }
+    const int gc = n0 + c, gk = k0 + kk;
+    ok = gc < N && gk < K;
+    return tb ? (size_t)gc * K + gk : (size_t)gk * N + gc;
+  };
+  float pa[4], pb[4];
+  auto gl = [&](int k0) {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      int r, kk, c;
+      bool ok;
+      const size_t ia = a_at(j, k0, r, kk, ok);
+      pa[j] = as_global(A)[ok ? ia : 0];
+      const size_t ib = b_at(j, k0, c, kk, ok);
+      pb[j] = as_global(B)[ok ? ib : 0];
+    }
+  };
+  auto st = [&](int k0) {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      int r, kk, c;
+      bool ok;
+      a_at(j, k0, r, kk, ok);
+      As[kk][r] = ok ? pa[j] : 0.0f;
+      b_at(j, k0, c, kk, ok);
+      Bs[kk][c] = ok ? pb[j] : 0.0f;
+    }
+  };
+  gl(0);
   for (int k0 = 0; k0 < K; k0 += FK) {
-    // A slice: 128 rows x 8 k (4 per thread)
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const int e = tid + 256 * j;
-      int r, kk;
-      if (ta) {  // A stored K x M: read along m
-        kk = e >> 7;
-        r = e & 127;
-      } else {
-        r = e >> 3;
-        kk = e & 7;
-      }
-      const int gr = m0 + r, gk = k0 + kk;
-      As[kk][r] = ld_or_g(A, ta ? (size_t)gk * M + gr : (size_t)gr * K + gk, 0, gr < M && gk < K,
-                          0.0f);
-    }
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const int e = tid + 256 * j;
-      int c, kk;
-      if (tb) {  // B stored N x K: read along k
-        c = e >> 3;
-        kk = e & 7;
-      } else {
-        kk = e >> 7;
-        c = e & 127;
-      }
-      const int gc = n0 + c, gk = k0 + kk;
-      Bs[kk][c] = ld_or_g(B, tb ? (size_t)gc * K + gk : (size_t)gk * N + gc, 0, gc < N && gk < K,
-                          0.0f);
-    }
+    st(k0);
     __syncthreads();
+    if (k0 + FK < K) gl(k0 + FK);
 #pragma unroll
     for (int kk = 0; kk < FK; ++kk) {
       float a[8], b[8];
@@ -661,21 +685,24 @@ __global__ __launch_bounds__(256) void batch_bf16x3_kernel(const skl_gemm_desc* 
         acc1[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
       }
     // per thread and slice: 4 groups of 4 consecutive k of A (128 x 32) and
-    // of B (32 x 128), loaded unconditionally from clamped indices; each
-    // group's three bf16 terms go to LDS as one 8-byte store per plane
+    // of B (32 x 128).  The prefetch loads go straight into ra / rb from
+    // clamped indices (a select right after a load makes the compiler wait
+    // for it); the out-of-range elements are zeroed when the slice is
+    // written to LDS, each group's three bf16 terms as one 8-byte store per
+    // plane.
     float ra[4][4], rb[4][4];
     auto gload = [&](int k0) {
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
         const int e = tid + 256 * j;
-        const int r = e >> 3, kq = (e & 7) * 4;  // A: row r, k kq..kq+3
+        const int r = e >> 3, kq = (e & 7) * 4;    // A: row r, k kq..kq+3
         const int n = e & 127, kb = (e >> 7) * 4;  // B: column n, k kb..kb+3
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
           const int gr = m0 + r, gk = k0 + kq + i;
-          ra[j][i] = ld_or_g(g.A, (size_t)gr * g.K + gk, 0, gr < g.M && gk < g.K, 0.0f);
+          ra[j][i] = as_global(g.A)[gr < g.M && gk < g.K ? (size_t)gr * g.K + gk : 0];
           const int gkb = k0 + kb + i, gc = n0 + n;
-          rb[j][i] = ld_or_g(g.B, (size_t)gkb * g.N + gc, 0, gkb < g.K && gc < g.N, 0.0f);
+          rb[j][i] = as_global(g.B)[gkb < g.K && gc < g.N ? (size_t)gkb * g.N + gc : 0];
         }
       }
     };
@@ -691,22 +718,28 @@ __global__ __launch_bounds__(256) void batch_bf16x3_kernel(const skl_gemm_desc* 
       __builtin_memcpy(&w, h, 8);
       *reinterpret_cast<unsigned long long*>(&plane[row][k]) = w;
     };
-    auto lstore = [&]() {
+    auto lstore = [&](int k0) {
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
         const int e = tid + 256 * j;
         const int r = e >> 3, kq = (e & 7) * 4;
         const int n = e & 127, kb = (e >> 7) * 4;
+        float va[4], vb[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          va[i] = m0 + r < g.M && k0 + kq + i < g.K ? ra[j][i] : 0.0f;
+          vb[i] = k0 + kb + i < g.K && n0 + n < g.N ? rb[j][i] : 0.0f;
+        }
 #pragma unroll
         for (int q = 0; q < 3; ++q) {
-          put4(As[q], r, kq, ra[j], q);
-          put4(Bs[q], n, kb, rb[j], q);
+          put4(As[q], r, kq, va, q);
+          put4(Bs[q], n, kb, vb, q);
         }
       }
     };
     gload(0);
     for (int k0 = 0; k0 < g.K; k0 += BKS) {
-      lstore();
+      lstore(k0);
       __syncthreads();
       if (k0 + BKS < g.K) gload(k0 + BKS);  // next slice in flight during the MFMAs
       const int fr = lane & 15, fk = (lane >> 4) * 8;
