@@ -11,9 +11,9 @@
 //          with i+j <= 2 (x0y0 into one FP32 accumulator, the five correction
 //          products into a second one) and adds the two at the end.
 //  FP8     MFMA v_mfma_scale_f32_16x16x128_f8f6f4 on OCP e4m3 operands, each
-//          128x128 tile of A and of B^T quantised with its own scale
-//          absmax/448; every 128-deep k-block product is rescaled by
-//          sA*sB into the FP32 accumulator.
+//          128x128 tile of A and of B^T quantised with its own power-of-two
+//          scale (the smallest >= absmax/448); the scales go to the MFMA as
+//          its E8M0 block scales, so it accumulates directly in FP32.
 //
 // Block tile 128x128, 256 threads = 4 waves as 2x2, each wave 64x64 = 4x4
 // MFMA tiles of 16x16.  blockIdx.x walks N, blockIdx.y walks M.
@@ -137,12 +137,31 @@ __device__ __forceinline__ void tile_coords(int& tm, int& tn)
 constexpr int BK = 32;
 constexpr int LDS_STRIDE = BK + 8;  // bf16 per LDS row (padding vs bank conflicts)
 
+// Six bf16 products of one k-step, ordered so that consecutive MFMAs write
+// different accumulators: each product type sweeps all 16 tiles before the
+// next, so a dependent MFMA is 16 instructions behind its predecessor.
+#define SKL_BF16X3_STEP(acc0, acc1, a, b)                                           \
+  do {                                                                               \
+    _Pragma("unroll") for (int i_ = 0; i_ < 4; ++i_)                                 \
+      _Pragma("unroll") for (int j_ = 0; j_ < 4; ++j_)                               \
+        acc0[i_][j_] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i_][0], b[j_][0],   \
+                                                               acc0[i_][j_], 0, 0, 0); \
+    constexpr int pa_[5] = {1, 0, 2, 0, 1}, pb_[5] = {1, 2, 0, 1, 0};                \
+    _Pragma("unroll") for (int t_ = 0; t_ < 5; ++t_)                                 \
+      _Pragma("unroll") for (int i_ = 0; i_ < 4; ++i_)                               \
+        _Pragma("unroll") for (int j_ = 0; j_ < 4; ++j_)                             \
+          acc1[i_][j_] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(                    \
+              a[i_][pa_[t_]], b[j_][pb_[t_]], acc1[i_][j_], 0, 0, 0);                \
+  } while (0)
+
 __global__ __launch_bounds__(256, 1) void gemm_bf16x3_kernel(
     const uint16_t* __restrict__ Ap, const uint16_t* __restrict__ Bp, float* __restrict__ C,
     int M, int N, int Mp, int Np, int Kp, int ldc)
 {
-  __shared__ __attribute__((aligned(16))) uint16_t As[3][BM][LDS_STRIDE];
-  __shared__ __attribute__((aligned(16))) uint16_t Bs[3][BN][LDS_STRIDE];
+  // double-buffered slices: slice k+1 is written to the other buffer while
+  // slice k is multiplied, one barrier per k-step
+  __shared__ __attribute__((aligned(16))) uint16_t As[2][3][BM][LDS_STRIDE];
+  __shared__ __attribute__((aligned(16))) uint16_t Bs[2][3][BN][LDS_STRIDE];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = wave >> 1, wn = wave & 1;
   int tm, tn;
@@ -171,41 +190,37 @@ __global__ __launch_bounds__(256, 1) void gemm_bf16x3_kernel(
       rb[j] = *reinterpret_cast<const i32x4*>(Bp + p * pb + (size_t)(n0 + r) * Kp + k0 + kc * 8);
     }
   };
-  auto lstore = [&]() {
+  auto lstore = [&](int buf) {
 #pragma unroll
     for (int j = 0; j < 6; ++j) {
       const int c = tid + 256 * j;
       const int p = c >> 9, r = (c >> 2) & 127, kc = c & 3;
-      *reinterpret_cast<i32x4*>(&As[p][r][kc * 8]) = ra[j];
-      *reinterpret_cast<i32x4*>(&Bs[p][r][kc * 8]) = rb[j];
+      *reinterpret_cast<i32x4*>(&As[buf][p][r][kc * 8]) = ra[j];
+      *reinterpret_cast<i32x4*>(&Bs[buf][p][r][kc * 8]) = rb[j];
     }
   };
 
+  const int nk = Kp / BK;
   gload(0);
-  for (int k0 = 0; k0 < Kp; k0 += BK) {
-    lstore();
-    __syncthreads();
-    if (k0 + BK < Kp) gload(k0 + BK);  // next slice in flight during the MFMAs
-    const int fr = lane & 15, fk = (lane >> 4) * 8;
+  lstore(0);
+  if (nk > 1) gload(BK);
+  __syncthreads();
+  const int fr = lane & 15, fk = (lane >> 4) * 8;
+  for (int k = 0; k < nk; ++k) {
+    const int cur = k & 1;
     bf16x8 a[4][3], b[4][3];
 #pragma unroll
     for (int i = 0; i < 4; ++i)
 #pragma unroll
       for (int p = 0; p < 3; ++p) {
-        a[i][p] = *reinterpret_cast<const bf16x8*>(&As[p][wm * 64 + i * 16 + fr][fk]);
-        b[i][p] = *reinterpret_cast<const bf16x8*>(&Bs[p][wn * 64 + i * 16 + fr][fk]);
+        a[i][p] = *reinterpret_cast<const bf16x8*>(&As[cur][p][wm * 64 + i * 16 + fr][fk]);
+        b[i][p] = *reinterpret_cast<const bf16x8*>(&Bs[cur][p][wn * 64 + i * 16 + fr][fk]);
       }
-#pragma unroll
-    for (int i = 0; i < 4; ++i)
-#pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        acc0[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i][0], b[j][0], acc0[i][j], 0, 0, 0);
-        acc1[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i][1], b[j][1], acc1[i][j], 0, 0, 0);
-        acc1[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i][0], b[j][2], acc1[i][j], 0, 0, 0);
-        acc1[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i][2], b[j][0], acc1[i][j], 0, 0, 0);
-        acc1[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i][0], b[j][1], acc1[i][j], 0, 0, 0);
-        acc1[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i][1], b[j][0], acc1[i][j], 0, 0, 0);
-      }
+    SKL_BF16X3_STEP(acc0, acc1, a, b);
+    if (k + 1 < nk) {
+      lstore(cur ^ 1);  // the other buffer was last read before the previous barrier
+      if (k + 2 < nk) gload((k + 2) * BK);
+    }
     __syncthreads();
   }
   // C fragment of a 16x16 tile: lane holds column lane%16, rows 4*(lane/16)+0..3
@@ -264,6 +279,11 @@ __device__ __forceinline__ uint8_t fp8_e4m3(float x)
   return (uint8_t)(sign | (ex << 3) | mant);
 }
 
+// One 128x128 tile of X (R x Cc row-major, or its transpose when `trans`)
+// per work-group: read once into registers along X's contiguous dimension,
+// absmax, a power-of-two scale s >= absmax/448 (stored as a float whose
+// exponent field is the E8M0 scale the MFMA takes), e4m3 bytes staged in LDS
+// and written row-wise in 16-byte chunks.
 __global__ __launch_bounds__(256) void quant_fp8_kernel(const float* __restrict__ X, int R,
                                                         int Cc, int trans,
                                                         uint8_t* __restrict__ out,
@@ -273,27 +293,46 @@ __global__ __launch_bounds__(256) void quant_fp8_kernel(const float* __restrict_
   const int tr = blockIdx.y, tc = blockIdx.x;
   const int r0 = tr * 128, c0 = tc * 128;
   __shared__ float red[256];
+  __shared__ __attribute__((aligned(16))) uint8_t qt[128][128 + 16];
+  float v[64];
   float amax = 0.0f;
-  for (int e = threadIdx.x; e < 128 * 128; e += 256) {
-    const int r = r0 + (e >> 7), c = c0 + (e & 127);
-    float x = 0.0f;
-    if (r < R && c < Cc) x = trans ? X[(size_t)c * R + r] : X[(size_t)r * Cc + c];
-    amax = fmaxf(amax, fabsf(x));
+#pragma unroll
+  for (int u = 0; u < 64; ++u) {
+    const int e = threadIdx.x + 256 * u;
+    // lanes run along X's contiguous dimension
+    const int r = trans ? r0 + (e & 127) : r0 + (e >> 7);
+    const int c = trans ? c0 + (e >> 7) : c0 + (e & 127);
+    const bool ok = r < R && c < Cc;
+    const size_t i = trans ? (size_t)c * R + r : (size_t)r * Cc + c;
+    const float x = as_global(X)[ok ? i : 0];
+    v[u] = ok ? x : 0.0f;
+    amax = fmaxf(amax, fabsf(v[u]));
   }
   red[threadIdx.x] = amax;
   __syncthreads();
-  for (int s = 128; s > 0; s >>= 1) {
-    if ((int)threadIdx.x < s) red[threadIdx.x] = fmaxf(red[threadIdx.x], red[threadIdx.x + s]);
+  for (int k = 128; k > 0; k >>= 1) {
+    if ((int)threadIdx.x < k) red[threadIdx.x] = fmaxf(red[threadIdx.x], red[threadIdx.x + k]);
     __syncthreads();
   }
-  const float scale = red[0] > 0.0f ? red[0] / 448.0f : 1.0f;
-  const float inv = 1.0f / scale;
+  int ex = 0;
+  if (red[0] > 0.0f) frexpf(red[0] / 448.0f, &ex);  // absmax/448 = m * 2^ex, m in [0.5, 1)
+  const float scale = ldexpf(1.0f, ex);
+  const float inv = ldexpf(1.0f, -ex);
   if (threadIdx.x == 0) scales[tr * (Cp / 128) + tc] = scale;
-  for (int e = threadIdx.x; e < 128 * 128; e += 256) {
-    const int r = r0 + (e >> 7), c = c0 + (e & 127);
-    float x = 0.0f;
-    if (r < R && c < Cc) x = trans ? X[(size_t)c * R + r] : X[(size_t)r * Cc + c];
-    out[(size_t)r * Cp + c] = fp8_e4m3(x * inv);
+#pragma unroll
+  for (int u = 0; u < 64; ++u) {
+    const int e = threadIdx.x + 256 * u;
+    const int lr = trans ? (e & 127) : (e >> 7);
+    const int lc = trans ? (e >> 7) : (e & 127);
+    qt[lr][lc] = fp8_e4m3(v[u] * inv);
+  }
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const int q = threadIdx.x + 256 * j;  // 128 rows x 8 chunks of 16 bytes
+    const int lr = q >> 3, kc = q & 7;
+    *reinterpret_cast<i32x4*>(out + (size_t)(r0 + lr) * Cp + c0 + kc * 16) =
+        *reinterpret_cast<const i32x4*>(&qt[lr][kc * 16]);
   }
 }
 
@@ -341,7 +380,10 @@ __global__ __launch_bounds__(256, 1) void gemm_fp8_kernel(
     }
     __syncthreads();
     if (kb + 1 < nkb) gload(kb + 1);
-    const float s = sa[(m0 / 128) * nkb + kb] * sb[(n0 / 128) * nkb + kb];
+    // the tile scales are powers of two: their exponent fields are the E8M0
+    // block scales of the MFMA, which then accumulates directly
+    const int ea = (int)((__float_as_uint(sa[(m0 / 128) * nkb + kb]) >> 23) & 0xFF);
+    const int eb = (int)((__float_as_uint(sb[(n0 / 128) * nkb + kb]) >> 23) & 0xFF);
     // operand fragment: lane holds row lane%16, k bytes 32*(lane/16)..+31
     const int fr = lane & 15, fk = (lane >> 4) * 32;
     i32x8 a[4], b[4];
@@ -353,13 +395,9 @@ __global__ __launch_bounds__(256, 1) void gemm_fp8_kernel(
 #pragma unroll
     for (int i = 0; i < 4; ++i)
 #pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        f32x4 t = {0.f, 0.f, 0.f, 0.f};
-        // format 0 = e4m3 for both operands; scales 127 = 2^0 (unscaled)
-        t = __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(a[i], b[j], t, 0, 0, 0, 127, 0,
-                                                             127);
-        acc[i][j] += t * s;
-      }
+      for (int j = 0; j < 4; ++j)  // format 0 = e4m3 for both operands
+        acc[i][j] = __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(a[i], b[j], acc[i][j], 0, 0,
+                                                                     0, ea, 0, eb);
     __syncthreads();
   }
   const int cc = lane & 15, cr = (lane >> 4) * 4;
@@ -751,17 +789,7 @@ __global__ __launch_bounds__(256) void batch_bf16x3_kernel(const skl_gemm_desc* 
           a[i][q] = *reinterpret_cast<const bf16x8*>(&As[q][wm * 64 + i * 16 + fr][fk]);
           b[i][q] = *reinterpret_cast<const bf16x8*>(&Bs[q][wn * 64 + i * 16 + fr][fk]);
         }
-#pragma unroll
-      for (int i = 0; i < 4; ++i)
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-          acc0[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i][0], b[j][0], acc0[i][j], 0, 0, 0);
-          acc1[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i][1], b[j][1], acc1[i][j], 0, 0, 0);
-          acc1[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i][0], b[j][2], acc1[i][j], 0, 0, 0);
-          acc1[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i][2], b[j][0], acc1[i][j], 0, 0, 0);
-          acc1[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i][0], b[j][1], acc1[i][j], 0, 0, 0);
-          acc1[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i][1], b[j][0], acc1[i][j], 0, 0, 0);
-        }
+      SKL_BF16X3_STEP(acc0, acc1, a, b);
       __syncthreads();
     }
     const int cc = lane & 15, cr = (lane >> 4) * 4;
