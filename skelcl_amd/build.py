@@ -31,6 +31,12 @@ COMMON = [
 
 TEMPLATES = ["prelude", "map", "reduce", "scan", "mapoverlap", "allpairs"]
 
+# Per-source options.  matmul.hip: MFMA accumulators in the VGPR form; with
+# the default (AGPR) form the register allocator gives each MFMA a result
+# register other than its accumulator input and copies the accumulators
+# round the k-loop (hundreds of v_accvgpr moves per slice).
+EXTRA = {"matmul.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form=1"]}
+
 
 def _embed() -> str:
     out = os.path.join(BUILD, "embedded_kernels.cpp")
@@ -61,7 +67,7 @@ def _sources() -> list[str]:
 def _compile(src: str) -> str:
     obj = os.path.join(BUILD, os.path.basename(src) + ".o")
     lang = ["-x", "hip"]
-    cmd = [HIPCC, *COMMON, *lang, "-c", src, "-o", obj]
+    cmd = [HIPCC, *COMMON, *EXTRA.get(os.path.basename(src), []), *lang, "-c", src, "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"compile failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")

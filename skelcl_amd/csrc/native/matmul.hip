@@ -19,6 +19,7 @@
 // MFMA tiles of 16x16.  blockIdx.x walks N, blockIdx.y walks M.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 
 #include "skelcl_skeletons.h"
@@ -55,57 +56,86 @@ __device__ __forceinline__ void split3(float x, uint16_t& h0, uint16_t& h1, uint
   h2 = bf16_rne(r2);
 }
 
-// X is R x C row-major (ld = C) or, when trans, C x R row-major read as its
-// transpose.  Writes 3 planes of Rp x Cp bf16 (row-major, zero padded):
-// plane p at out + p * Rp * Cp.
-__global__ __launch_bounds__(256) void split3_kernel(const float* __restrict__ X, int R, int Cc,
-                                                     int trans, uint16_t* __restrict__ out,
-                                                     int Rp, int Cp)
+// ---------------------------------------------------------------------------
+// Tiled plane layout of the bf16x3 operands.  An operand X (R x K, read K
+// contiguous; R padded to Rp % 128 == 0, K to Kp % 64 == 0, zeros) is stored
+// as blocks of 128 rows x 32 k, one per (row tile t, k-slice s, plane p), at
+// block index (t * (Kp / 32) + s) * 3 + p, 4096 bf16 each.  Inside a block,
+// row r holds its 32 k as four 16-byte chunks, chunk c at position
+// c ^ swz(r).  A block is the exact LDS image the GEMM kernel multiplies
+// from, so one k-slice of one row tile (three planes) is a contiguous 24 KB
+// run that global_load_lds copies in 1 KB pieces.  swz spreads the 16 rows
+// of an MFMA fragment read over 16 distinct 16-byte bank groups (rows r,
+// r+2, r+4, r+6 and r, r+4, r+8, r+12 differ in swz), for 32 or 64 banks.
+constexpr int BK = 32;          // k per slice
+constexpr int SLICE = BM * BK;  // bf16 per block
+__device__ __forceinline__ int swz(int r) { return ((r >> 1) ^ (r >> 2)) & 3; }
+
+// Offset (bf16) of (row r, k) in plane 0; for k % 8 == 0 it starts an
+// aligned 8-element chunk.
+__device__ __forceinline__ size_t tiled_offset(int r, int k, int nk)
 {
-  const size_t plane = (size_t)Rp * Cp;
-  const size_t total = plane;
-  for (size_t e = (size_t)blockIdx.x * 256 + threadIdx.x; e < total;
-       e += (size_t)gridDim.x * 256) {
-    const int r = (int)(e / Cp), c = (int)(e - (size_t)r * Cp);
-    float x = 0.0f;
-    if (r < R && c < Cc) x = trans ? X[(size_t)c * R + r] : X[(size_t)r * Cc + c];
-    uint16_t h0, h1, h2;
-    split3(x, h0, h1, h2);
-    out[e] = h0;
-    out[plane + e] = h1;
-    out[2 * plane + e] = h2;
+  const int rr = r & (BM - 1), kk = k & (BK - 1);
+  return ((size_t)(r >> 7) * nk + (k >> 5)) * 3 * SLICE + rr * BK +
+         ((((kk >> 3) ^ swz(rr)) << 3) | (kk & 7));
+}
+
+// Split eight values and store the three 16-byte chunks at out + o + p*SLICE.
+__device__ __forceinline__ void split_store8(const float (&x)[8], uint16_t* out, size_t o)
+{
+  uint16_t h[3][8];
+#pragma unroll
+  for (int i = 0; i < 8; ++i) split3(x[i], h[0][i], h[1][i], h[2][i]);
+#pragma unroll
+  for (int p = 0; p < 3; ++p) {
+    i32x4 v;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) v[i] = (int)((uint32_t)h[p][2 * i] | ((uint32_t)h[p][2 * i + 1] << 16));
+    *reinterpret_cast<i32x4*>(out + o + (size_t)p * SLICE) = v;
   }
 }
 
-// Transposing variant for a row-major K x N matrix B into N x K planes with
-// coalesced reads and writes through a 32x32 LDS tile.
-__global__ __launch_bounds__(256) void split3_transpose_kernel(const float* __restrict__ X,
-                                                               int R, int Cc,
-                                                               uint16_t* __restrict__ out,
-                                                               int Rp, int Cp)
+// X row-major R x Kc (ld = Kc) into the tiled planes, one thread per 8-k chunk.
+__global__ __launch_bounds__(256) void split3_kernel(const float* __restrict__ X, int R, int Kc,
+                                                     uint16_t* __restrict__ out, int Rp, int Kp)
 {
-  // X is Cc x R row-major (ld = R); the planes are R x Cc (Rp x Cp), i.e.
-  // out[r][c] = X[c][r].
-  __shared__ float t[32][33];
-  const size_t plane = (size_t)Rp * Cp;
-  const int r0 = blockIdx.x * 32, c0 = blockIdx.y * 32;
-  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
-  for (int k = ty; k < 32; k += 8) {
-    const int c = c0 + k, r = r0 + tx;
-    t[k][tx] = ld_or(X, (size_t)c * R + r, 0, c < Cc && r < R, 0.0f);
+  const int nk = Kp / BK, cpr = Kp / 8;
+  const size_t total = (size_t)Rp * cpr;
+  for (size_t e = (size_t)blockIdx.x * 256 + threadIdx.x; e < total;
+       e += (size_t)gridDim.x * 256) {
+    const int r = (int)(e / cpr), k0 = (int)(e - (size_t)r * cpr) * 8;
+    const bool rok = r < R;
+    const size_t base = (size_t)(rok ? r : 0) * Kc + k0;
+    float x[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) x[i] = ld_or(X, base + i, 0, rok && k0 + i < Kc, 0.0f);
+    split_store8(x, out, tiled_offset(r, k0, nk));
+  }
+}
+
+// X row-major Kc x R (ld = R), read as its transpose (element (r, k) =
+// X[k][r]), into the tiled planes: 64 r x 32 k per work-group through LDS,
+// reads coalesced along r, one thread per 8-k chunk on the way out.
+__global__ __launch_bounds__(256) void split3_transpose_kernel(const float* __restrict__ X,
+                                                               int R, int Kc,
+                                                               uint16_t* __restrict__ out,
+                                                               int Rp, int Kp)
+{
+  __shared__ float t[32][65];
+  const int nk = Kp / BK;
+  const int r0 = blockIdx.x * 64, k0 = blockIdx.y * 32;
+  const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;  // 64 x 4
+#pragma unroll
+  for (int kk = ty; kk < 32; kk += 4) {
+    const int k = k0 + kk, r = r0 + tx;
+    t[kk][tx] = ld_or(X, (size_t)k * R + r, 0, k < Kc && r < R, 0.0f);
   }
   __syncthreads();
-  for (int k = ty; k < 32; k += 8) {
-    const int r = r0 + k, c = c0 + tx;
-    if (r < Rp && c < Cp) {
-      uint16_t h0, h1, h2;
-      split3(t[tx][k], h0, h1, h2);
-      const size_t e = (size_t)r * Cp + c;
-      out[e] = h0;
-      out[plane + e] = h1;
-      out[2 * plane + e] = h2;
-    }
-  }
+  const int r = threadIdx.x >> 2, q = threadIdx.x & 3;  // 64 rows x 4 chunks
+  float x[8];
+#pragma unroll
+  for (int i = 0; i < 8; ++i) x[i] = t[q * 8 + i][r];
+  split_store8(x, out, tiled_offset(r0 + r, k0 + q * 8, nk));
 }
 
 // ---------------------------------------------------------------------------
@@ -132,10 +162,7 @@ __device__ __forceinline__ void tile_coords(int& tm, int& tn)
 }
 
 // ---------------------------------------------------------------------------
-// bf16x3 GEMM.  Ap: 3 planes Mp x Kp, Bp: 3 planes Np x Kp (B^T), Kp % 32 == 0,
-// Mp % 128 == 0, Np % 128 == 0.
-constexpr int BK = 32;
-constexpr int LDS_STRIDE = BK + 8;  // bf16 per LDS row (padding vs bank conflicts)
+// bf16x3 GEMM on the tiled planes of A (Mp rows) and B^T (Np rows).
 
 // Six bf16 products of one k-step, ordered so that consecutive MFMAs write
 // different accumulators: each product type sweeps all 16 tiles before the
@@ -154,20 +181,50 @@ constexpr int LDS_STRIDE = BK + 8;  // bf16 per LDS row (padding vs bank conflic
               a[i_][pa_[t_]], b[j_][pb_[t_]], acc1[i_][j_], 0, 0, 0);                \
   } while (0)
 
+typedef __attribute__((address_space(3))) void lds_ptr_t;
+
+// Fragments of one k-slice: a[i][p] is plane p of the 16-row A tile i, b
+// likewise for B^T.
+struct Bf16x3Frag {
+  bf16x8 a[4][3], b[4][3];
+};
+
+__device__ __forceinline__ void bf16x3_read(Bf16x3Frag& f, const uint16_t* As,
+                                            const uint16_t* Bs, int p, int arow, int brow,
+                                            int pos)
+{
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    f.a[i][p] = *reinterpret_cast<const bf16x8*>(As + p * SLICE + (arow + i * 16) * BK + pos);
+    f.b[i][p] = *reinterpret_cast<const bf16x8*>(Bs + p * SLICE + (brow + i * 16) * BK + pos);
+  }
+}
+
+__device__ __forceinline__ void bf16x3_mm(f32x4 (&acc)[4][4], const Bf16x3Frag& f, int pa, int pb)
+{
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+      acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(f.a[i][pa], f.b[j][pb], acc[i][j], 0, 0, 0);
+}
+
 __global__ __launch_bounds__(256, 1) void gemm_bf16x3_kernel(
     const uint16_t* __restrict__ Ap, const uint16_t* __restrict__ Bp, float* __restrict__ C,
-    int M, int N, int Mp, int Np, int Kp, int ldc)
+    int M, int N, int Kp, int ldc)
 {
-  // double-buffered slices: slice k+1 is written to the other buffer while
-  // slice k is multiplied, one barrier per k-step
-  __shared__ __attribute__((aligned(16))) uint16_t As[2][3][BM][LDS_STRIDE];
-  __shared__ __attribute__((aligned(16))) uint16_t Bs[2][3][BN][LDS_STRIDE];
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  // Two slice buffers (the tiled blocks of A and B^T), filled by
+  // global_load_lds; separate objects, so that the compiler's wait for the
+  // fill of one does not block reads of the other.
+  __shared__ __attribute__((aligned(1024))) uint16_t As0[3 * SLICE], Bs0[3 * SLICE];
+  __shared__ __attribute__((aligned(1024))) uint16_t As1[3 * SLICE], Bs1[3 * SLICE];
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int wm = wave >> 1, wn = wave & 1;
   int tm, tn;
   tile_coords(tm, tn);
   const int m0 = tm * BM, n0 = tn * BN;
-  const size_t pa = (size_t)Mp * Kp, pb = (size_t)Np * Kp;
+  const int nk = Kp / BK;
 
   f32x4 acc0[4][4], acc1[4][4];
 #pragma unroll
@@ -178,51 +235,92 @@ __global__ __launch_bounds__(256, 1) void gemm_bf16x3_kernel(
       acc1[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
     }
 
-  // global -> register staging: 3 planes x 128 rows x 4 chunks of 8 bf16
-  // = 1536 chunks per operand, 6 per thread
-  i32x4 ra[6], rb[6];
-  auto gload = [&](int k0) {
+  // Fill of slice s: per operand the 24 KB run of its three blocks, 24
+  // pieces of 1 KB, six per wave; lane l copies bytes 16 l .. 16 l + 15 of
+  // a piece.  Address = wave-uniform base + 32-bit lane offset (saddr form).
+  // Slices past the end are clamped to the last one: a harmless reload into
+  // a buffer that is not read again.
+  const char* ga = reinterpret_cast<const char*>(Ap) + (size_t)tm * nk * (3 * SLICE * 2);
+  const char* gb = reinterpret_cast<const char*>(Bp) + (size_t)tn * nk * (3 * SLICE * 2);
+  const unsigned loff = (unsigned)lane * 16u;
+  auto fill = [&](uint16_t* As, uint16_t* Bs, int sl) {
+    const size_t so = (size_t)(sl < nk ? sl : nk - 1) * (3 * SLICE * 2);
 #pragma unroll
     for (int j = 0; j < 6; ++j) {
-      const int c = tid + 256 * j;
-      const int p = c >> 9, r = (c >> 2) & 127, kc = c & 3;
-      ra[j] = *reinterpret_cast<const i32x4*>(Ap + p * pa + (size_t)(m0 + r) * Kp + k0 + kc * 8);
-      rb[j] = *reinterpret_cast<const i32x4*>(Bp + p * pb + (size_t)(n0 + r) * Kp + k0 + kc * 8);
-    }
-  };
-  auto lstore = [&](int buf) {
-#pragma unroll
-    for (int j = 0; j < 6; ++j) {
-      const int c = tid + 256 * j;
-      const int p = c >> 9, r = (c >> 2) & 127, kc = c & 3;
-      *reinterpret_cast<i32x4*>(&As[buf][p][r][kc * 8]) = ra[j];
-      *reinterpret_cast<i32x4*>(&Bs[buf][p][r][kc * 8]) = rb[j];
+      const int piece = wave * 6 + j;
+      __builtin_amdgcn_global_load_lds(ga + so + piece * 1024 + loff,
+                                       (lds_ptr_t*)(As + piece * 512), 16, 0, 0);
+      __builtin_amdgcn_global_load_lds(gb + so + piece * 1024 + loff,
+                                       (lds_ptr_t*)(Bs + piece * 512), 16, 0, 0);
     }
   };
 
-  const int nk = Kp / BK;
-  gload(0);
-  lstore(0);
-  if (nk > 1) gload(BK);
-  __syncthreads();
-  const int fr = lane & 15, fk = (lane >> 4) * 8;
-  for (int k = 0; k < nk; ++k) {
-    const int cur = k & 1;
-    bf16x8 a[4][3], b[4][3];
+  // MFMA fragment: lane holds row lane % 16, k-chunk lane / 16 of a 16-row tile
+  const int fr = lane & 15;
+  const int pos = (((lane >> 4) ^ swz(fr)) * 8);
+  const int arow = wm * 64 + fr, brow = wn * 64 + fr;
+
+  // Slice k from (Ac, Bc) with plane 0 already in f; (An, Bn) receives
+  // slice k+1 (its fill was issued one slice earlier).
+  //  1. x0y0 with the plane-1 reads interleaved, x1y0 + x0y1 with the
+  //     plane-2 reads interleaved, x1y1;
+  //  2. wait for this wave's fill of slice k+1 and its LDS reads, barrier:
+  //     slice k+1 is in LDS and nobody reads (Ac, Bc) any more;
+  //  3. fill slice k+2 into (Ac, Bc); plane 0 of slice k+1 into g,
+  //     interleaved with x2y0 + x0y2.
+  // A fill thus has one slice of MFMA work to arrive.  (The compiler waits
+  // for every outstanding fill before any LDS read that may alias one, so
+  // fills further ahead would be waited for early.)  Scheduling barriers pin
+  // the phases; group barriers (mask 0x008 MFMA, 0x100 LDS read) interleave
+  // the reads with the MFMAs.
+  auto slice = [&](uint16_t* Ac, uint16_t* Bc, const uint16_t* An, const uint16_t* Bn, int k,
+                   Bf16x3Frag& f, Bf16x3Frag& g) {
+    bf16x3_mm(acc0, f, 0, 0);
+    bf16x3_read(f, Ac, Bc, 1, arow, brow, pos);
+    bf16x3_mm(acc1, f, 1, 0);
+    bf16x3_mm(acc1, f, 0, 1);
+    bf16x3_read(f, Ac, Bc, 2, arow, brow, pos);
+    bf16x3_mm(acc1, f, 1, 1);
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
-#pragma unroll
-      for (int p = 0; p < 3; ++p) {
-        a[i][p] = *reinterpret_cast<const bf16x8*>(&As[cur][p][wm * 64 + i * 16 + fr][fk]);
-        b[i][p] = *reinterpret_cast<const bf16x8*>(&Bs[cur][p][wn * 64 + i * 16 + fr][fk]);
-      }
-    SKL_BF16X3_STEP(acc0, acc1, a, b);
-    if (k + 1 < nk) {
-      lstore(cur ^ 1);  // the other buffer was last read before the previous barrier
-      if (k + 2 < nk) gload((k + 2) * BK);
+    for (int q = 0; q < 8; ++q) {
+      __builtin_amdgcn_sched_group_barrier(0x008, 2, 0);
+      __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
     }
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      __builtin_amdgcn_sched_group_barrier(0x008, 4, 0);
+      __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
+    }
+    __builtin_amdgcn_sched_group_barrier(0x008, 16, 0);
+    __builtin_amdgcn_sched_barrier(0);
+    __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
+    fill(Ac, Bc, k + 2);
+    __builtin_amdgcn_sched_barrier(0);
+    bf16x3_read(g, An, Bn, 0, arow, brow, pos);
+    bf16x3_mm(acc1, f, 2, 0);
+    bf16x3_mm(acc1, f, 0, 2);
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
+      __builtin_amdgcn_sched_group_barrier(0x008, 4, 0);
+    }
+    __builtin_amdgcn_sched_barrier(0);
+  };
+
+  Bf16x3Frag f0, f1;
+  fill(As0, Bs0, 0);
+  fill(As1, Bs1, 1);
+  __builtin_amdgcn_s_waitcnt(0);
+  __syncthreads();
+  bf16x3_read(f0, As0, Bs0, 0, arow, brow, pos);
+  // nk is even (Kp % 64 == 0)
+  for (int k = 0; k < nk; k += 2) {
+    slice(As0, Bs0, As1, Bs1, k, f0, f1);
+    slice(As1, Bs1, As0, Bs0, k + 1, f1, f0);
   }
+  // no fill may still be writing LDS when the work-group ends
+  __builtin_amdgcn_s_waitcnt(0);
   // C fragment of a 16x16 tile: lane holds column lane%16, rows 4*(lane/16)+0..3
   const int cc = lane & 15, cr = (lane >> 4) * 4;
 #pragma unroll
@@ -542,7 +640,7 @@ extern "C" size_t skl_internal_matmul_workspace(size_t M, size_t N, size_t K, in
 {
   const size_t Mp = round_up(M, 128), Np = round_up(N, 128);
   if (precision == SKL_PREC_BF16X3) {
-    const size_t Kp = round_up(K, 32);
+    const size_t Kp = round_up(K, 64);
     return 3 * (Mp + Np) * Kp * 2 + 256;
   }
   if (precision == SKL_PREC_FP8) {
@@ -558,35 +656,27 @@ extern "C" int skl_internal_matmul(float* C, const float* A, const float* B, int
 {
   const int Mp = (int)round_up(M, 128), Np = (int)round_up(N, 128);
   if (precision == SKL_PREC_BF16X3) {
-    const int Kp = (int)round_up(K, 32);
+    const int Kp = (int)round_up(K, 64);
     uint16_t* Ap = static_cast<uint16_t*>(ws);
     uint16_t* Bp = Ap + 3 * (size_t)Mp * Kp;
-    // A planes: Mp x Kp from A (M x K, or K x M when ta)
-    {
-      const size_t tot = (size_t)Mp * Kp;
-      unsigned g = (unsigned)((tot + 255) / 256);
-      if (g > 65536) g = 65536;
-      if (!ta) {
-        split3_kernel<<<g, 256, 0, s>>>(A, M, K, 0, Ap, Mp, Kp);
-      } else {
-        dim3 grid((Mp + 31) / 32, (Kp + 31) / 32);
-        split3_transpose_kernel<<<grid, 256, 0, s>>>(A, M, K, Ap, Mp, Kp);
-      }
+    // tiled planes of A (M x K, or K x M when ta)
+    if (!ta) {
+      const size_t tot = (size_t)Mp * (Kp / 8);
+      const unsigned g = (unsigned)std::min<size_t>((tot + 255) / 256, 65536);
+      split3_kernel<<<g, 256, 0, s>>>(A, M, K, Ap, Mp, Kp);
+    } else {
+      split3_transpose_kernel<<<dim3(Mp / 64, Kp / 32), 256, 0, s>>>(A, M, K, Ap, Mp, Kp);
     }
-    // B^T planes: Np x Kp from B (K x N, or N x K when tb)
-    {
-      if (tb) {
-        const size_t tot = (size_t)Np * Kp;
-        unsigned g = (unsigned)((tot + 255) / 256);
-        if (g > 65536) g = 65536;
-        split3_kernel<<<g, 256, 0, s>>>(B, N, K, 0, Bp, Np, Kp);
-      } else {
-        dim3 grid((Np + 31) / 32, (Kp + 31) / 32);
-        split3_transpose_kernel<<<grid, 256, 0, s>>>(B, N, K, Bp, Np, Kp);
-      }
+    // tiled planes of B^T (B is K x N, or N x K when tb)
+    if (tb) {
+      const size_t tot = (size_t)Np * (Kp / 8);
+      const unsigned g = (unsigned)std::min<size_t>((tot + 255) / 256, 65536);
+      split3_kernel<<<g, 256, 0, s>>>(B, N, K, Bp, Np, Kp);
+    } else {
+      split3_transpose_kernel<<<dim3(Np / 64, Kp / 32), 256, 0, s>>>(B, N, K, Bp, Np, Kp);
     }
     dim3 grid(Np / BN, Mp / BM);
-    gemm_bf16x3_kernel<<<grid, 256, 0, s>>>(Ap, Bp, C, M, N, Mp, Np, Kp, N);
+    gemm_bf16x3_kernel<<<grid, 256, 0, s>>>(Ap, Bp, C, M, N, Kp, N);
     return st(hipGetLastError());
   }
   if (precision == SKL_PREC_FP8) {
