@@ -266,6 +266,10 @@ int skl_autotune_args(skl_skeleton s, size_t n, const skl_arg* args, int nargs);
 /* the tuned shape in use: {unroll, grid blocks per CU, scan packs per lane,
  * MapOverlap rows per thread} */
 int skl_skeleton_launch_shape(skl_skeleton s, int shape[4]);
+/* Pin a launch shape (same layout; ranges unroll 1-16, blocks per CU 1-1024,
+ * scan packs 1-32, rows 1-16).  No counterpart in the reference (its launch
+ * shapes are fixed): for callers that tune offline or pin a measured shape. */
+int skl_skeleton_set_launch_shape(skl_skeleton s, const int shape[4]);
 
 #ifdef __cplusplus
 }

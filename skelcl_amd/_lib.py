@@ -143,6 +143,7 @@ SIGNATURES = {
     "skl_autotune": (_i, [_vp, _sz, _i]),
     "skl_autotune_args": (_i, [_vp, _sz, _argp, _i]),
     "skl_skeleton_launch_shape": (_i, [_vp, C.POINTER(_i)]),
+    "skl_skeleton_set_launch_shape": (_i, [_vp, C.POINTER(_i)]),
 }
 
 _lib = None

@@ -93,8 +93,10 @@ std::vector<Shape> candidates(const skl_skeleton_s* s)
   std::vector<Shape> c;
   switch (s->kind) {
     case SKL_MAP: case SKL_ZIP: case SKL_REDUCE:
+      // small grids keep 24-32 KB of loads in flight per CU with few
+      // waves, where HBM streams peak on MI355X; larger ones suit short inputs
       for (int u : {1, 2, 4, 8})
-        for (int b : {4, 8, 16, 64}) c.push_back(Shape{u, b, d.scan_j, d.mo_rb});
+        for (int b : {1, 2, 3, 4, 8, 16, 64}) c.push_back(Shape{u, b, d.scan_j, d.mo_rb});
       break;
     case SKL_SCAN:
       for (int j : {2, 4, 8, 16}) c.push_back(Shape{d.unroll, d.blocks_per_cu, j, d.mo_rb});
@@ -226,6 +228,16 @@ extern "C" int skl_autotune(skl_skeleton s, size_t n, int enable)
 extern "C" int skl_autotune_args(skl_skeleton s, size_t n, const skl_arg* args, int nargs)
 {
   return tune(s, n, args, nargs);
+}
+
+extern "C" int skl_skeleton_set_launch_shape(skl_skeleton s, const int shape[4])
+{
+  if (!s || !shape) return fail(SKL_ERR_INVALID, "NULL argument");
+  if (shape[0] < 1 || shape[0] > 16 || shape[1] < 1 || shape[1] > 1024 || shape[2] < 1 ||
+      shape[2] > 32 || shape[3] < 1 || shape[3] > 16)
+    return fail(SKL_ERR_INVALID, "launch shape out of range");
+  apply(s, Shape{shape[0], shape[1], shape[2], shape[3]});
+  return SKL_OK;
 }
 
 extern "C" int skl_skeleton_launch_shape(skl_skeleton s, int shape[4])

@@ -338,52 +338,18 @@ __global__ __launch_bounds__(256, 1) void gemm_bf16x3_kernel(
 
 // ---------------------------------------------------------------------------
 // FP8 (OCP e4m3) with one scale per 128x128 tile.
-// Quantise X (R x C view, optional transpose as split3) into Rp x Cp bytes,
+// Quantise X (R x C view, optionally transposed) into Rp x Cp bytes,
 // Rp % 128 == 0, Cp % 128 == 0; scales[(r/128) * (Cp/128) + c/128].
-// One work-group per 128x128 tile.
-__device__ __forceinline__ uint8_t fp8_e4m3(float x)
-{
-  // round-to-nearest-even into OCP e4m3 (bias 7, max 448, no inf)
-  const uint32_t u = __float_as_uint(x);
-  const uint32_t sign = (u >> 24) & 0x80u;
-  float a = fabsf(x);
-  if (!(a == a)) return (uint8_t)(sign | 0x7F);  // NaN
-  if (a >= 464.0f) return (uint8_t)(sign | 0x7E);  // saturate to 448
-  if (a < 0x1p-10f) return (uint8_t)sign;          // below half the smallest subnormal
-  int e;
-  frexpf(a, &e);                 // a = m * 2^e, m in [0.5, 1)
-  int ue = e - 1;                // a in [2^ue, 2^(ue+1))
-  if (ue < -6) ue = -6;          // subnormal range: spacing 2^-9
-  const float q = ldexpf(1.0f, ue - 3);  // spacing of this binade
-  float m = rintf(a / q);        // round half to even
-  float v = m * q;
-  if (v >= 464.0f) return (uint8_t)(sign | 0x7E);
-  // encode v
-  if (v == 0.0f) return (uint8_t)sign;
-  int ve;
-  float vm = frexpf(v, &ve);     // v = vm * 2^ve, vm in [0.5,1)
-  int ex = ve - 1 + 7;           // biased exponent
-  uint32_t mant;
-  if (ex <= 0) {                 // subnormal: v = mant * 2^-9
-    mant = (uint32_t)rintf(v * 512.0f);
-    return (uint8_t)(sign | mant);
-  }
-  mant = (uint32_t)rintf((vm * 2.0f - 1.0f) * 8.0f);
-  if (mant == 8) {
-    mant = 0;
-    ++ex;
-  }
-  if (ex > 15 || (ex == 15 && mant == 7)) return (uint8_t)(sign | 0x7E);
-  return (uint8_t)(sign | (ex << 3) | mant);
-}
-
 // One 128x128 tile of X (R x Cc row-major, or its transpose when `trans`)
-// per work-group: read once into registers along X's contiguous dimension,
-// absmax, a power-of-two scale s >= absmax/448 (stored as a float whose
-// exponent field is the E8M0 scale the MFMA takes), e4m3 bytes staged in LDS
-// and written row-wise in 16-byte chunks.
+// per work-group: read once into registers, 4 consecutive elements of X's
+// contiguous dimension per load, absmax, a power-of-two scale s >=
+// absmax/448 (stored as a float whose exponent field is the E8M0 scale the
+// MFMA takes), e4m3 bytes by the hardware conversion (v_cvt_pk_fp8_f32,
+// round to nearest even; |x / s| <= 448 stays in range) staged in LDS and
+// written row-wise in 16-byte chunks.  `vec`: X is 16-byte aligned and its
+// row length is a multiple of 4, so in-bounds quads load as one float4.
 __global__ __launch_bounds__(256) void quant_fp8_kernel(const float* __restrict__ X, int R,
-                                                        int Cc, int trans,
+                                                        int Cc, int trans, int vec,
                                                         uint8_t* __restrict__ out,
                                                         float* __restrict__ scales, int Rp,
                                                         int Cp)
@@ -392,19 +358,26 @@ __global__ __launch_bounds__(256) void quant_fp8_kernel(const float* __restrict_
   const int r0 = tr * 128, c0 = tc * 128;
   __shared__ float red[256];
   __shared__ __attribute__((aligned(16))) uint8_t qt[128][128 + 16];
-  float v[64];
+  f32x4 v[16];
   float amax = 0.0f;
+  // quad u: 32 quads per line of the contiguous dimension, 128 lines
+  const size_t ld = trans ? (size_t)R : (size_t)Cc;
+  const int nl = trans ? Cc : R, nc = trans ? R : Cc;   // lines, length
+  const int l0 = trans ? c0 : r0, e0 = trans ? r0 : c0;  // tile origin
 #pragma unroll
-  for (int u = 0; u < 64; ++u) {
-    const int e = threadIdx.x + 256 * u;
-    // lanes run along X's contiguous dimension
-    const int r = trans ? r0 + (e & 127) : r0 + (e >> 7);
-    const int c = trans ? c0 + (e >> 7) : c0 + (e & 127);
-    const bool ok = r < R && c < Cc;
-    const size_t i = trans ? (size_t)c * R + r : (size_t)r * Cc + c;
-    const float x = as_global(X)[ok ? i : 0];
-    v[u] = ok ? x : 0.0f;
-    amax = fmaxf(amax, fabsf(v[u]));
+  for (int u = 0; u < 16; ++u) {
+    const int q = threadIdx.x + 256 * u;
+    const int l = l0 + (q >> 5), e = e0 + (q & 31) * 4;
+    const bool lok = l < nl;
+    const size_t base = (size_t)(lok ? l : 0) * ld;
+    if (vec && lok && e + 3 < nc) {
+      v[u] = *as_global(reinterpret_cast<const f32x4*>(X + base + e));
+    } else {
+#pragma unroll
+      for (int i = 0; i < 4; ++i) v[u][i] = ld_or_g(X, base + e + i, 0, lok && e + i < nc, 0.0f);
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) amax = fmaxf(amax, fabsf(v[u][i]));
   }
   red[threadIdx.x] = amax;
   __syncthreads();
@@ -418,11 +391,17 @@ __global__ __launch_bounds__(256) void quant_fp8_kernel(const float* __restrict_
   const float inv = ldexpf(1.0f, -ex);
   if (threadIdx.x == 0) scales[tr * (Cp / 128) + tc] = scale;
 #pragma unroll
-  for (int u = 0; u < 64; ++u) {
-    const int e = threadIdx.x + 256 * u;
-    const int lr = trans ? (e & 127) : (e >> 7);
-    const int lc = trans ? (e >> 7) : (e & 127);
-    qt[lr][lc] = fp8_e4m3(v[u] * inv);
+  for (int u = 0; u < 16; ++u) {
+    const int q = threadIdx.x + 256 * u;
+    const int ll = q >> 5, le = (q & 31) * 4;
+    int w = __builtin_amdgcn_cvt_pk_fp8_f32(v[u][0] * inv, v[u][1] * inv, 0, false);
+    w = __builtin_amdgcn_cvt_pk_fp8_f32(v[u][2] * inv, v[u][3] * inv, w, true);
+    if (!trans) {
+      *reinterpret_cast<int*>(&qt[ll][le]) = w;
+    } else {
+#pragma unroll
+      for (int i = 0; i < 4; ++i) qt[le + i][ll] = (uint8_t)(w >> (8 * i));
+    }
   }
   __syncthreads();
 #pragma unroll
@@ -685,9 +664,11 @@ extern "C" int skl_internal_matmul(float* C, const float* A, const float* B, int
     uint8_t* Bq = Aq + (size_t)Mp * Kp;
     float* sa = reinterpret_cast<float*>(Bq + (size_t)Np * Kp);
     float* sb = sa + (size_t)(Mp / 128) * (Kp / 128);
-    quant_fp8_kernel<<<dim3(Kp / 128, Mp / 128), 256, 0, s>>>(A, M, K, ta, Aq, sa, Mp, Kp);
+    const int va = ((uintptr_t)A % 16 == 0) && ((ta ? M : K) % 4 == 0);
+    const int vb = ((uintptr_t)B % 16 == 0) && ((tb ? K : N) % 4 == 0);
+    quant_fp8_kernel<<<dim3(Kp / 128, Mp / 128), 256, 0, s>>>(A, M, K, ta, va, Aq, sa, Mp, Kp);
     // B^T: rows n, columns k; element (n, k) = B[k][n] (or B[n][k] when tb)
-    quant_fp8_kernel<<<dim3(Kp / 128, Np / 128), 256, 0, s>>>(B, N, K, tb ? 0 : 1, Bq, sb, Np,
+    quant_fp8_kernel<<<dim3(Kp / 128, Np / 128), 256, 0, s>>>(B, N, K, tb ? 0 : 1, vb, Bq, sb, Np,
                                                                Kp);
     dim3 grid(Np / BN, Mp / BM);
     gemm_fp8_kernel<<<grid, 256, 0, s>>>(Aq, Bq, sa, sb, C, M, N, Kp, N);

@@ -31,11 +31,11 @@ bool skl_skeleton_s::precise_add() const
 void skl_skeleton_s::apply_default_shape(int cus)
 {
   // measured on MI355X at 2^28 elements: the streaming Reduce is fastest
-  // with 4 work-groups per CU and 2 packs per thread per stride; Map/Zip
-  // with 8 per CU and 4 packs
+  // with one work-group per CU and 8 packs per thread per stride (16 in
+  // flight with the next stride's loads); Map/Zip with 8 per CU and 4 packs
   if (kind == SKL_REDUCE) {
-    unroll = 2;
-    max_blocks = cus * 4;
+    unroll = 8;
+    max_blocks = cus;
   } else {
     unroll = 4;
     max_blocks = cus * 8;
@@ -601,6 +601,9 @@ extern "C" int skl_skeleton_create(const skl_skeleton_desc* desc, skl_skeleton* 
   if (!out) return fail(SKL_ERR_INVALID, "out is NULL");
   auto* s = new skl_skeleton_s;
   int st = setup_skeleton(desc, s);
+  // the launch shape is compiled into the kernels: set it before the first
+  // program is generated
+  if (st == SKL_OK) s->apply_default_shape(cu_count());
   if (st == SKL_OK && s->kind != SKL_ALLPAIRS_GENERAL) {
     skl_program p;
     st = get_program(s, first_key(s), &p);
@@ -609,7 +612,6 @@ extern "C" int skl_skeleton_create(const skl_skeleton_desc* desc, skl_skeleton* 
     delete s;
     return st;
   }
-  s->apply_default_shape(cu_count());
   *out = s;
   return SKL_OK;
 }
@@ -620,6 +622,7 @@ extern "C" int skl_skeleton_check(const skl_skeleton_desc* desc, char** source, 
   if (source) *source = nullptr;
   if (log) *log = nullptr;
   SKL_TRY(setup_skeleton(desc, &s));
+  s.apply_default_shape(cu_count());
   std::string src;
   SKL_TRY(generate(&s, s.kind == SKL_ALLPAIRS_GENERAL ? "dim256" : first_key(&s), &src));
   if (source) {

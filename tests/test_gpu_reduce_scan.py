@@ -226,3 +226,78 @@ def test_scan_full_size_properties(skl):
     cs = np.cumsum(hx.astype(np.uint64)) % (1 << 32)
     idx = np.arange(0, n, 9973)
     assert np.array_equal(hi[idx], cs[idx].astype(np.uint32))
+
+
+# Pinned launch shapes (unroll, blocks per CU, packs per lane): the Scan tile
+# is 256 x packs x V elements, so these cover tiles of 2048 to 16384 int32.
+SCAN_SHAPES = [(4, 8, 2), (4, 8, 4), (4, 8, 8), (4, 8, 16)]
+
+
+def _pin(s, unroll, blocks, packs):
+    import ctypes as C
+    from skelcl_amd import _lib
+    sh = (C.c_int * 4)(unroll, blocks, packs, 4)
+    _lib.check(_lib.lib().skl_skeleton_set_launch_shape(s.handle, sh))
+    got = (C.c_int * 4)()
+    _lib.check(_lib.lib().skl_skeleton_launch_shape(s.handle, got))
+    assert tuple(got) == (unroll, blocks, packs, 4)
+
+
+@pytest.mark.parametrize("shape", SCAN_SHAPES)
+@pytest.mark.parametrize("n", [1, 1000, 65536 + 3, 5 * (1 << 20) + 11])
+@pytest.mark.parametrize("inclusive", [False, True])
+def test_scan_pinned_tile_int_parity(skl, shape, n, inclusive):
+    """int32 Scan(+) bit-exact vs the oracle for every tile size."""
+    x = oracle.gen_i32_below(0x2101, n, 1000)
+    s = skl.Scan("int(int)", ADD_I)
+    _pin(s, *shape)
+    out = np.asarray(s(skl.Vector(x), inclusive=inclusive)).view(np.uint32)
+    assert np.array_equal(out, oracle.scan_add_u32(x, inclusive))
+
+
+def test_scan_tile_switches_keep_lookback_state(skl):
+    """Launches of different sizes and tile sizes in a row on one skeleton:
+    the look-back words of earlier launches (other tile counts) never match."""
+    s = skl.Scan("int(int)", ADD_I)
+    for k, shape in enumerate(SCAN_SHAPES * 2):
+        _pin(s, *shape)
+        n = 3000000 - 100003 * k if k % 2 == 0 else 1000 + k
+        x = oracle.gen_i32_below(200 + k, n, 1000)
+        out = np.asarray(s(skl.Vector(x), inclusive=k % 3 == 0)).view(np.uint32)
+        assert np.array_equal(out, oracle.scan_add_u32(x, k % 3 == 0)), k
+
+
+@pytest.mark.parametrize("shape", SCAN_SHAPES[:2])
+@pytest.mark.parametrize("n", [4097, 300001])
+def test_scan_pinned_tile_backward(skl, shape, n):
+    """The reversed scan (Scan(+) adjoint) with small tiles."""
+    s = skl.Scan("float(float)", ADD_F, "0.0f")
+    _pin(s, *shape)
+    g = oracle.gen_f32_sym(0x5105, n)
+    d = np.asarray(s.backward(skl.Vector(g), inclusive=True)).astype(np.float64)
+    ref = oracle.scan_add_adjoint_f64(g, True)
+    tol = np.spacing(np.abs(ref).astype(np.float32)).astype(np.float64) + 1e-9
+    assert np.all(np.abs(d - ref) <= tol)
+
+
+def test_fresh_skeleton_uses_default_shape(skl):
+    """A new skeleton's first program is generated with its default launch
+    shape (the shape is compiled into the kernel)."""
+    import ctypes as C
+    from skelcl_amd import _lib
+    r = skl.Reduce("float(float)", ADD_F)
+    sp = C.c_void_p()
+    _lib.check(_lib.lib().skl_skeleton_source(r.handle, C.byref(sp)))
+    src = _lib.take_string(sp)
+    got = (C.c_int * 4)()
+    _lib.check(_lib.lib().skl_skeleton_launch_shape(r.handle, got))
+    assert f"#define SKL_UNROLL {got[0]}\n" in src
+
+
+def test_launch_shape_rejects_out_of_range(skl):
+    import ctypes as C
+    from skelcl_amd import _lib
+    s = skl.Scan("int(int)", ADD_I)
+    for bad in [(0, 1, 4, 4), (2, 0, 4, 4), (2, 1, 33, 4), (2, 1, 4, 0), (17, 1, 4, 4)]:
+        sh = (C.c_int * 4)(*bad)
+        assert _lib.lib().skl_skeleton_set_launch_shape(s.handle, sh) != 0, bad
