@@ -209,21 +209,24 @@ __device__ __forceinline__ void bf16x3_mm(f32x4 (&acc)[4][4], const Bf16x3Frag& 
       acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(f.a[i][pa], f.b[j][pb], acc[i][j], 0, 0, 0);
 }
 
-__global__ __launch_bounds__(256, 1) void gemm_bf16x3_kernel(
+// MB 128-row blocks of A per work-group tile (MB x 128 rows by 128
+// columns), 4 x MB waves of 64 x 64 (2 x MB down, 2 across).
+template <int MB>
+__global__ __launch_bounds__(256 * MB, 1) void gemm_bf16x3_kernel(
     const uint16_t* __restrict__ Ap, const uint16_t* __restrict__ Bp, float* __restrict__ C,
     int M, int N, int Kp, int ldc)
 {
   // Two slice buffers (the tiled blocks of A and B^T), filled by
   // global_load_lds; separate objects, so that the compiler's wait for the
   // fill of one does not block reads of the other.
-  __shared__ __attribute__((aligned(1024))) uint16_t As0[3 * SLICE], Bs0[3 * SLICE];
-  __shared__ __attribute__((aligned(1024))) uint16_t As1[3 * SLICE], Bs1[3 * SLICE];
+  __shared__ __attribute__((aligned(1024))) uint16_t As0[MB * 3 * SLICE], Bs0[3 * SLICE];
+  __shared__ __attribute__((aligned(1024))) uint16_t As1[MB * 3 * SLICE], Bs1[3 * SLICE];
   const int tid = threadIdx.x, lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int wm = wave >> 1, wn = wave & 1;
   int tm, tn;
   tile_coords(tm, tn);
-  const int m0 = tm * BM, n0 = tn * BN;
+  const int m0 = tm * (MB * BM), n0 = tn * BN;
   const int nk = Kp / BK;
 
   f32x4 acc0[4][4], acc1[4][4];
@@ -235,30 +238,38 @@ __global__ __launch_bounds__(256, 1) void gemm_bf16x3_kernel(
       acc1[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
     }
 
-  // Fill of slice s: per operand the 24 KB run of its three blocks, 24
-  // pieces of 1 KB, six per wave; lane l copies bytes 16 l .. 16 l + 15 of
-  // a piece.  Address = wave-uniform base + 32-bit lane offset (saddr form).
-  // Slices past the end are clamped to the last one: a harmless reload into
-  // a buffer that is not read again.
-  const char* ga = reinterpret_cast<const char*>(Ap) + (size_t)tm * nk * (3 * SLICE * 2);
-  const char* gb = reinterpret_cast<const char*>(Bp) + (size_t)tn * nk * (3 * SLICE * 2);
+  // Fill of slice s: per 128-row block the 24 KB run of its three planes, 24
+  // pieces of 1 KB; each wave copies 6 pieces of A and 6 / MB of B; lane l
+  // copies bytes 16 l .. 16 l + 15 of a piece.  Address = wave-uniform base +
+  // 32-bit lane offset (saddr form).  Slices past the end are clamped to the
+  // last one: a harmless reload into a buffer that is not read again.
+  constexpr size_t RUN = 3 * SLICE * 2;  // bytes per block and slice
+  const char* ga = reinterpret_cast<const char*>(Ap) + (size_t)tm * MB * nk * RUN;
+  const char* gb = reinterpret_cast<const char*>(Bp) + (size_t)tn * nk * RUN;
   const unsigned loff = (unsigned)lane * 16u;
   auto fill = [&](uint16_t* As, uint16_t* Bs, int sl) {
-    const size_t so = (size_t)(sl < nk ? sl : nk - 1) * (3 * SLICE * 2);
+    const size_t so = (size_t)(sl < nk ? sl : nk - 1) * RUN;
 #pragma unroll
     for (int j = 0; j < 6; ++j) {
-      const int piece = wave * 6 + j;
-      __builtin_amdgcn_global_load_lds(ga + so + piece * 1024 + loff,
+      const int piece = wave * 6 + j;  // 0 .. 24 MB - 1
+      const int blk = piece / 24, q = piece % 24;
+      __builtin_amdgcn_global_load_lds(ga + (size_t)blk * nk * RUN + so + q * 1024 + loff,
                                        (lds_ptr_t*)(As + piece * 512), 16, 0, 0);
+    }
+#pragma unroll
+    for (int j = 0; j < 6 / MB; ++j) {
+      const int piece = wave * (6 / MB) + j;  // 0 .. 23
       __builtin_amdgcn_global_load_lds(gb + so + piece * 1024 + loff,
                                        (lds_ptr_t*)(Bs + piece * 512), 16, 0, 0);
     }
   };
 
-  // MFMA fragment: lane holds row lane % 16, k-chunk lane / 16 of a 16-row tile
+  // MFMA fragment: lane holds row lane % 16, k-chunk lane / 16 of a 16-row
+  // tile; this wave's A rows lie in block wm / 2 of the tile
   const int fr = lane & 15;
   const int pos = (((lane >> 4) ^ swz(fr)) * 8);
-  const int arow = wm * 64 + fr, brow = wn * 64 + fr;
+  const int ablk = (wm >> 1) * 3 * SLICE;
+  const int arow = (wm & 1) * 64 + fr, brow = wn * 64 + fr;
 
   // Slice k from (Ac, Bc) with plane 0 already in f; (An, Bn) receives
   // slice k+1 (its fill was issued one slice earlier).
@@ -276,10 +287,10 @@ __global__ __launch_bounds__(256, 1) void gemm_bf16x3_kernel(
   auto slice = [&](uint16_t* Ac, uint16_t* Bc, const uint16_t* An, const uint16_t* Bn, int k,
                    Bf16x3Frag& f, Bf16x3Frag& g) {
     bf16x3_mm(acc0, f, 0, 0);
-    bf16x3_read(f, Ac, Bc, 1, arow, brow, pos);
+    bf16x3_read(f, Ac + ablk, Bc, 1, arow, brow, pos);
     bf16x3_mm(acc1, f, 1, 0);
     bf16x3_mm(acc1, f, 0, 1);
-    bf16x3_read(f, Ac, Bc, 2, arow, brow, pos);
+    bf16x3_read(f, Ac + ablk, Bc, 2, arow, brow, pos);
     bf16x3_mm(acc1, f, 1, 1);
 #pragma unroll
     for (int q = 0; q < 8; ++q) {
@@ -297,7 +308,7 @@ __global__ __launch_bounds__(256, 1) void gemm_bf16x3_kernel(
     __syncthreads();
     fill(Ac, Bc, k + 2);
     __builtin_amdgcn_sched_barrier(0);
-    bf16x3_read(g, An, Bn, 0, arow, brow, pos);
+    bf16x3_read(g, An + ablk, Bn, 0, arow, brow, pos);
     bf16x3_mm(acc1, f, 2, 0);
     bf16x3_mm(acc1, f, 0, 2);
 #pragma unroll
@@ -313,7 +324,7 @@ __global__ __launch_bounds__(256, 1) void gemm_bf16x3_kernel(
   fill(As1, Bs1, 1);
   __builtin_amdgcn_s_waitcnt(0);
   __syncthreads();
-  bf16x3_read(f0, As0, Bs0, 0, arow, brow, pos);
+  bf16x3_read(f0, As0 + ablk, Bs0, 0, arow, brow, pos);
   // nk is even (Kp % 64 == 0)
   for (int k = 0; k < nk; k += 2) {
     slice(As0, Bs0, As1, Bs1, k, f0, f1);
@@ -620,7 +631,7 @@ extern "C" size_t skl_internal_matmul_workspace(size_t M, size_t N, size_t K, in
   const size_t Mp = round_up(M, 128), Np = round_up(N, 128);
   if (precision == SKL_PREC_BF16X3) {
     const size_t Kp = round_up(K, 64);
-    return 3 * (Mp + Np) * Kp * 2 + 256;
+    return 3 * (round_up(M, 256) + Np) * Kp * 2 + 256;  // A rows in pairs of blocks
   }
   if (precision == SKL_PREC_FP8) {
     const size_t Kp = round_up(K, 128);
@@ -636,15 +647,16 @@ extern "C" int skl_internal_matmul(float* C, const float* A, const float* B, int
   const int Mp = (int)round_up(M, 128), Np = (int)round_up(N, 128);
   if (precision == SKL_PREC_BF16X3) {
     const int Kp = (int)round_up(K, 64);
+    const int Ma = (int)round_up(M, 256);  // A rows: whole pairs of 128-row blocks
     uint16_t* Ap = static_cast<uint16_t*>(ws);
-    uint16_t* Bp = Ap + 3 * (size_t)Mp * Kp;
+    uint16_t* Bp = Ap + 3 * (size_t)Ma * Kp;
     // tiled planes of A (M x K, or K x M when ta)
     if (!ta) {
-      const size_t tot = (size_t)Mp * (Kp / 8);
+      const size_t tot = (size_t)Ma * (Kp / 8);
       const unsigned g = (unsigned)std::min<size_t>((tot + 255) / 256, 65536);
-      split3_kernel<<<g, 256, 0, s>>>(A, M, K, Ap, Mp, Kp);
+      split3_kernel<<<g, 256, 0, s>>>(A, M, K, Ap, Ma, Kp);
     } else {
-      split3_transpose_kernel<<<dim3(Mp / 64, Kp / 32), 256, 0, s>>>(A, M, K, Ap, Mp, Kp);
+      split3_transpose_kernel<<<dim3(Ma / 64, Kp / 32), 256, 0, s>>>(A, M, K, Ap, Ma, Kp);
     }
     // tiled planes of B^T (B is K x N, or N x K when tb)
     if (tb) {
@@ -654,8 +666,21 @@ extern "C" int skl_internal_matmul(float* C, const float* A, const float* B, int
     } else {
       split3_transpose_kernel<<<dim3(Np / 64, Kp / 32), 256, 0, s>>>(B, N, K, Bp, Np, Kp);
     }
-    dim3 grid(Np / BN, Mp / BM);
-    gemm_bf16x3_kernel<<<grid, 256, 0, s>>>(Ap, Bp, C, M, N, Kp, N);
+    // 256 x 128 tiles (8 waves, twice the MFMA work per fill) when there are
+    // enough of them for every CU; 128 x 128 otherwise
+    static int cus = 0;
+    if (cus == 0) {
+      int dev = 0;
+      if (hipGetDevice(&dev) != hipSuccess ||
+          hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+          cus <= 0)
+        cus = 256;
+    }
+    if ((long long)(Ma / 256) * (Np / BN) >= cus) {
+      gemm_bf16x3_kernel<2><<<dim3(Np / BN, Ma / 256), 512, 0, s>>>(Ap, Bp, C, M, N, Kp, N);
+    } else {
+      gemm_bf16x3_kernel<1><<<dim3(Np / BN, Ma / 128), 256, 0, s>>>(Ap, Bp, C, M, N, Kp, N);
+    }
     return st(hipGetLastError());
   }
   if (precision == SKL_PREC_FP8) {
