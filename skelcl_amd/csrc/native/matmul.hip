@@ -96,8 +96,11 @@ __device__ __forceinline__ void split_store8(const float (&x)[8], uint16_t* out,
 }
 
 // X row-major R x Kc (ld = Kc) into the tiled planes, one thread per 8-k chunk.
+// `vec`: X is 16-byte aligned and Kc % 4 == 0, so each half of a chunk is one
+// float4 that is wholly inside or wholly outside the matrix.
 __global__ __launch_bounds__(256) void split3_kernel(const float* __restrict__ X, int R, int Kc,
-                                                     uint16_t* __restrict__ out, int Rp, int Kp)
+                                                     int vec, uint16_t* __restrict__ out,
+                                                     int Rp, int Kp)
 {
   const int nk = Kp / BK, cpr = Kp / 8;
   const size_t total = (size_t)Rp * cpr;
@@ -107,8 +110,20 @@ __global__ __launch_bounds__(256) void split3_kernel(const float* __restrict__ X
     const bool rok = r < R;
     const size_t base = (size_t)(rok ? r : 0) * Kc + k0;
     float x[8];
+    if (vec) {
+      const bool ok0 = rok && k0 + 4 <= Kc, ok1 = rok && k0 + 8 <= Kc;
+      const f32x4* X4 = reinterpret_cast<const f32x4*>(X);
+      const f32x4 a = as_global(X4)[ok0 ? base / 4 : 0];
+      const f32x4 b = as_global(X4)[ok1 ? base / 4 + 1 : 0];
 #pragma unroll
-    for (int i = 0; i < 8; ++i) x[i] = ld_or(X, base + i, 0, rok && k0 + i < Kc, 0.0f);
+      for (int i = 0; i < 4; ++i) {
+        x[i] = ok0 ? a[i] : 0.0f;
+        x[4 + i] = ok1 ? b[i] : 0.0f;
+      }
+    } else {
+#pragma unroll
+      for (int i = 0; i < 8; ++i) x[i] = ld_or(X, base + i, 0, rok && k0 + i < Kc, 0.0f);
+    }
     split_store8(x, out, tiled_offset(r, k0, nk));
   }
 }
@@ -654,7 +669,8 @@ extern "C" int skl_internal_matmul(float* C, const float* A, const float* B, int
     if (!ta) {
       const size_t tot = (size_t)Ma * (Kp / 8);
       const unsigned g = (unsigned)std::min<size_t>((tot + 255) / 256, 65536);
-      split3_kernel<<<g, 256, 0, s>>>(A, M, K, Ap, Ma, Kp);
+      const int vec = (uintptr_t)A % 16 == 0 && K % 4 == 0;
+      split3_kernel<<<g, 256, 0, s>>>(A, M, K, vec, Ap, Ma, Kp);
     } else {
       split3_transpose_kernel<<<dim3(Ma / 64, Kp / 32), 256, 0, s>>>(A, M, K, Ap, Ma, Kp);
     }
@@ -662,7 +678,8 @@ extern "C" int skl_internal_matmul(float* C, const float* A, const float* B, int
     if (tb) {
       const size_t tot = (size_t)Np * (Kp / 8);
       const unsigned g = (unsigned)std::min<size_t>((tot + 255) / 256, 65536);
-      split3_kernel<<<g, 256, 0, s>>>(B, N, K, Bp, Np, Kp);
+      const int vec = (uintptr_t)B % 16 == 0 && K % 4 == 0;
+      split3_kernel<<<g, 256, 0, s>>>(B, N, K, vec, Bp, Np, Kp);
     } else {
       split3_transpose_kernel<<<dim3(Np / 64, Kp / 32), 256, 0, s>>>(B, N, K, Bp, Np, Kp);
     }

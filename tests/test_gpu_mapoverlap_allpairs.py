@@ -157,6 +157,26 @@ def test_matmul_bf16x3_accuracy(skl, M, N, K):
     assert e3 <= max(e32, 2e-7)
 
 
+@pytest.mark.parametrize("ta,tb", [(0, 0), (1, 1)])
+def test_matmul_bf16x3_large_tiles_ragged(skl, ta, tb):
+    """Shapes with at least one 256 x 128 tile per CU take the 8-wave kernel:
+    ragged M, N, K (A padded to a pair of 128-row blocks), both transposes."""
+    M, N, K = 4100, 4000, 200
+    A = oracle.gen_f32_sym(0x4101, M * K).reshape((K, M) if ta else (M, K))
+    B = oracle.gen_f32_sym(0x4102, K * N).reshape((N, K) if tb else (K, N))
+    Am = np.ascontiguousarray(A.T if ta else A).astype(np.float64)
+    Bm = np.ascontiguousarray(B.T if tb else B).astype(np.float64)
+    ref = Am @ Bm
+    c3 = _matmul(skl, A, B, skl.PREC_BF16X3, ta, tb).astype(np.float64)
+    c32 = _matmul(skl, A, B, skl.PREC_FP32, ta, tb).astype(np.float64)
+    e3 = np.linalg.norm(c3 - ref) / np.linalg.norm(ref)
+    e32 = np.linalg.norm(c32 - ref) / np.linalg.norm(ref)
+    print(f"{M}x{N}x{K} ta={ta} tb={tb}: bf16x3 {e3:.3e}  fp32 {e32:.3e}")
+    assert e3 <= max(e32, 2e-7)
+    # every row and column is covered (no tile left out at the ragged edges)
+    assert np.abs(c3 - ref).max() <= 1e-5 * np.abs(ref).max()
+
+
 def test_matmul_bf16x3_integers_exact(skl):
     A, B = _ints(11, (96, 160)), _ints(12, (160, 80))
     c = _matmul(skl, A, B, skl.PREC_BF16X3)
