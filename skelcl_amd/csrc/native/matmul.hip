@@ -366,6 +366,17 @@ __global__ __launch_bounds__(256 * MB, 1) void gemm_bf16x3_kernel(
 // FP8 (OCP e4m3) with one scale per 128x128 tile.
 // Quantise X (R x C view, optionally transposed) into Rp x Cp bytes,
 // Rp % 128 == 0, Cp % 128 == 0; scales[(r/128) * (Cp/128) + c/128].
+// Tiled FP8 operand layout: blocks of 128 rows x 128 k-bytes (16 KB), block
+// index (row tile) * (Kp / 128) + (k block), so one k-block of one row tile
+// is a contiguous 16 KB run that global_load_lds copies in 1 KB pieces.
+// Inside a block a row is eight 16-byte chunks, chunk c at position
+// c ^ swz8(r): the 16 lanes of an MFMA fragment read (rows r0 .. r0+15, the
+// same chunk) then fall in 16 distinct 16-byte bank groups, for 32 or 64
+// banks (swz8 is a bijection on rows 0-7 and on the even and the odd rows of
+// 0-15).  It depends on bits 0-3 of the row only, so every 16-row fragment
+// of a block sees the same pattern.
+__device__ __forceinline__ int swz8(int r) { return (r ^ ((r >> 3) & 1)) & 7; }
+
 // One 128x128 tile of X (R x Cc row-major, or its transpose when `trans`)
 // per work-group: read once into registers, 4 consecutive elements of X's
 // contiguous dimension per load, absmax, a power-of-two scale s >=
@@ -430,88 +441,147 @@ __global__ __launch_bounds__(256) void quant_fp8_kernel(const float* __restrict_
     }
   }
   __syncthreads();
+  // tiled output: this tile is block tr * (Cp / 128) + tc, rows of 128
+  // bytes, chunk kc of row r at position kc ^ swz8(r)
+  uint8_t* blk = out + ((size_t)tr * (Cp / 128) + tc) * (128 * 128);
 #pragma unroll
   for (int j = 0; j < 4; ++j) {
     const int q = threadIdx.x + 256 * j;  // 128 rows x 8 chunks of 16 bytes
     const int lr = q >> 3, kc = q & 7;
-    *reinterpret_cast<i32x4*>(out + (size_t)(r0 + lr) * Cp + c0 + kc * 16) =
+    *reinterpret_cast<i32x4*>(blk + lr * 128 + ((kc ^ swz8(lr)) << 4)) =
         *reinterpret_cast<const i32x4*>(&qt[lr][kc * 16]);
   }
 }
 
-// Aq: Mp x Kp bytes, Bq: Np x Kp bytes (B^T); sa: (Mp/128) x (Kp/128),
-// sb: (Np/128) x (Kp/128).
-__global__ __launch_bounds__(256, 1) void gemm_fp8_kernel(
+// FP8 product on the tiled operands: Aq (Mp/128 row blocks), Bq (B^T,
+// Np/128 row blocks); sa: (Mp/128) x (Kp/128), sb: (Np/128) x (Kp/128).
+// Work-group tile MB x 128 rows by NB x 128 columns; waves of WR x 16 rows by
+// 64 columns (WR 16-row MFMA tiles, 4 16-column ones).  One 128-k block per
+// slice, two LDS buffers filled by global_load_lds (the fill of slice k+2 is
+// issued after the barrier that ends slice k, so it has one slice of MFMA
+// work to arrive).
+template <int MB, int NB, int WR>
+__global__ __launch_bounds__(MB * NB * 128 * 128 / (WR * 16 * 64) * 64, 1) void gemm_fp8_kernel(
     const uint8_t* __restrict__ Aq, const uint8_t* __restrict__ Bq,
     const float* __restrict__ sa, const float* __restrict__ sb, float* __restrict__ C, int M,
     int N, int Kp, int ldc)
 {
   constexpr int KB = 128;
-  constexpr int STR = KB + 16;  // bytes per LDS row
-  __shared__ __attribute__((aligned(16))) uint8_t As[BM][STR];
-  __shared__ __attribute__((aligned(16))) uint8_t Bs[BN][STR];
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int wm = wave >> 1, wn = wave & 1;
+  constexpr int BLK = 128 * KB;                    // bytes per block
+  constexpr int WAVES_N = NB * 2;                  // 64 columns per wave
+  constexpr int WAVES = (MB * 128 / (WR * 16)) * WAVES_N;
+  constexpr int PA = MB * (BLK / 1024) / WAVES;  // 1 KB pieces of A per wave per slice
+  constexpr int PB = NB * (BLK / 1024) / WAVES;  // and of B
+  static_assert(PA * WAVES == MB * (BLK / 1024) && PB * WAVES == NB * (BLK / 1024),
+                "fill pieces must split evenly over the waves");
+  __shared__ __attribute__((aligned(1024))) uint8_t As0[MB * BLK], Bs0[NB * BLK];
+  __shared__ __attribute__((aligned(1024))) uint8_t As1[MB * BLK], Bs1[NB * BLK];
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int wm = wave / WAVES_N, wn = wave % WAVES_N;
   int tm, tn;
   tile_coords(tm, tn);
-  const int m0 = tm * BM, n0 = tn * BN;
+  const int m0 = tm * (MB * 128), n0 = tn * (NB * 128);
   const int nkb = Kp / KB;
-  f32x4 acc[4][4];
+  f32x4 acc[WR][4];
 #pragma unroll
-  for (int i = 0; i < 4; ++i)
+  for (int i = 0; i < WR; ++i)
 #pragma unroll
     for (int j = 0; j < 4; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
-  // 128 rows x 128 bytes = 1024 chunks of 16 B per operand, 4 per thread
-  i32x4 ra[4], rb[4];
-  auto gload = [&](int kb) {
+
+  // fill of k-block s: the 16 MB pieces of 1 KB of A (block piece / 16) and
+  // the 16 NB of B, PA and PB per wave; lane l copies bytes 16 l .. 16 l + 15
+  // of a piece.  Slices past the end are clamped to the last one.
+  const unsigned loff = (unsigned)lane * 16u;
+  auto fill = [&](uint8_t* As, uint8_t* Bs, int sl) {
+    const size_t so = (size_t)(sl < nkb ? sl : nkb - 1) * BLK;
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const int c = tid + 256 * j;
-      const int r = c >> 3, kc = c & 7;
-      ra[j] = *reinterpret_cast<const i32x4*>(Aq + (size_t)(m0 + r) * Kp + kb * KB + kc * 16);
-      rb[j] = *reinterpret_cast<const i32x4*>(Bq + (size_t)(n0 + r) * Kp + kb * KB + kc * 16);
+    for (int j = 0; j < PA; ++j) {
+      const int piece = wave * PA + j, b = piece >> 4, q = piece & 15;
+      __builtin_amdgcn_global_load_lds(
+          Aq + ((size_t)(tm * MB + b) * nkb) * BLK + so + q * 1024 + loff,
+          (lds_ptr_t*)(As + piece * 1024), 16, 0, 0);
+    }
+#pragma unroll
+    for (int j = 0; j < PB; ++j) {
+      const int piece = wave * PB + j, b = piece >> 4, q = piece & 15;
+      __builtin_amdgcn_global_load_lds(
+          Bq + ((size_t)(tn * NB + b) * nkb) * BLK + so + q * 1024 + loff,
+          (lds_ptr_t*)(Bs + piece * 1024), 16, 0, 0);
     }
   };
-  gload(0);
-  for (int kb = 0; kb < nkb; ++kb) {
+
+  // fragment: lane holds row lane % 16, k bytes 32 (lane / 16) .. + 31, i.e.
+  // chunks 2 (lane / 16) and 2 (lane / 16) + 1 of that row
+  const int fr = lane & 15, c0 = (lane >> 4) * 2;
+  const int sw = swz8(fr);
+  const int p0 = ((c0 ^ sw) << 4), p1 = (((c0 + 1) ^ sw) << 4);
+  const int arow0 = wm * (WR * 16);                // within the A tile
+  const int bblk = (wn >> 1) * BLK, brow0 = (wn & 1) * 64;
+  auto rdA = [&](const uint8_t* As, int i) {
+    const int r = arow0 + i * 16 + fr;
+    const uint8_t* row = As + (r >> 7) * BLK + (r & 127) * 128;
+    const i32x4 lo = *reinterpret_cast<const i32x4*>(row + p0);
+    const i32x4 hi = *reinterpret_cast<const i32x4*>(row + p1);
+    return i32x8{lo[0], lo[1], lo[2], lo[3], hi[0], hi[1], hi[2], hi[3]};
+  };
+  auto rdB = [&](const uint8_t* Bs, int j) {
+    const uint8_t* row = Bs + bblk + (brow0 + j * 16 + fr) * 128;
+    const i32x4 lo = *reinterpret_cast<const i32x4*>(row + p0);
+    const i32x4 hi = *reinterpret_cast<const i32x4*>(row + p1);
+    return i32x8{lo[0], lo[1], lo[2], lo[3], hi[0], hi[1], hi[2], hi[3]};
+  };
+  // the tile scales are powers of two: their exponent fields are the E8M0
+  // block scales of the MFMA, which then accumulates directly
+  const int ablock = tm * MB + (arow0 >> 7), bblock = tn * NB + (wn >> 1);
+  auto slice = [&](uint8_t* Ac, uint8_t* Bc, int k) {
+    const int ea = (int)((__float_as_uint(sa[(size_t)ablock * nkb + k]) >> 23) & 0xFF);
+    const int eb = (int)((__float_as_uint(sb[(size_t)bblock * nkb + k]) >> 23) & 0xFF);
+    // B fragments for the whole slice; A fragments one ahead of their four
+    // MFMAs (scheduling barriers keep the compiler from hoisting all of them,
+    // which would not fit in the registers)
+    i32x8 b[4];
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const int c = tid + 256 * j;
-      const int r = c >> 3, kc = c & 7;
-      *reinterpret_cast<i32x4*>(&As[r][kc * 16]) = ra[j];
-      *reinterpret_cast<i32x4*>(&Bs[r][kc * 16]) = rb[j];
-    }
-    __syncthreads();
-    if (kb + 1 < nkb) gload(kb + 1);
-    // the tile scales are powers of two: their exponent fields are the E8M0
-    // block scales of the MFMA, which then accumulates directly
-    const int ea = (int)((__float_as_uint(sa[(m0 / 128) * nkb + kb]) >> 23) & 0xFF);
-    const int eb = (int)((__float_as_uint(sb[(n0 / 128) * nkb + kb]) >> 23) & 0xFF);
-    // operand fragment: lane holds row lane%16, k bytes 32*(lane/16)..+31
-    const int fr = lane & 15, fk = (lane >> 4) * 32;
-    i32x8 a[4], b[4];
+    for (int j = 0; j < 4; ++j) b[j] = rdB(Bc, j);
+    i32x8 a = rdA(Ac, 0);
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      a[i] = *reinterpret_cast<const i32x8*>(&As[wm * 64 + i * 16 + fr][fk]);
-      b[i] = *reinterpret_cast<const i32x8*>(&Bs[wn * 64 + i * 16 + fr][fk]);
-    }
-#pragma unroll
-    for (int i = 0; i < 4; ++i)
+    for (int i = 0; i < WR; ++i) {
+      const i32x8 an = i + 1 < WR ? rdA(Ac, i + 1) : a;
 #pragma unroll
       for (int j = 0; j < 4; ++j)  // format 0 = e4m3 for both operands
-        acc[i][j] = __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(a[i], b[j], acc[i][j], 0, 0,
+        acc[i][j] = __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(a, b[j], acc[i][j], 0, 0,
                                                                      0, ea, 0, eb);
+      __builtin_amdgcn_sched_barrier(0);
+      a = an;
+    }
+    __builtin_amdgcn_sched_barrier(0);
+    __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
+    fill(Ac, Bc, k + 2);
+    __builtin_amdgcn_sched_barrier(0);
+  };
+
+  fill(As0, Bs0, 0);
+  fill(As1, Bs1, 1);
+  __builtin_amdgcn_s_waitcnt(0);
+  __syncthreads();
+  int k = 0;
+  for (; k + 2 <= nkb; k += 2) {
+    slice(As0, Bs0, k);
+    slice(As1, Bs1, k + 1);
   }
+  if (k < nkb) slice(As0, Bs0, k);
+  __builtin_amdgcn_s_waitcnt(0);  // no fill may still be writing LDS at the end
+
   const int cc = lane & 15, cr = (lane >> 4) * 4;
 #pragma unroll
-  for (int i = 0; i < 4; ++i)
+  for (int i = 0; i < WR; ++i)
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
       const int col = n0 + wn * 64 + j * 16 + cc;
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int row = m0 + wm * 64 + i * 16 + cr + r;
+        const int row = m0 + arow0 + i * 16 + cr + r;
         if (row < M && col < N) C[(size_t)row * ldc + col] = acc[i][j][r];
       }
     }
@@ -640,6 +710,17 @@ int st(hipError_t e) { return e == hipSuccess ? SKL_OK : SKL_ERR_HIP; }
 
 }  // namespace
 
+// Compute units of the current device (256 if the query fails).
+static int device_cus()
+{
+  int dev = 0, cus = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+      cus <= 0)
+    cus = 256;
+  return cus;
+}
+
 // Workspace sizes (bytes) of the MFMA forms for an M x N x K product.
 extern "C" size_t skl_internal_matmul_workspace(size_t M, size_t N, size_t K, int precision)
 {
@@ -649,8 +730,8 @@ extern "C" size_t skl_internal_matmul_workspace(size_t M, size_t N, size_t K, in
     return 3 * (round_up(M, 256) + Np) * Kp * 2 + 256;  // A rows in pairs of blocks
   }
   if (precision == SKL_PREC_FP8) {
-    const size_t Kp = round_up(K, 128);
-    return (Mp + Np) * Kp + ((Mp + Np) / 128) * (Kp / 128) * 4 + 512;
+    const size_t Kp = round_up(K, 128), Ma = round_up(M, 256), Na = round_up(N, 256);
+    return (Ma + Na) * Kp + ((Ma + Na) / 128) * (Kp / 128) * 4 + 512;
   }
   return 0;
 }
@@ -685,15 +766,7 @@ extern "C" int skl_internal_matmul(float* C, const float* A, const float* B, int
     }
     // 256 x 128 tiles (8 waves, twice the MFMA work per fill) when there are
     // enough of them for every CU; 128 x 128 otherwise
-    static int cus = 0;
-    if (cus == 0) {
-      int dev = 0;
-      if (hipGetDevice(&dev) != hipSuccess ||
-          hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
-          cus <= 0)
-        cus = 256;
-    }
-    if ((long long)(Ma / 256) * (Np / BN) >= cus) {
+    if ((long long)(Ma / 256) * (Np / BN) >= device_cus()) {
       gemm_bf16x3_kernel<2><<<dim3(Np / BN, Ma / 256), 512, 0, s>>>(Ap, Bp, C, M, N, Kp, N);
     } else {
       gemm_bf16x3_kernel<1><<<dim3(Np / BN, Ma / 128), 256, 0, s>>>(Ap, Bp, C, M, N, Kp, N);
@@ -702,18 +775,25 @@ extern "C" int skl_internal_matmul(float* C, const float* A, const float* B, int
   }
   if (precision == SKL_PREC_FP8) {
     const int Kp = (int)round_up(K, 128);
+    const int Ma = (int)round_up(M, 256), Na = (int)round_up(N, 256);  // pairs of blocks
     uint8_t* Aq = static_cast<uint8_t*>(ws);
-    uint8_t* Bq = Aq + (size_t)Mp * Kp;
-    float* sa = reinterpret_cast<float*>(Bq + (size_t)Np * Kp);
-    float* sb = sa + (size_t)(Mp / 128) * (Kp / 128);
+    uint8_t* Bq = Aq + (size_t)Ma * Kp;
+    float* sa = reinterpret_cast<float*>(Bq + (size_t)Na * Kp);
+    float* sb = sa + (size_t)(Ma / 128) * (Kp / 128);
     const int va = ((uintptr_t)A % 16 == 0) && ((ta ? M : K) % 4 == 0);
     const int vb = ((uintptr_t)B % 16 == 0) && ((tb ? K : N) % 4 == 0);
-    quant_fp8_kernel<<<dim3(Kp / 128, Mp / 128), 256, 0, s>>>(A, M, K, ta, va, Aq, sa, Mp, Kp);
+    quant_fp8_kernel<<<dim3(Kp / 128, Ma / 128), 256, 0, s>>>(A, M, K, ta, va, Aq, sa, Ma, Kp);
     // B^T: rows n, columns k; element (n, k) = B[k][n] (or B[n][k] when tb)
-    quant_fp8_kernel<<<dim3(Kp / 128, Np / 128), 256, 0, s>>>(B, N, K, tb ? 0 : 1, vb, Bq, sb, Np,
+    quant_fp8_kernel<<<dim3(Kp / 128, Na / 128), 256, 0, s>>>(B, N, K, tb ? 0 : 1, vb, Bq, sb, Na,
                                                                Kp);
-    dim3 grid(Np / BN, Mp / BM);
-    gemm_fp8_kernel<<<grid, 256, 0, s>>>(Aq, Bq, sa, sb, C, M, N, Kp, N);
+    // 256 x 128 tiles (8 waves of 64 x 64) when every CU gets one
+    if ((long long)(Ma / 256) * (Na / 128) >= device_cus()) {
+      gemm_fp8_kernel<2, 1, 4><<<dim3(Na / 128, Ma / 256), 512, 0, s>>>(Aq, Bq, sa, sb, C, M, N,
+                                                                         Kp, N);
+    } else {
+      gemm_fp8_kernel<1, 1, 4><<<dim3(Na / 128, Ma / 128), 256, 0, s>>>(Aq, Bq, sa, sb, C, M, N,
+                                                                        Kp, N);
+    }
     return st(hipGetLastError());
   }
   dim3 grid((N + BN - 1) / BN, (M + BM - 1) / BM);
