@@ -34,6 +34,11 @@ SEED_SCAN, SEED_RED, SEED_X, SEED_Y = 0x2001, 0x2002, 0x2003, 0x2004
 SAXPY_A = 2.5
 
 
+def progress(msg: str) -> None:
+    """One line per finished stage on stderr (the JSON result goes to stdout)."""
+    print(f"bench: {msg}", file=sys.stderr, flush=True)
+
+
 def cpu_baseline(nthreads: int):
     """The oracle (reference-order CPU restatement) on the host cores: one
     step of the same workload at full size, timed per skeleton."""
@@ -148,6 +153,7 @@ def run_extras(skelcl, L, ck, stream, args):
         "int_reduce_exact": bool(np.uint32(got_int & 0xFFFFFFFF)
                                  == oracle.reduce_add_u32_reforder(k1)),
     }
+    progress("dot_2^20 done")
 
     # ---- host buffers handed over: saxpy 2^26 through the Vector API
     # (upload, kernel, download; pageable numpy memory), wall clock
@@ -166,6 +172,7 @@ def run_extras(skelcl, L, ck, stream, args):
         "wall_ms": round(w * 1e3, 3), "GBps_algorithmic": round(12 * n2 / w / 1e9, 1),
         "bit_exact_vs_fma": bool(np.array_equal(res, oracle.saxpy_f32_fma(SAXPY_A, hx, hy))),
     }
+    progress("saxpy_2^26_host_buffers done")
     del hx, hy, res
 
     # ---- configuration 3: MapOverlap 8192^2 Gaussian R=5 NEUTRAL + adjoint
@@ -209,6 +216,7 @@ def run_extras(skelcl, L, ck, stream, args):
         "max_rel_err_vs_fp64": float(np.max(np.abs(o[pix] - ref) / np.maximum(1, np.abs(ref)))),
         "adjoint_max_abs_err_vs_fp64": float(np.max(np.abs(d[pix] - dref))),
     }
+    progress("mapoverlap_8192^2_r5 done")
     del img, Wb, outb, dinb, o, d, himg, hW
 
     # ---- configuration 4: AllPairs 8192^3 (FP32 / BF16X3 / FP8), sq-Euclidean
@@ -267,6 +275,7 @@ def run_extras(skelcl, L, ck, stream, args):
                                "max_rel_err_vs_fp64": float(np.max(np.abs(dg - dref) /
                                                                    np.maximum(1, dref)))}
     ex["allpairs_8192^3"] = ap
+    progress("allpairs_8192^3 done")
     del P, Pt, D
 
     # ---- configuration 5: batched segmented Reduce/Scan + 512 small products
@@ -322,6 +331,7 @@ def run_extras(skelcl, L, ck, stream, args):
                                                      for s in chk)),
         "gemms": len(shapes), "gemm_flop": flops, "gemm": bt,
     }
+    progress("batched done")
     return ex
 
 
@@ -385,6 +395,7 @@ def main() -> int:
             sh = (C.c_int * 4)()
             ck(L.skl_skeleton_launch_shape(sk.handle, sh))
             shapes[name] = dict(zip(("unroll", "blocks_per_cu", "scan_j", "mo_rb"), list(sh)))
+        progress("launch shapes tuned")
 
     def run_scan():
         ck(L.skl_scan_run(scan.handle, sout.ptr, xs.ptr, N, 1, None, 0, st))
@@ -436,6 +447,7 @@ def main() -> int:
     if args.copy_calibration:  # profiling run: the step is the workload legs only
         ms_per_step = sum(statistics.mean(per[name]) for name, _, _ in legs if name != "copy")
     value = total_bytes / (ms_per_step * 1e-3) / 1e9
+    progress("timed steps done")
     legs_out = {}
     for name, _, b in legs:
         avg = statistics.mean(per[name])
@@ -483,6 +495,7 @@ def main() -> int:
     if not args.no_cpu:
         nthreads = min(os.cpu_count() or 1, 16)
         cb = cpu_baseline(nthreads)
+        progress("cpu baseline done")
         ref64 = cb["reduce_fp64"]
         result["err_vs_fp64"] = {
             "reduce_gpu_abs": abs(got_red - ref64), "reduce_cpu_reforder_abs":
