@@ -350,6 +350,63 @@ __device__ __forceinline__ void skl_store_pack(P* dst, const P& v)
   *dst = v;
 }
 
+// Pack load / store at an element address that is only 4-byte aligned
+// (sizeof(T) % 4 == 0): one dword-aligned vector access, which gfx9 memory
+// instructions accept.
+template <typename P, typename T>
+__device__ __forceinline__ P skl_load_pack_u(const T* src)
+{
+  static_assert(sizeof(T) % 4 == 0, "dword-aligned packs need 4-byte elements");
+  P v;
+  if constexpr (sizeof(P) == 16) {
+    typedef unsigned int u4a __attribute__((ext_vector_type(4), aligned(4)));
+#ifdef SKL_NT_LOAD
+    const u4a t = __builtin_nontemporal_load(reinterpret_cast<const u4a*>(src));
+#else
+    const u4a t = *reinterpret_cast<const u4a*>(src);
+#endif
+    __builtin_memcpy(&v, &t, 16);
+  } else if constexpr (sizeof(P) == 8) {
+    typedef unsigned int u2a __attribute__((ext_vector_type(2), aligned(4)));
+#ifdef SKL_NT_LOAD
+    const u2a t = __builtin_nontemporal_load(reinterpret_cast<const u2a*>(src));
+#else
+    const u2a t = *reinterpret_cast<const u2a*>(src);
+#endif
+    __builtin_memcpy(&v, &t, 8);
+  } else {
+    __builtin_memcpy(&v, src, sizeof(P));
+  }
+  return v;
+}
+
+template <typename P, typename T>
+__device__ __forceinline__ void skl_store_pack_u(T* dst, const P& v)
+{
+  static_assert(sizeof(T) % 4 == 0, "dword-aligned packs need 4-byte elements");
+  if constexpr (sizeof(P) == 16) {
+    typedef unsigned int u4a __attribute__((ext_vector_type(4), aligned(4)));
+    u4a t;
+    __builtin_memcpy(&t, &v, 16);
+#ifdef SKL_NT_STORE
+    __builtin_nontemporal_store(t, reinterpret_cast<u4a*>(dst));
+#else
+    *reinterpret_cast<u4a*>(dst) = t;
+#endif
+  } else if constexpr (sizeof(P) == 8) {
+    typedef unsigned int u2a __attribute__((ext_vector_type(2), aligned(4)));
+    u2a t;
+    __builtin_memcpy(&t, &v, 8);
+#ifdef SKL_NT_STORE
+    __builtin_nontemporal_store(t, reinterpret_cast<u2a*>(dst));
+#else
+    *reinterpret_cast<u2a*>(dst) = t;
+#endif
+  } else {
+    __builtin_memcpy(dst, &v, sizeof(P));
+  }
+}
+
 // Double-float pair used by the compensated float paths.
 struct skl_df {
   float hi, lo;

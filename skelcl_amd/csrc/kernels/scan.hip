@@ -121,17 +121,28 @@ extern "C" __global__ __launch_bounds__(SKL_WG) SKL_SCAN_ATTR void skl_scan(
   const unsigned long long wbase = tbase + (unsigned long long)wv * WAVE_ELEMS;
 #ifdef SKL_SCAN_REVERSE
   // scan read from the end (the adjoint of a scan): element e of the scan
-  // is element n-1-e of the arrays.  When n % V == 0 the V elements of a
-  // pack, e0 .. e0+V-1, are the aligned array pack at n-e0-V read backwards.
-  const bool full = tbase + TILE <= n && n % SKL_V == 0;
+  // is element n-1-e of the arrays.  The V elements of a pack, e0 .. e0+V-1,
+  // are the array run at n-e0-V read backwards: an aligned pack when
+  // n % V == 0, otherwise only element-aligned, which for 4-byte multiples
+  // is still one dword-aligned vector access.
+  constexpr bool SKL_REV_ANY = sizeof(SKL_T) % 4 == 0;
+  const bool full = tbase + TILE <= n && (SKL_REV_ANY || n % SKL_V == 0);
 #define SKL_RIDX(e) (n - 1 - (e))
-#define SKL_PIDX(e0) ((n - (e0) - SKL_V) / SKL_V)
 #define SKL_PCOMP(k) (SKL_V - 1 - (k))
+#define SKL_PLOAD(e0)                                                              \
+  (SKL_REV_ANY ? skl_load_pack_u<pack_t>(in_ + (n - (e0) - SKL_V))                   \
+               : skl_load_pack(reinterpret_cast<const pack_t*>(in_) + (n - (e0) - SKL_V) / SKL_V))
+#define SKL_PSTORE(e0, w)                                                          \
+  do {                                                                             \
+    if constexpr (SKL_REV_ANY) skl_store_pack_u(out_ + (n - (e0) - SKL_V), (w));     \
+    else skl_store_pack(reinterpret_cast<pack_t*>(out_) + (n - (e0) - SKL_V) / SKL_V, (w)); \
+  } while (0)
 #else
   const bool full = tbase + TILE <= n;
 #define SKL_RIDX(e) (e)
-#define SKL_PIDX(e0) ((e0) / SKL_V)
 #define SKL_PCOMP(k) (k)
+#define SKL_PLOAD(e0) skl_load_pack(reinterpret_cast<const pack_t*>(in_) + (e0) / SKL_V)
+#define SKL_PSTORE(e0, w) skl_store_pack(reinterpret_cast<pack_t*>(out_) + (e0) / SKL_V, (w))
 #endif
 
   // ---- load + per-lane inclusive scan of each pack
@@ -140,7 +151,7 @@ extern "C" __global__ __launch_bounds__(SKL_WG) SKL_SCAN_ATTR void skl_scan(
     pack_t pk[SKL_J];
 #pragma unroll
     for (int j = 0; j < SKL_J; ++j)
-      pk[j] = skl_load_pack(reinterpret_cast<const pack_t*>(in_) + SKL_PIDX(wbase + ((unsigned long long)j * 64 + lane) * SKL_V));
+      pk[j] = SKL_PLOAD(wbase + ((unsigned long long)j * 64 + lane) * SKL_V);
 #pragma unroll
     for (int j = 0; j < SKL_J; ++j)
 #pragma unroll
@@ -283,7 +294,7 @@ extern "C" __global__ __launch_bounds__(SKL_WG) SKL_SCAN_ATTR void skl_scan(
       pack_t w;
 #pragma unroll
       for (int k = 0; k < SKL_V; ++k) w.v[SKL_PCOMP(k)] = o.v[k];
-      skl_store_pack(reinterpret_cast<pack_t*>(out_) + SKL_PIDX(e0), w);
+      SKL_PSTORE(e0, w);
     } else {
 #pragma unroll
       for (int k = 0; k < SKL_V; ++k)

@@ -49,9 +49,12 @@ def test_reduce_backward_broadcast(skl):
     assert d.shape == (12345,) and np.all(d == np.float32(0.75))
 
 
-@pytest.mark.parametrize("n", [1, 1000, 4097, 300001, 1 << 20, 65540])
+@pytest.mark.parametrize("n", [1, 1000, 4097, 300001, 1 << 20, 65540, 3 * (1 << 20) + 2,
+                               5 * (1 << 20) + 3])
 @pytest.mark.parametrize("inclusive", [False, True])
 def test_scan_backward_reversed_scan(skl, n, inclusive):
+    """Reversed scan; lengths with n % 4 != 0 read and write their full tiles
+    as dword-aligned packs."""
     s = skl.Scan("float(float)", "float func(float x, float y){ return x+y; }", "0.0f")
     g = oracle.gen_f32_sym(0x5005, n)
     d = np.asarray(s.backward(skl.Vector(g), inclusive=inclusive)).astype(np.float64)
