@@ -511,7 +511,9 @@ static int setup_skeleton(const skl_skeleton_desc* d, skl_skeleton_s* s)
   s->func = str_or(d->func, "func");
   s->identity = str_or(d->identity, "0");
   s->neutral = str_or(d->neutral, "0");
-  s->wg = d->work_group_size > 0 ? d->work_group_size : 256;
+  // Scan: 512-thread tiles (measured on MI355X at 2^28: int32 5.56 TB/s
+  // against 4.83 with 256 threads, float 13% faster); everything else 256
+  s->wg = d->work_group_size > 0 ? d->work_group_size : (s->kind == SKL_SCAN ? 512 : 256);
   if (s->wg > 1024) return fail(SKL_ERR_INVALID, "work-group size %d exceeds 1024", s->wg);
   s->precision = d->precision;
   s->overlap = d->overlap_range;
@@ -997,7 +999,9 @@ static int scan_launch(skl_skeleton s, const std::string& key, void* out, const 
   size_t al = s->V * s->types[0].size;
   if (!aligned(in, al) || !aligned(out, al))
     return fail(SKL_ERR_INVALID, "Scan buffers must be aligned to %zu bytes", al);
-  unsigned long long tile = 256ull * s->scan_j * s->V;
+  if (s->wg % 64 != 0)
+    return fail(SKL_ERR_INVALID, "Scan work-group size %d is not a multiple of 64", s->wg);
+  unsigned long long tile = (unsigned long long)s->wg * s->scan_j * s->V;
   unsigned long long tiles = (n + tile - 1) / tile;
   if (tiles > 0xFFFFFFFFull) return fail(SKL_ERR_INVALID, "Scan input too large");
   size_t words = (carry_size(s) + 3) / 4;
@@ -1009,7 +1013,7 @@ static int scan_launch(skl_skeleton s, const std::string& key, void* out, const 
   skl_kernel k;
   SKL_TRY(get_kernel(s, key, "skl_scan", &k));
   auto a = with_extras({&in, &out, &nn, &incl, &lb, &ctl}, pk);
-  return launch(k, (unsigned)tiles, 1, 256, 1, 0, a, stream);
+  return launch(k, (unsigned)tiles, 1, (unsigned)s->wg, 1, 0, a, stream);
 }
 
 extern "C" int skl_mapoverlap_run(skl_skeleton s, void* out, const void* in, size_t rows,
