@@ -776,7 +776,8 @@ extern "C" int skl_map_run(skl_skeleton s, void* out, const void* in, size_t n,
   if (fast) {
     SKL_TRY(get_kernel(s, "main", "skl_map", &k));
     unsigned long long npk = n / s->V;
-    return launch(k, blocks_for(npk, 256ull * s->unroll, s->max_blocks), 1, 256, 1, 0, a, stream);
+    return launch(k, blocks_for(npk, (unsigned long long)s->wg * s->unroll, s->max_blocks), 1,
+                  s->wg, 1, 0, a, stream);
   }
   SKL_TRY(get_kernel(s, "main", "skl_map_simple", &k));
   return launch(k, blocks_for(n, s->wg, 0xFFFFFFFFull), 1, s->wg, 1, pk.dyn_lds, a, stream);
@@ -801,7 +802,7 @@ extern "C" int skl_map_index_run(skl_skeleton s, void* out, size_t n, size_t off
   auto a = with_extras({&out, &nn, &off}, pk);
   if (!s->simple && pk.dyn_lds == 0 && aligned(out, s->V * s->types[0].size)) {
     SKL_TRY(get_kernel(s, "main", "skl_map_index", &k));
-    return launch(k, blocks_for(n / s->V, 256, s->max_blocks), 1, 256, 1, 0, a, stream);
+    return launch(k, blocks_for(n / s->V, s->wg, s->max_blocks), 1, s->wg, 1, 0, a, stream);
   }
   SKL_TRY(get_kernel(s, "main", "skl_map_index_simple", &k));
   return launch(k, blocks_for(n, s->wg, 0xFFFFFFFFull), 1, s->wg, 1, pk.dyn_lds, a, stream);
@@ -856,8 +857,8 @@ extern "C" int skl_zip_run(skl_skeleton s, void* out, const void* left, const vo
   auto a = with_extras({&left, &right, &out, &nn}, pk);
   if (fast) {
     SKL_TRY(get_kernel(s, "main", "skl_zip", &k));
-    return launch(k, blocks_for(n / s->V, 256ull * s->unroll, s->max_blocks), 1, 256, 1, 0, a,
-                  stream);
+    return launch(k, blocks_for(n / s->V, (unsigned long long)s->wg * s->unroll, s->max_blocks),
+                  1, s->wg, 1, 0, a, stream);
   }
   SKL_TRY(get_kernel(s, "main", "skl_zip_simple", &k));
   return launch(k, blocks_for(n, s->wg, 0xFFFFFFFFull), 1, s->wg, 1, pk.dyn_lds, a, stream);
@@ -876,7 +877,10 @@ static int reduce_launch(skl_skeleton_s* s, const std::string& key, void* out,
   if (!out) return fail(SKL_ERR_INVALID, "NULL output buffer");
   size_t csz = (carry_size(s) + 7) & ~(size_t)7;
   unsigned long long npk = (n + s->V - 1) / s->V;
-  unsigned blocks = blocks_for(npk, 256, (unsigned long long)s->max_blocks);
+  if (s->wg > 64 && s->wg % 64 != 0)
+    return fail(SKL_ERR_INVALID, "Reduce work-group size %d: use at most 64 or a multiple of 64",
+                s->wg);
+  unsigned blocks = blocks_for(npk, s->wg, (unsigned long long)s->max_blocks);
   // scratch: top counter at 0, 64 sub-counters 256 bytes apart, partials
   constexpr size_t kCounters = 256 * 65;
   SKL_TRY(ensure_scratch(s, kCounters + (size_t)s->max_blocks * csz));
@@ -893,7 +897,7 @@ static int reduce_launch(skl_skeleton_s* s, const std::string& key, void* out,
   a.push_back(&partials);
   a.push_back(&ticket);
   a.insert(a.end(), pk.ptrs.begin(), pk.ptrs.end());
-  return launch(k, blocks, 1, 256, 1, pk.dyn_lds, a, stream);
+  return launch(k, blocks, 1, s->wg, 1, pk.dyn_lds, a, stream);
 }
 
 extern "C" int skl_reduce_run(skl_skeleton s, void* out, const void* in, size_t n,
@@ -966,7 +970,8 @@ int skl_run_backward_map(skl_skeleton s, void* d_in, const void* in, const void*
   skl_kernel k;
   SKL_TRY(get_kernel(s, "backward", "skl_map_backward", &k));
   auto a = with_extras({&in, &d_out, &d_in, &nn}, pk);
-  return launch(k, blocks_for(n, 256ull * 4, s->max_blocks), 1, 256, 1, pk.dyn_lds, a, stream);
+  return launch(k, blocks_for(n, (unsigned long long)s->wg * 4, s->max_blocks), 1, s->wg, 1,
+                pk.dyn_lds, a, stream);
 }
 
 int skl_run_backward_zip(skl_skeleton s, void* d_left, void* d_right, const void* left,
@@ -984,7 +989,8 @@ int skl_run_backward_zip(skl_skeleton s, void* d_left, void* d_right, const void
   skl_kernel k;
   SKL_TRY(get_kernel(s, "backward", "skl_zip_backward", &k));
   auto a = with_extras({&left, &right, &d_out, &d_left, &d_right, &nn}, pk);
-  return launch(k, blocks_for(n, 256ull * 4, s->max_blocks), 1, 256, 1, pk.dyn_lds, a, stream);
+  return launch(k, blocks_for(n, (unsigned long long)s->wg * 4, s->max_blocks), 1, s->wg, 1,
+                pk.dyn_lds, a, stream);
 }
 
 static int scan_launch(skl_skeleton s, const std::string& key, void* out, const void* in,

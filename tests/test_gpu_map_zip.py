@@ -181,3 +181,33 @@ def test_generators_bit_exact(skl):
     assert np.array_equal(b.to_numpy(np.float32, n), oracle.gen_f32_unit(0x3001, n, 255.0))
     _lib.check(_lib.lib().skl_generate_i32_below(b.ptr, n, 0x2001, 1000, None))
     assert np.array_equal(b.to_numpy(np.int32, n), oracle.gen_i32_below(0x2001, n, 1000))
+
+
+@pytest.mark.parametrize("wg", [64, 128, 512, 1024])
+def test_pinned_work_group_sizes(skl, wg):
+    """Every streaming skeleton launches with the work-group size it was
+    compiled for (setWorkGroupSize): exact results at 64 .. 1024 threads."""
+    import oracle
+    n = 1000003
+    a, b = oracle.gen_f32_sym(71, n), oracle.gen_f32_sym(72, n)
+    m = skl.Map("float(float)", "float func(float x){ return -x; }")
+    m.set_work_group_size(wg)
+    assert np.array_equal(np.asarray(m(skl.Vector(a))), -a)
+    z = skl.Zip("float(float, float)", "float func(float x, float y){ return x + y; }")
+    z.set_work_group_size(wg)
+    assert np.array_equal(np.asarray(z(skl.Vector(a), skl.Vector(b))), a + b)
+    xi = oracle.gen_i32_below(73, n, 1000)
+    r = skl.Reduce("int(int)", "int func(int x, int y){ return x + y; }")
+    r.set_work_group_size(wg)
+    assert np.int64(np.asarray(r(skl.Vector(xi)))[0]) == int(xi.astype(np.int64).sum())
+    s = skl.Scan("int(int)", "int func(int x, int y){ return x + y; }")
+    s.set_work_group_size(wg)
+    out = np.asarray(s(skl.Vector(xi), inclusive=True)).astype(np.int64)
+    assert np.array_equal(out, np.cumsum(xi.astype(np.int64)))
+
+
+def test_reduce_rejects_ragged_work_group_size(skl):
+    r = skl.Reduce("int(int)", "int func(int x, int y){ return x + y; }")
+    r.set_work_group_size(100)
+    with pytest.raises(Exception):
+        r(skl.Vector(np.ones(1000, np.int32)))
