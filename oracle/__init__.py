@@ -325,9 +325,15 @@ def allpairs_sqdist_f64(L, Rt, rows=None):
 
 
 # ---- segmented -------------------------------------------------------------------
+def _check_offsets(offsets, n):
+    if offsets.size < 1 or offsets[0] < 0 or offsets[-1] > n or np.any(np.diff(offsets) < 0):
+        raise ValueError("offsets must be non-decreasing and inside [0, n]")
+
+
 def segmented_sum_f64(x, offsets):
     x = _c(x, np.float32)
     offsets = _c(offsets, np.int64)
+    _check_offsets(offsets, x.size)
     out = np.empty(offsets.size - 1, np.float64)
     lib().orc_segmented_sum_f64(x, offsets, offsets.size - 1, out)
     return out
@@ -336,6 +342,7 @@ def segmented_sum_f64(x, offsets):
 def segmented_scan_f64(x, offsets, inclusive):
     x = _c(x, np.float32)
     offsets = _c(offsets, np.int64)
+    _check_offsets(offsets, x.size)
     out = np.empty(x.size, np.float64)
     lib().orc_segmented_scan_f64(x, offsets, offsets.size - 1, int(bool(inclusive)), out)
     return out

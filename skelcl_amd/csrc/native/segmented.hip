@@ -2,42 +2,30 @@
 // many uneven segments, given by device offsets, in one launch after a
 // one-work-group prologue.
 //
-// The prologue validates the offsets (non-decreasing, inside [0, n]), cuts
-// segment s (length L) into ceil(L / CH) chunks and numbers the chunks
-// class-major: segment s belongs to class s % 64, and position
-// p = (s % 64) * M + s / 64 (M = ceil(nseg / 64)) holds its chunk count;
-// cstart[] is the prefix over positions, so a class's chunks form one range
-// of global chunk ids and a segment's chunks are consecutive.
-//   Reduce (CH = 16384): a static grid-stride loop over chunks -- chunks do
-//     not wait on each other -- stores each chunk's sum as an FP64 partial
-//     (compensated float accumulation, FP64 work-group combine); a second
-//     kernel, one wave per segment, adds the segment's partials in a fixed
-//     order and writes out[s].  No ticket, no acquire/release.
-//   Scan (CH = 4096 = one register tile of 256 lanes x 16 elements): each
-//     class has its own relaxed ticket counter (64 counters 256 bytes apart:
-//     one hot counter would serialise every draw at ~12 ns), handing out the
-//     class's chunks in order; a work-group starts on class blockIdx % 64 and
-//     moves on when that class runs dry.  A chunk only waits on earlier
-//     chunks of its own segment, drawn earlier from the same counter.
-//     Tile-local prefixes in FP64; chunk aggregates and inclusive prefixes
-//     are published in epoch-tagged status words (AGG / INCL, 32 bits of the
-//     FP64 value per word) and wave 0 looks back over windows of 64
-//     predecessors within the segment.  Every output is
-//     float(carry + local prefix), all in FP64.
-// Invalid offsets set an error flag: no chunk runs, Reduce writes NaN.
+// Reduce: a one-work-group prologue validates the offsets (non-decreasing,
+// inside [0, n]), cuts segment s (length L) into ceil(L / 16384) chunks and
+// numbers them class-major (segment s in class s % 64, position
+// p = (s % 64) * M + s / 64, M = ceil(nseg / 64)); cstart[] is the prefix of
+// chunk counts over positions.  A static grid-stride loop over chunks stores
+// each chunk's sum as an FP64 partial (compensated float accumulation, FP64
+// work-group combine); a second kernel, one wave per segment, adds the
+// segment's partials in a fixed order and writes out[s].
+// Scan: the elements are cut into 16384-element tiles regardless of the
+// segments and scanned in one pass with the segmented operator (see the
+// flat Scan section below); a one-work-group prologue validates the offsets
+// and records each tile's first segment.  All sums in FP64, every output is
+// float(carry + local prefix).
+// Invalid offsets set an error flag: no chunk or tile runs, Reduce writes NaN.
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
 
 #include "skelcl_rt.h"
 #include "ld_util.h"
-#include "work_ticket.h"
 
 namespace {
 
 constexpr int WG = 256;
-constexpr int ROWS = 16;                              // scan rows of 64 per wave
-constexpr long long SCAN_CH = (long long)WG * ROWS;  // 4096 elements per scan chunk
 constexpr long long SUM_CH = 16384;                  // elements per reduce chunk
 constexpr unsigned ST_AGG = 1u, ST_INCL = 2u;
 
@@ -218,169 +206,302 @@ __device__ __forceinline__ unsigned read_status(const unsigned long long* pub, l
   return t0 & 3u;
 }
 
-// thread 0: the next chunk of class q or, once q runs dry, of the following
-// classes; -1 when every class is done.  A dry class is recognised by a
-// plain read first, so its counter takes no further RMWs.
-__device__ __forceinline__ long long draw_chunk(unsigned long long* ctr,
-                                                const long long* cstart, int m, long long total,
-                                                int& q)
+// ---- flat segmented Scan -------------------------------------------------
+// The n elements are cut into tiles of FTILE, independent of the segments.
+// Element e is a head when e == offsets[s] for some s (offsets[nseg] marks
+// the end of the last segment); a scan value restarts at every head.  The
+// pair (open, v) -- "no head seen yet", partial sum -- combines as
+//   (o1, v1) (+) (o2, v2) = (o1 && o2, o2 ? v1 + v2 : v2),
+// which is associative, so the tile runs the same single-pass decoupled
+// look-back as the plain Scan.  A tile that holds a head knows its inclusive
+// value at once (the sum after its last head) and publishes it as INCL
+// straight away, so a look-back never passes a tile with a head.
+constexpr int FWG = 256;                                   // threads per tile
+constexpr int FJ = 16;                                     // float4 packs per lane
+constexpr long long FTILE = (long long)FWG * FJ * 4;       // 16384 elements
+constexpr int FWAVES = FWG / 64;
+
+// v of the lane named by the DPP control (0.0 where the control names no
+// lane or the row is outside ROW_MASK), FP64 moved as two dwords
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ double dpp_f64(double v)
 {
-  if (total == 0) return -1;
-  for (int tries = 0; tries < NCLASS; ++tries) {
-    const long long lo = cstart[q * m], hi = cstart[(q + 1) * m];
-    unsigned long long* c = ctr + q * CTR_STRIDE;
-    if (lo + (long long)__hip_atomic_load(c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < hi) {
-      const long long j = ticket_draw(c);
-      if (lo + j < hi) return lo + j;
-    }
-    q = (q + 1) & (NCLASS - 1);
-  }
-  return -1;
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, ROW_MASK, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, ROW_MASK, 0xf, false);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
 }
 
-// wave 0: lane 0 draws the next chunk (draw_chunk), then the wave finds its
-// position with a 64-way search inside the chunk's class -- one coalesced
-// load of 64 prefix values per step, one step for classes of <= 64
-// positions -- instead of a chain of dependent loads.  Writes *s_c, *s_pos.
-__device__ __forceinline__ void next_chunk(unsigned long long* ctr, const long long* cstart,
-                                           int m, long long total, int& q, long long* s_c,
-                                           int* s_pos)
+// inclusive segmented wave scan: lane l ends with the sum of lanes
+// [hl, l], hl = the last lane <= l holding a head (0 when none).  Steps:
+// row_shr 1, 2, 4, 8 inside each 16-lane row, then row_bcast 15 (lane 15 of
+// each row into rows 1 and 3) and row_bcast 31 (lane 31 into rows 2 and 3);
+// a step adds its source lane s only when s >= hl.  All VALU, no LDS.
+__device__ __forceinline__ double seg_wave_scan(double v, int lane, int hl)
 {
-  const int lane = threadIdx.x & 63;
-  long long c = -1;
-  if (lane == 0) c = draw_chunk(ctr, cstart, m, total, q);
-  c = __shfl(c, 0, 64);
-  const int cq = __shfl(q, 0, 64);
-  int pos = 0;
-  if (c >= 0) {
-    int lo = cq * m, hi = cq * m + m;  // cstart[lo] <= c < cstart[hi]
-    while (hi - lo > 1) {
-      const int step = (hi - lo + 63) / 64;
-      const int idx = lo + lane * step;
-      const bool in = idx < hi;
-      const long long v = cstart[in ? idx : lo];
-      const unsigned long long mk = __ballot(in && v <= c);  // bit 0 is always set
-      lo += (63 - __builtin_clzll(mk)) * step;
-      hi = lo + step < hi ? lo + step : hi;
-    }
-    pos = lo;
-  }
-  if (lane == 0) {
-    *s_c = c;
-    *s_pos = pos;
-  }
+  double t;
+  t = dpp_f64<0x111, 0xf>(v);
+  if (lane - 1 >= hl) v += t;
+  t = dpp_f64<0x112, 0xf>(v);
+  if (lane - 2 >= hl) v += t;
+  t = dpp_f64<0x114, 0xf>(v);
+  if (lane - 4 >= hl) v += t;
+  t = dpp_f64<0x118, 0xf>(v);
+  if (lane - 8 >= hl) v += t;
+  t = dpp_f64<0x142, 0xa>(v);
+  if ((lane & ~15) - 1 >= hl) v += t;
+  t = dpp_f64<0x143, 0xc>(v);
+  if (31 >= hl) v += t;
+  return v;
 }
 
-__global__ __launch_bounds__(WG) void seg_scan_kernel(const float* __restrict__ in,
-                                                      const int64_t* __restrict__ off, int m,
-                                                      const long long* __restrict__ cstart,
-                                                      unsigned long long* __restrict__ ctr,
-                                                      unsigned long long* __restrict__ pub,
-                                                      float* __restrict__ out, int inclusive,
-                                                      const SegCtl* __restrict__ ctl)
+// value of lane - 1 (0.0 in lane 0)
+__device__ __forceinline__ double wave_shr1_f64(double v) { return dpp_f64<0x138, 0xf>(v); }
+
+__device__ __forceinline__ double readlane63_f64(double v)
 {
-  __shared__ long long s_c;
-  __shared__ int s_pos;
-  __shared__ double s_wtot[WG / 64];
+  const long long b = __double_as_longlong(v);
+  const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)b, 63);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(b >> 32), 63);
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+// Prologue, one thread per run s = -1 .. nseg (run -1 is [0, offsets[0]),
+// run s < nseg is segment s, run nseg is [offsets[nseg], inf)):
+//   * validates what it reads (offsets[0] >= 0, offsets[nseg] <= n,
+//     offsets[s] <= offsets[s + 1]); a failure stores epoch << 2 | bit into
+//     *err, epoch being the one the ticket word holds for the coming scan
+//     launch, so a flag left by an earlier launch never matches;
+//   * writes tfirst[t] = s for the tiles t = 0 .. ntiles starting inside a
+//     valid run: tfirst[t] = the largest s with offsets[s] <= t * FTILE, -1
+//     before offsets[0].  Valid runs are disjoint, so the writes total
+//     ntiles + 1 whatever the segment count.
+__global__ __launch_bounds__(256) void seg_flat_prologue(const int64_t* __restrict__ off,
+                                                         int nseg, unsigned long long n,
+                                                         long long ntiles,
+                                                         int* __restrict__ tfirst,
+                                                         const unsigned long long* ticket,
+                                                         unsigned* err)
+{
+  const long long s = (long long)blockIdx.x * 256 + threadIdx.x - 1;
+  if (s > nseg) return;
+  const long long a = s < 0 ? 0 : off[s];
+  const long long b = s < nseg ? off[s + 1] : (long long)n;
+  unsigned bad = 0;
+  if (s < 0 && (b < 0 || (unsigned long long)off[nseg] > n || off[nseg] < 0)) bad = 2u;
+  if (s >= 0 && s < nseg && b < a) bad = 1u;
+  if (bad) {
+    const unsigned ep = (unsigned)(__hip_atomic_load(ticket, __ATOMIC_RELAXED,
+                                                     __HIP_MEMORY_SCOPE_AGENT) >> 32);
+    __hip_atomic_store(err, (ep << 2) | bad, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return;
+  }
+  if (a < 0 || b < a || (unsigned long long)b > n) return;  // a neighbour flags it
+  long long t1 = ntiles + 1;
+  if (s < nseg) {
+    const long long tb = (b + FTILE - 1) / FTILE;
+    if (tb < t1) t1 = tb;
+  }
+  for (long long t = (a + FTILE - 1) / FTILE; t < t1; ++t) tfirst[t] = (int)s;
+}
+
+__global__ __launch_bounds__(FWG) void seg_flat_scan_kernel(
+    const float* __restrict__ in, float* __restrict__ out, unsigned long long n,
+    const int64_t* __restrict__ off, int nseg, const int* __restrict__ tfirst,
+    unsigned long long* __restrict__ ticket, unsigned long long* __restrict__ pub,
+    int inclusive, const unsigned* __restrict__ err)
+{
+  typedef float f4u __attribute__((ext_vector_type(4), aligned(4)));
+  __shared__ unsigned long long s_tile;
+  __shared__ unsigned s_epoch, s_err;
+  __shared__ unsigned s_bits[FTILE / 32];
+  __shared__ double s_wval[FWAVES];
+  __shared__ int s_wopen[FWAVES];
   __shared__ double s_carry;
-  const long long total = (long long)ctl->total;
-  const unsigned epoch = ctl->epoch;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  int q = blockIdx.x & (NCLASS - 1);  // lane 0 of wave 0: current class
-  if (wv == 0) next_chunk(ctr, cstart, m, total, q, &s_c, &s_pos);
+
+  // ticket word = epoch << 32 | tile; the work-group drawing the last tile
+  // re-arms it for the next launch (every other draw has happened by then)
+  if (threadIdx.x == 0) {
+    const unsigned long long w =
+        __hip_atomic_fetch_add(ticket, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const unsigned long long t = w & 0xFFFFFFFFull;
+    const unsigned ep = (unsigned)(w >> 32);
+    if (t == (unsigned long long)gridDim.x - 1)
+      __hip_atomic_store(ticket, (unsigned long long)((ep + 1u) & 0x3FFFFFFFu) << 32,
+                         __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    s_tile = t;
+    s_epoch = ep;
+    const unsigned e = __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    s_err = (e >> 2) == ep && (e & 3u) != 0;
+  }
+  for (int i = threadIdx.x; i < (int)(FTILE / 32); i += FWG) s_bits[i] = 0;
   __syncthreads();
-  for (;;) {
-    const long long c = ticket_read(s_c);
-    const int pos = (int)__builtin_amdgcn_readfirstlane((unsigned)s_pos);
-    __syncthreads();
-    if (c < 0) break;
-    const int s = seg_of_pos(pos, m);
-    const long long cfirst = cstart[pos];
-    const long long k = c - cfirst;
-    const long long lo = off[s] + k * SCAN_CH;
-    const long long end = off[s + 1];
-    const long long hi = lo + SCAN_CH < end ? lo + SCAN_CH : end;
-    const long long wbase = lo + (long long)wv * (64 * ROWS);
+  if (s_err) return;  // invalid offsets: write nothing
+  const unsigned long long tile = s_tile;
+  const unsigned epoch = s_epoch;
+  const unsigned long long tbase = tile * (unsigned long long)FTILE;
+  const unsigned long long wbase = tbase + (unsigned long long)wv * (64 * FJ * 4);
+  const bool full = tbase + FTILE <= n;
 
-    // row j of this wave = 64 consecutive elements, one per lane
-    float x[ROWS];
-    double inc[ROWS];
+  float x[FJ][4];
+  if (full) {
 #pragma unroll
-    for (int j = 0; j < ROWS; ++j) {
-      const long long e = wbase + j * 64 + lane;
-      x[j] = ld_or(in, (size_t)e, (size_t)lo, e < hi, 0.0f);
+    for (int j = 0; j < FJ; ++j) {
+      const f4u v = __builtin_nontemporal_load(
+          reinterpret_cast<const f4u*>(in + wbase + ((unsigned long long)j * 64 + lane) * 4));
+#pragma unroll
+      for (int k = 0; k < 4; ++k) x[j][k] = v[k];
     }
-    double wsum = 0.0;  // sum of the rows before row j
+  } else {
 #pragma unroll
-    for (int j = 0; j < ROWS; ++j) {
-      double v = (double)x[j];
+    for (int j = 0; j < FJ; ++j)
 #pragma unroll
-      for (int o = 1; o < 64; o <<= 1) {
-        const double t = __shfl_up(v, o, 64);
-        if (lane >= o) v += t;
+      for (int k = 0; k < 4; ++k) {
+        const unsigned long long e = wbase + ((unsigned long long)j * 64 + lane) * 4 + k;
+        x[j][k] = ld_or(in, (size_t)e, 0, e < n, 0.0f);
       }
-      const double rt = __shfl(v, 63, 64);
-      inc[j] = wsum + v;  // wave-local inclusive prefix
-      wsum += rt;
-    }
-    if (lane == 0) s_wtot[wv] = wsum;
-    __syncthreads();
-    double wpre = 0.0, agg = 0.0;
-#pragma unroll
-    for (int w = 0; w < WG / 64; ++w) {
-      const double t = s_wtot[w];
-      if (w < wv) wpre += t;
-      agg += t;
-    }
+  }
 
-    // decoupled look-back over this segment's earlier chunks (wave 0)
-    if (wv == 0) {
-      if (k == 0) {
-        if (lane == 0) {
-          publish(pub, c, ST_INCL, agg, epoch);
-          s_carry = 0.0;
+  // head bits of the tile: offsets[s0 .. s1].  A few are read by wave 0
+  // with scalar loads, which leave the data loads above in flight (a vector
+  // load issued after them could only be waited for together with them).
+  {
+    const int s1 = tfirst[tile + 1];
+    int s0 = tfirst[tile];
+    if (s0 < 0) s0 = 0;
+    if (s1 - s0 < 16) {
+      if (wv == 0)
+        for (int s = s0; s <= s1; ++s) {
+          const long long p = off[s] - (long long)tbase;
+          if (lane == 0 && p >= 0 && p < FTILE) atomicOr(&s_bits[p >> 5], 1u << (p & 31));
         }
-      } else {
-        if (lane == 0) publish(pub, c, ST_AGG, agg, epoch);
-        double excl = 0.0;
-        long long wend = c;  // window [wend - 64, wend)
-        for (;;) {
-          const long long idx = wend - 64 + lane;
-          unsigned st;
-          double v = 0.0;
-          if (idx < cfirst) {
-            st = ST_INCL;  // before the segment: identity
-          } else {
-            do {
-              st = read_status(pub, idx, epoch, &v);
-            } while (st == 0);
-          }
-          const unsigned long long im = __ballot(st == ST_INCL);
-          const int h = im ? 63 - __builtin_clzll(im) : -1;
-          // lanes [max(h, 0), 63] in order
-          double r = lane >= h ? v : 0.0;
-#pragma unroll
-          for (int o = 32; o >= 1; o >>= 1) r += __shfl_xor(r, o, 64);
-          excl += r;
-          if (h >= 0) break;
-          wend -= 64;
-        }
-        if (lane == 0) {
-          publish(pub, c, ST_INCL, excl + agg, epoch);
-          s_carry = excl;
-        }
+    } else {
+      for (int s = s0 + (int)threadIdx.x; s <= s1; s += FWG) {
+        const long long p = off[s] - (long long)tbase;
+        if (p >= 0 && p < FTILE) atomicOr(&s_bits[p >> 5], 1u << (p & 31));
       }
     }
-    __syncthreads();
-    const double base = s_carry + wpre;
+  }
+  __syncthreads();
+  const long long lo = off[0], hi = off[nseg];
+
+  // per pack: (open, sum after its last head); one segmented wave scan per
+  // row of 64 packs
+  double lpv[FJ];
+  unsigned lpo = 0;  // bit j: pack j's wave-exclusive prefix is open
+  bool ropen = true;
+  double rval = 0.0;
 #pragma unroll
-    for (int j = 0; j < ROWS; ++j) {
-      const long long e = wbase + j * 64 + lane;
-      const double p = inclusive ? inc[j] : inc[j] - (double)x[j];
-      if (e < hi) out[e] = (float)(base + p);
+  for (int j = 0; j < FJ; ++j) {
+    const int p = (wv * 64 * FJ + j * 64 + lane) * 4;
+    const unsigned hb = (s_bits[p >> 5] >> (p & 31)) & 0xFu;
+    double ps = 0.0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      if (hb & (1u << k)) ps = 0.0;
+      ps += (double)x[j][k];
     }
-    if (wv == 0) next_chunk(ctr, cstart, m, total, q, &s_c, &s_pos);
-    __syncthreads();
+    const unsigned long long M = __ballot(hb != 0);
+    const unsigned long long le = M & ((2ull << lane) - 1ull);
+    const int hl = le ? 63 - __builtin_clzll(le) : 0;
+    const double inc = seg_wave_scan(ps, lane, hl);
+    const double ex = wave_shr1_f64(inc);
+    const bool eopen = (M & ((1ull << lane) - 1ull)) == 0;
+    const double rt = readlane63_f64(inc);
+    lpv[j] = eopen ? rval + ex : ex;
+    if (eopen && ropen) lpo |= 1u << j;
+    if (M) {
+      ropen = false;
+      rval = rt;
+    } else {
+      rval += rt;
+    }
+  }
+  // keep the floats, not their FP64 conversions, live until the outputs
+#pragma unroll
+  for (int j = 0; j < FJ; ++j)
+#pragma unroll
+    for (int k = 0; k < 4; ++k) asm volatile("" : "+v"(x[j][k]));
+  if (lane == 0) {
+    s_wval[wv] = rval;
+    s_wopen[wv] = ropen ? 1 : 0;
+  }
+  __syncthreads();
+  bool wopen = true, topen = true;
+  double wval = 0.0, tval = 0.0;
+#pragma unroll
+  for (int w = 0; w < FWAVES; ++w) {
+    const double v = s_wval[w];
+    const bool o = s_wopen[w] != 0;
+    if (w < wv) {
+      wval = o ? wval + v : v;
+      wopen = wopen && o;
+    }
+    tval = o ? tval + v : v;
+    topen = topen && o;
+  }
+
+  // look-back (wave 0); a tile whose first element is a head needs no carry
+  if (wv == 0) {
+    const bool need = tile > 0 && (s_bits[0] & 1u) == 0;
+    if (lane == 0) publish(pub, (long long)tile, topen ? ST_AGG : ST_INCL, tval, epoch);
+    double excl = 0.0;
+    if (need) {
+      long long wend = (long long)tile;  // window [wend - 64, wend)
+      for (;;) {
+        const long long idx = wend - 64 + lane;
+        unsigned st = ST_INCL;
+        double v = 0.0;
+        if (idx >= 0) {
+          do {
+            st = read_status(pub, idx, epoch, &v);
+          } while (st == 0);
+        }
+        const unsigned long long im = __ballot(st == ST_INCL);
+        const int h = im ? 63 - __builtin_clzll(im) : -1;
+        double r = lane >= h ? v : 0.0;  // lanes [max(h, 0), 63]
+#pragma unroll
+        for (int o = 32; o >= 1; o >>= 1) r += __shfl_xor(r, o, 64);
+        excl += r;
+        if (h >= 0) break;
+        wend -= 64;
+      }
+    }
+    if (lane == 0) {
+      if (topen) publish(pub, (long long)tile, ST_INCL, excl + tval, epoch);
+      s_carry = excl;
+    }
+  }
+  __syncthreads();
+  const double wcarry = wopen ? s_carry + wval : wval;
+
+#pragma unroll
+  for (int j = 0; j < FJ; ++j) {
+    const int p = (wv * 64 * FJ + j * 64 + lane) * 4;
+    const unsigned hb = (s_bits[p >> 5] >> (p & 31)) & 0xFu;
+    double r = (lpo >> j) & 1u ? wcarry + lpv[j] : lpv[j];
+    float o[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      if (hb & (1u << k)) r = 0.0;
+      const double ex = r;
+      r += (double)x[j][k];
+      o[k] = (float)(inclusive ? r : ex);
+    }
+    const unsigned long long e0 = wbase + ((unsigned long long)j * 64 + lane) * 4;
+    if (full && (long long)e0 >= lo && (long long)e0 + 4 <= hi) {
+      f4u w;
+#pragma unroll
+      for (int k = 0; k < 4; ++k) w[k] = o[k];
+      __builtin_nontemporal_store(w, reinterpret_cast<f4u*>(out + e0));
+    } else {
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const long long e = (long long)e0 + k;
+        if (e >= lo && e < hi) out[e] = o[k];
+      }
+    }
   }
 }
 
@@ -421,13 +542,23 @@ void carve(void* ws, SegCtl** ctl, unsigned long long** ctr, long long** cstart,
   *data = p + head_bytes(nseg);
 }
 
+// flat Scan workspace: tile ticket (256 B) | error word (256 B) | tfirst over
+// the tiles (+1), padded to 256 | two 64-bit status words per tile
+long long flat_tiles(unsigned long long n) { return (long long)((n + FTILE - 1) / FTILE); }
+size_t flat_table_bytes(long long ntiles) { return ((size_t)(ntiles + 1) * 4 + 255) & ~(size_t)255; }
+size_t flat_bytes(unsigned long long n)
+{
+  const long long t = flat_tiles(n);
+  return 512 + flat_table_bytes(t) + (size_t)t * 16;
+}
+
 }  // namespace
 
 extern "C" size_t skl_internal_segmented_workspace(unsigned long long n, int nseg)
 {
-  const size_t sum = max_chunks(n, nseg, SUM_CH) * 8;
-  const size_t scan = max_chunks(n, nseg, SCAN_CH) * 16;
-  return head_bytes(nseg) + (sum > scan ? sum : scan);
+  const size_t sum = head_bytes(nseg) + max_chunks(n, nseg, SUM_CH) * 8;
+  const size_t scan = flat_bytes(n);
+  return sum > scan ? sum : scan;
 }
 
 extern "C" int skl_internal_segmented_sum(float* out, const float* in, unsigned long long n,
@@ -454,18 +585,17 @@ extern "C" int skl_internal_segmented_scan(float* out, const float* in, unsigned
                                            const int64_t* offsets, int nseg, int inclusive,
                                            void* ws, hipStream_t s)
 {
-  SegCtl* ctl;
-  unsigned long long* ctr;
-  long long* cstart;
-  void* data;
-  carve(ws, &ctl, &ctr, &cstart, &data, nseg);
-  const int m = classes_m(nseg);
-  seg_prologue<<<1, 1024, 0, s>>>(offsets, nseg, m, n, SCAN_CH, cstart, ctr, ctl);
-  const size_t chunks = max_chunks(n, nseg, SCAN_CH);
-  const size_t cap = (size_t)device_cus() * 8;
-  const unsigned grid = (unsigned)(chunks < cap ? chunks : cap);
-  seg_scan_kernel<<<grid, WG, 0, s>>>(in, offsets, m, cstart, ctr,
-                                      static_cast<unsigned long long*>(data), out,
-                                      inclusive ? 1 : 0, ctl);
+  const long long ntiles = flat_tiles(n);
+  if (ntiles > 0x7FFFFFFE) return SKL_ERR_INVALID;
+  char* p = static_cast<char*>(ws);
+  unsigned long long* ticket = reinterpret_cast<unsigned long long*>(p);
+  unsigned* err = reinterpret_cast<unsigned*>(p + 256);
+  int* tfirst = reinterpret_cast<int*>(p + 512);
+  unsigned long long* pub =
+      reinterpret_cast<unsigned long long*>(p + 512 + flat_table_bytes(ntiles));
+  seg_flat_prologue<<<(unsigned)((nseg + 2 + 255) / 256), 256, 0, s>>>(offsets, nseg, n, ntiles,
+                                                                      tfirst, ticket, err);
+  seg_flat_scan_kernel<<<(unsigned)ntiles, FWG, 0, s>>>(in, out, n, offsets, nseg, tfirst,
+                                                        ticket, pub, inclusive ? 1 : 0, err);
   return hipGetLastError() == hipSuccess ? SKL_OK : SKL_ERR_HIP;
 }

@@ -204,6 +204,81 @@ def test_segmented_invalid_offsets_flagged_on_device(skl):
     got = np.asarray(r.segmented(skl.Vector(x), off)).astype(np.float64)
     ref = oracle.segmented_sum_f64(x, off)
     assert np.all(np.abs(got - ref) <= np.spacing(np.abs(ref).astype(np.float32)) + 1e-12)
+    for incl in (False, True):
+        got = np.asarray(s.segmented(skl.Vector(x), off, inclusive=incl)).astype(np.float64)
+        ref = oracle.segmented_scan_f64(x, off, incl)
+        assert np.all(np.abs(got - ref) <= np.spacing(np.abs(ref).astype(np.float32)) + 1e-9)
+
+
+def _seg_scan_raw(skl, x, off, n, inclusive, fill=7.0):
+    from skelcl_amd import _lib
+    X = skl.DeviceBuffer.from_array(x)
+    o = skl.DeviceBuffer.from_array(np.array(off, np.int64))
+    y = skl.DeviceBuffer.from_array(np.full(n, fill, np.float32))
+    s = skl.Scan("float(float)", "float func(float x, float y){ return x+y; }", "0.0f")
+    _lib.check(_lib.lib().skl_scan_segmented_run(s.handle, y.ptr, X.ptr, n, o.ptr, len(off) - 1,
+                                                 int(inclusive), None))
+    return y.to_numpy(np.float32, n)
+
+
+SEG_TILE = 16384  # elements per tile of the segmented Scan
+
+
+@pytest.mark.parametrize("case", ["inner_range", "dense_tiny", "tile_edges", "ragged_tail"])
+@pytest.mark.parametrize("inclusive", [False, True])
+def test_segmented_scan_layouts(skl, case, inclusive):
+    """Segment layouts against tile boundaries: segments that leave the ends
+    of the array uncovered (untouched outputs), runs of empty and one-element
+    segments (many heads per tile), heads exactly on, before and after tile
+    starts, and a last tile cut short at a length that is not a multiple of 4."""
+    rng = np.random.default_rng({"inner_range": 1, "dense_tiny": 2, "tile_edges": 3,
+                                 "ragged_tail": 4}[case])
+    if case == "inner_range":
+        n = 5 * SEG_TILE + 123
+        off = [1000, 1000, 20000, 20001, 60000, 4 * SEG_TILE + 5]
+    elif case == "dense_tiny":
+        lens = rng.integers(0, 4, 30000)
+        off = np.concatenate([[0], np.cumsum(lens)]).tolist()
+        n = off[-1] + 50
+    elif case == "tile_edges":
+        T = SEG_TILE
+        off = [0, T - 1, T, T + 1, 2 * T, 3 * T, 3 * T, 5 * T - 1, 7 * T, 7 * T + 3]
+        n = off[-1]
+    else:
+        n = 9 * SEG_TILE + 7
+        lens = rng.integers(1, 40000, 12)
+        off = np.concatenate([[0], np.minimum(np.cumsum(lens), n)]).tolist()
+        off[-1] = n
+    x = oracle.gen_f32_sym(0x5010, n)
+    got = _seg_scan_raw(skl, x, off, n, inclusive)
+    ref = oracle.segmented_scan_f64(x, np.array(off, np.int64), inclusive)
+    inside = np.zeros(n, bool)
+    inside[off[0]:off[-1]] = True
+    tol = np.spacing(np.abs(ref[inside]).astype(np.float32)).astype(np.float64) + 1e-9
+    assert np.all(np.abs(got[inside].astype(np.float64) - ref[inside]) <= tol)
+    assert np.all(got[~inside] == 7.0)
+
+
+def test_segmented_scan_repeated_and_resized(skl):
+    """The tile ticket and status words are reused across calls: alternate
+    sizes on one skeleton, every result checked."""
+    from skelcl_amd import _lib
+    s = skl.Scan("float(float)", "float func(float x, float y){ return x+y; }", "0.0f")
+    for it, n in enumerate([300000, 40000, 300000, 17, 1 << 20]):
+        lens = oracle.gen_loguniform(0x5020 + it, 40, 1.0, 4.0)
+        off = np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)
+        off = np.minimum(off, n)
+        off[-1] = n
+        x = oracle.gen_f32_sym(0x5030 + it, n)
+        X = skl.DeviceBuffer.from_array(x)
+        O = skl.DeviceBuffer.from_array(off)
+        y = skl.DeviceBuffer.from_array(np.zeros(n, np.float32))
+        _lib.check(_lib.lib().skl_scan_segmented_run(s.handle, y.ptr, X.ptr, n, O.ptr,
+                                                     len(off) - 1, 1, None))
+        got = y.to_numpy(np.float32, n).astype(np.float64)
+        ref = oracle.segmented_scan_f64(x, off, True)
+        tol = np.spacing(np.abs(ref).astype(np.float32)).astype(np.float64) + 1e-9
+        assert np.all(np.abs(got - ref) <= tol), n
 
 
 @pytest.mark.parametrize("prec", [1, 2])
