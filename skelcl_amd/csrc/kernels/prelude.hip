@@ -292,6 +292,60 @@ __device__ __forceinline__ T skl_shfl(const T& v, int src)
   return r;
 }
 
+// DPP moves (VALU, no LDS), 32 bits at a time: lane l takes the value of the
+// lane that CTRL names inside the rows of ROW_MASK; other lanes keep theirs.
+template <int CTRL, int ROW_MASK, typename T>
+__device__ __forceinline__ T skl_dpp(const T& v)
+{
+  constexpr int W = (int)((sizeof(T) + 3) / 4);
+  int w[W];
+  T r;
+  __builtin_memcpy(w, &v, sizeof(T));
+#pragma unroll
+  for (int i = 0; i < W; ++i)
+    w[i] = __builtin_amdgcn_update_dpp(w[i], w[i], CTRL, ROW_MASK, 0xf, false);
+  __builtin_memcpy(&r, w, sizeof(T));
+  return r;
+}
+template <typename T>
+__device__ __forceinline__ T skl_readlane(const T& v, int src)
+{
+  constexpr int W = (int)((sizeof(T) + 3) / 4);
+  int w[W];
+  T r;
+  __builtin_memcpy(w, &v, sizeof(T));
+#pragma unroll
+  for (int i = 0; i < W; ++i) w[i] = __builtin_amdgcn_readlane(w[i], src);
+  __builtin_memcpy(&r, w, sizeof(T));
+  return r;
+}
+
+// Inclusive wave64 scan in lane order (lane l ends with v_0 (+) .. (+) v_l;
+// comb need not commute): row_shr 1, 2, 4, 8 inside each 16-lane row, then
+// row_bcast 15 (lane 15 of rows 0 and 2 into rows 1 and 3) and row_bcast 31
+// (lane 31 into rows 2 and 3).
+template <typename T, typename F>
+__device__ __forceinline__ T skl_wave_scan(T v, int lane, F comb)
+{
+  T o;
+  o = skl_dpp<0x111, 0xf>(v);
+  if ((lane & 15) >= 1) v = comb(o, v);
+  o = skl_dpp<0x112, 0xf>(v);
+  if ((lane & 15) >= 2) v = comb(o, v);
+  o = skl_dpp<0x114, 0xf>(v);
+  if ((lane & 15) >= 4) v = comb(o, v);
+  o = skl_dpp<0x118, 0xf>(v);
+  if ((lane & 15) >= 8) v = comb(o, v);
+  o = skl_dpp<0x142, 0xa>(v);
+  if (lane & 16) v = comb(o, v);
+  o = skl_dpp<0x143, 0xc>(v);
+  if (lane >= 32) v = comb(o, v);
+  return v;
+}
+// value of lane - 1 (lane 0 keeps its own)
+template <typename T>
+__device__ __forceinline__ T skl_wave_shr1(const T& v) { return skl_dpp<0x138, 0xf>(v); }
+
 // Packs of N elements moved with one wide load/store.
 template <typename T, int N> struct alignas(N * sizeof(T)) skl_pack { T v[N]; };
 template <typename T> struct skl_pack<T, 1> { T v[1]; };

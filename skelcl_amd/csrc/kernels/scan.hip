@@ -184,15 +184,10 @@ extern "C" __global__ __launch_bounds__(SKL_WG) SKL_SCAN_ATTR void skl_scan(
   SKL_ACC_T rowpre = SKL_ACC_ID;
 #pragma unroll
   for (int j = 0; j < SKL_J; ++j) {
-    SKL_ACC_T inc = q[j][SKL_V - 1];
-#pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-      SKL_ACC_T o = skl_shfl_up(inc, off);
-      if (lane >= off) inc = comb(o, inc);
-    }
-    SKL_ACC_T exc = skl_shfl_up(inc, 1);
+    const SKL_ACC_T inc = skl_wave_scan(q[j][SKL_V - 1], lane, comb);
+    SKL_ACC_T exc = skl_wave_shr1(inc);
     if (lane == 0) exc = SKL_ACC_ID;
-    SKL_ACC_T rt = skl_shfl(inc, 63);
+    const SKL_ACC_T rt = skl_readlane(inc, 63);
     lp[j] = comb(rowpre, exc);
     rowpre = comb(rowpre, rt);
   }
