@@ -10,7 +10,7 @@
 // each chunk's sum as an FP64 partial (compensated float accumulation, FP64
 // work-group combine); a second kernel, one wave per segment, adds the
 // segment's partials in a fixed order and writes out[s].
-// Scan: the elements are cut into 16384-element tiles regardless of the
+// Scan: the elements are cut into 32768-element tiles regardless of the
 // segments and scanned in one pass with the segmented operator (see the
 // flat Scan section below); a one-work-group prologue validates the offsets
 // and records each tile's first segment.  All sums in FP64, every output is
@@ -216,9 +216,9 @@ __device__ __forceinline__ unsigned read_status(const unsigned long long* pub, l
 // look-back as the plain Scan.  A tile that holds a head knows its inclusive
 // value at once (the sum after its last head) and publishes it as INCL
 // straight away, so a look-back never passes a tile with a head.
-constexpr int FWG = 256;                                   // threads per tile
+constexpr int FWG = 512;                                   // threads per tile
 constexpr int FJ = 16;                                     // float4 packs per lane
-constexpr long long FTILE = (long long)FWG * FJ * 4;       // 16384 elements
+constexpr long long FTILE = (long long)FWG * FJ * 4;       // 32768 elements
 constexpr int FWAVES = FWG / 64;
 
 // v of the lane named by the DPP control (0.0 where the control names no
@@ -315,6 +315,7 @@ __global__ __launch_bounds__(FWG) void seg_flat_scan_kernel(
   __shared__ unsigned long long s_tile;
   __shared__ unsigned s_epoch, s_err;
   __shared__ unsigned s_bits[FTILE / 32];
+  __shared__ double s_lpv[FJ][FWG];  // per pack: wave-exclusive prefix (own thread only)
   __shared__ double s_wval[FWAVES];
   __shared__ int s_wopen[FWAVES];
   __shared__ double s_carry;
@@ -388,7 +389,6 @@ __global__ __launch_bounds__(FWG) void seg_flat_scan_kernel(
 
   // per pack: (open, sum after its last head); one segmented wave scan per
   // row of 64 packs
-  double lpv[FJ];
   unsigned lpo = 0;  // bit j: pack j's wave-exclusive prefix is open
   bool ropen = true;
   double rval = 0.0;
@@ -409,7 +409,7 @@ __global__ __launch_bounds__(FWG) void seg_flat_scan_kernel(
     const double ex = wave_shr1_f64(inc);
     const bool eopen = (M & ((1ull << lane) - 1ull)) == 0;
     const double rt = readlane63_f64(inc);
-    lpv[j] = eopen ? rval + ex : ex;
+    s_lpv[j][threadIdx.x] = eopen ? rval + ex : ex;
     if (eopen && ropen) lpo |= 1u << j;
     if (M) {
       ropen = false;
@@ -480,7 +480,8 @@ __global__ __launch_bounds__(FWG) void seg_flat_scan_kernel(
   for (int j = 0; j < FJ; ++j) {
     const int p = (wv * 64 * FJ + j * 64 + lane) * 4;
     const unsigned hb = (s_bits[p >> 5] >> (p & 31)) & 0xFu;
-    double r = (lpo >> j) & 1u ? wcarry + lpv[j] : lpv[j];
+    const double lp = s_lpv[j][threadIdx.x];
+    double r = (lpo >> j) & 1u ? wcarry + lp : lp;
     float o[4];
 #pragma unroll
     for (int k = 0; k < 4; ++k) {

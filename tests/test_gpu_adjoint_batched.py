@@ -221,7 +221,7 @@ def _seg_scan_raw(skl, x, off, n, inclusive, fill=7.0):
     return y.to_numpy(np.float32, n)
 
 
-SEG_TILE = 16384  # elements per tile of the segmented Scan
+SEG_TILE = 32768  # elements per tile of the segmented Scan
 
 
 @pytest.mark.parametrize("case", ["inner_range", "dense_tiny", "tile_edges", "ragged_tail"])
