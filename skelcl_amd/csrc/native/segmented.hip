@@ -1,19 +1,17 @@
 // Batched segmented Reduce(+) / Scan(+) on float (north-star extension):
-// many uneven segments, given by device offsets, in one launch after a
-// one-work-group prologue.
+// many uneven segments, given by device offsets, in one main launch after a
+// small prologue launch.
 //
 // Reduce: a one-work-group prologue validates the offsets (non-decreasing,
-// inside [0, n]), cuts segment s (length L) into ceil(L / 16384) chunks and
-// numbers them class-major (segment s in class s % 64, position
-// p = (s % 64) * M + s / 64, M = ceil(nseg / 64)); cstart[] is the prefix of
-// chunk counts over positions.  A static grid-stride loop over chunks stores
-// each chunk's sum as an FP64 partial (compensated float accumulation, FP64
-// work-group combine); a second kernel, one wave per segment, adds the
-// segment's partials in a fixed order and writes out[s].
+// inside [0, n]), cuts segment s (length L) into ceil(L / 16384) chunks,
+// records each segment's first chunk and each chunk's segment.  One
+// work-group per chunk stores the chunk's FP64 sum as a partial; a second
+// kernel, one wave per segment, adds the segment's partials in a fixed order
+// and writes out[s].
 // Scan: the elements are cut into 32768-element tiles regardless of the
 // segments and scanned in one pass with the segmented operator (see the
-// flat Scan section below); a one-work-group prologue validates the offsets
-// and records each tile's first segment.  All sums in FP64, every output is
+// flat Scan section below); a prologue, one thread per segment, validates
+// the offsets and records each tile's first segment.  All sums in FP64, every output is
 // float(carry + local prefix).
 // Invalid offsets set an error flag: no chunk or tile runs, Reduce writes NaN.
 #include <hip/hip_runtime.h>
@@ -26,93 +24,67 @@
 namespace {
 
 constexpr int WG = 256;
-constexpr long long SUM_CH = 16384;                  // elements per reduce chunk
+constexpr int SUM_J = 16;                                 // float4 packs per thread
+constexpr long long SUM_CH = (long long)WG * SUM_J * 4;   // 16384 elements per chunk
 constexpr unsigned ST_AGG = 1u, ST_INCL = 2u;
-
-constexpr int NCLASS = 64;
-constexpr int CTR_STRIDE = 32;  // counters 256 bytes apart (in 64-bit words)
 
 struct SegCtl {
   unsigned long long total;  // number of chunks (0 after an error)
-  unsigned int epoch;        // scan: tags this launch's status words
   unsigned int error;        // 1: negative length, 2: offsets outside [0, n]
 };
 
-__device__ __forceinline__ int seg_of_pos(int p, int m) { return (p / m) + NCLASS * (p % m); }
-__device__ __forceinline__ int pos_of_seg(int s, int m) { return (s % NCLASS) * m + s / NCLASS; }
-
-// one work-group of 1024 threads; npos = NCLASS * M positions
-__global__ __launch_bounds__(1024) void seg_prologue(const int64_t* __restrict__ off, int nseg,
-                                                     int m, unsigned long long n, long long ch,
-                                                     long long* __restrict__ cstart,
-                                                     unsigned long long* __restrict__ ctr,
-                                                     SegCtl* ctl)
+// Reduce prologue, one work-group of 1024 threads: validates the offsets,
+// cuts segment s (length L) into ceil(L / SUM_CH) chunks, writes
+// cstart[s] = its first chunk (cstart[nseg] = total) and cseg[c] = the
+// segment of chunk c (c < cap).
+__global__ __launch_bounds__(1024) void seg_sum_prologue(const int64_t* __restrict__ off,
+                                                         int nseg, unsigned long long n,
+                                                         long long cap,
+                                                         long long* __restrict__ cstart,
+                                                         int* __restrict__ cseg, SegCtl* ctl)
 {
-  __shared__ long long part[1024];
+  __shared__ long long wsum[1024 / 64];
   __shared__ long long carry;
   __shared__ unsigned err;
-  const int npos = NCLASS * m;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   if (threadIdx.x == 0) {
     carry = 0;
-    err = (off[0] < 0 || (unsigned long long)off[nseg] > n) ? 2u : 0u;
+    err = (off[0] < 0 || off[nseg] < 0 || (unsigned long long)off[nseg] > n) ? 2u : 0u;
   }
-  if (threadIdx.x < NCLASS) ctr[threadIdx.x * CTR_STRIDE] = 0;
   __syncthreads();
-  for (int base = 0; base < npos; base += 1024) {
-    const int p = base + threadIdx.x;
+  for (int base = 0; base < nseg; base += 1024) {
+    const int s = base + (int)threadIdx.x;
     long long cnt = 0;
-    if (p < npos) {
-      const int s = seg_of_pos(p, m);
-      if (s < nseg) {
-        const long long len = off[s + 1] - off[s];
-        if (len < 0) atomicOr(&err, 1u);
-        else cnt = (len + ch - 1) / ch;
-      }
+    if (s < nseg) {
+      const long long len = off[s + 1] - off[s];
+      if (len < 0) atomicOr(&err, 1u);
+      else cnt = (len + SUM_CH - 1) / SUM_CH;
     }
-    part[threadIdx.x] = cnt;
-    __syncthreads();
-    // inclusive scan of the 1024 counts (Hillis-Steele)
-    for (int d = 1; d < 1024; d <<= 1) {
-      long long v = threadIdx.x >= (unsigned)d ? part[threadIdx.x - d] : 0;
-      __syncthreads();
-      part[threadIdx.x] += v;
-      __syncthreads();
+    // inclusive scan of the 1024 counts: wave scans, then the wave totals
+    long long v = cnt;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const long long t = __shfl_up(v, o, 64);
+      if (lane >= o) v += t;
     }
-    if (p < npos) cstart[p] = carry + part[threadIdx.x] - cnt;
+    if (lane == 63) wsum[wv] = v;
     __syncthreads();
-    if (threadIdx.x == 1023) carry += part[1023];
+    long long pre = carry;
+    for (int w = 0; w < wv; ++w) pre += wsum[w];
+    const long long first = pre + v - cnt;
+    if (s < nseg) {
+      cstart[s] = first;
+      for (long long c = first; c < first + cnt && c < cap; ++c) cseg[c] = s;
+    }
+    __syncthreads();
+    if (threadIdx.x == 1023) carry = first + cnt;
     __syncthreads();
   }
   if (threadIdx.x == 0) {
-    cstart[npos] = carry;
+    cstart[nseg] = carry;
     ctl->total = err ? 0 : (unsigned long long)carry;
     ctl->error = err;
-    ctl->epoch = (ctl->epoch + 1u) & 0x3FFFFFFFu;
   }
-}
-
-// position of global chunk c: cstart[lo] <= c < cstart[lo + 1] (positions
-// with no chunks are skipped because cstart[lo + 1] > c)
-__device__ __forceinline__ int find_position(const long long* cstart, int npos, long long c)
-{
-  int lo = 0, hi = npos;
-  while (hi - lo > 1) {
-    int mid = (lo + hi) >> 1;
-    if (cstart[mid] <= c) lo = mid;
-    else hi = mid;
-  }
-  return lo;
-}
-
-// compensated (two-sum) accumulation in float
-struct DF {
-  float hi, lo;
-};
-__device__ __forceinline__ DF df_add(DF a, float x)
-{
-  float s = a.hi + x, bb = s - a.hi;
-  float e = (a.hi - (s - bb)) + (x - bb);
-  return DF{s, a.lo + e};
 }
 
 __device__ __forceinline__ double block_sum_double(double v, double* red)
@@ -124,44 +96,62 @@ __device__ __forceinline__ double block_sum_double(double v, double* red)
   double t = 0.0;
   if (threadIdx.x == 0)
     for (int i = 0; i < WG / 64; ++i) t += red[i];
-  __syncthreads();
   return t;  // valid in thread 0
 }
 
+// one work-group per chunk: FP64 sums (4 accumulators per thread, fixed
+// order), a full chunk read as 16 float4 packs per thread all in flight
+// (dword-aligned: a segment may start anywhere)
 __global__ __launch_bounds__(WG) void seg_sum_kernel(const float* __restrict__ in,
-                                                     const int64_t* __restrict__ off, int m,
+                                                     const int64_t* __restrict__ off,
                                                      const long long* __restrict__ cstart,
+                                                     const int* __restrict__ cseg,
                                                      double* __restrict__ partials,
                                                      const SegCtl* __restrict__ ctl)
 {
+  typedef float f4u __attribute__((ext_vector_type(4), aligned(4)));
   __shared__ double red[WG / 64];
-  const long long total = (long long)ctl->total;
-  for (long long c = blockIdx.x; c < total; c += gridDim.x) {
-    const int pos = find_position(cstart, NCLASS * m, c);
-    const int s = seg_of_pos(pos, m);
-    const long long k = c - cstart[pos];
-    const long long lo = off[s] + k * SUM_CH;
-    const long long end = off[s + 1];
-    const long long hi = lo + SUM_CH < end ? lo + SUM_CH : end;
-    DF acc{0.0f, 0.0f};
+  const long long c = blockIdx.x;
+  if (c >= (long long)ctl->total) return;
+  const int s = cseg[c];
+  const long long k = c - cstart[s];
+  const long long lo = off[s] + k * SUM_CH;
+  const long long end = off[s + 1];
+  const long long hi = lo + SUM_CH < end ? lo + SUM_CH : end;
+  double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+  if (hi - lo == SUM_CH) {
+    f4u v[SUM_J];
+#pragma unroll
+    for (int j = 0; j < SUM_J; ++j)
+      v[j] = __builtin_nontemporal_load(
+          reinterpret_cast<const f4u*>(in + lo + ((long long)j * WG + threadIdx.x) * 4));
+#pragma unroll
+    for (int j = 0; j < SUM_J; ++j) {
+      a0 += (double)v[j][0];
+      a1 += (double)v[j][1];
+      a2 += (double)v[j][2];
+      a3 += (double)v[j][3];
+    }
+  } else {
+    // the last chunk of a segment
     long long i = lo + threadIdx.x;
     for (; i + 3 * WG < hi; i += 4 * WG) {
       const float x0 = in[i], x1 = in[i + WG], x2 = in[i + 2 * WG], x3 = in[i + 3 * WG];
-      acc = df_add(acc, x0);
-      acc = df_add(acc, x1);
-      acc = df_add(acc, x2);
-      acc = df_add(acc, x3);
+      a0 += (double)x0;
+      a1 += (double)x1;
+      a2 += (double)x2;
+      a3 += (double)x3;
     }
-    for (; i < hi; i += WG) acc = df_add(acc, in[i]);
-    const double tot = block_sum_double((double)acc.hi + (double)acc.lo, red);
-    if (threadIdx.x == 0) partials[c] = tot;
+    for (; i < hi; i += WG) a0 += (double)in[i];
   }
+  const double tot = block_sum_double((a0 + a1) + (a2 + a3), red);
+  if (threadIdx.x == 0) partials[c] = tot;
 }
 
-// one wave per segment: partials in a fixed order (lane-strided FP64 sums,
-// then a fixed shuffle tree)
+// one wave per segment: its partials in a fixed order (lane-strided FP64
+// sums, then a fixed shuffle tree)
 __global__ __launch_bounds__(WG) void seg_combine_kernel(const long long* __restrict__ cstart,
-                                                         int nseg, int m,
+                                                         int nseg,
                                                          const double* __restrict__ partials,
                                                          float* __restrict__ out,
                                                          const SegCtl* __restrict__ ctl)
@@ -173,8 +163,7 @@ __global__ __launch_bounds__(WG) void seg_combine_kernel(const long long* __rest
     if (lane == 0) out[s] = __builtin_nanf("");
     return;
   }
-  const int pos = pos_of_seg(s, m);
-  const long long c0 = cstart[pos], c1 = cstart[pos + 1];
+  const long long c0 = cstart[s], c1 = cstart[s + 1];
   double v = 0.0;
   for (long long c = c0 + lane; c < c1; c += 64) v += partials[c];
   for (int o = 32; o >= 1; o >>= 1) v += __shfl_down(v, o, 64);
@@ -523,30 +512,20 @@ size_t max_chunks(unsigned long long n, int nseg, long long ch)
   return (size_t)(n / (unsigned long long)ch) + (size_t)nseg + 1;
 }
 
-int classes_m(int nseg) { return (nseg + NCLASS - 1) / NCLASS; }
+size_t pad256(size_t b) { return (b + 255) & ~(size_t)255; }
 
-// workspace: SegCtl (256 B) | 64 ticket counters, 256 B apart | cstart over
-// the 64 * M positions (+1) | pad to 256 | per-chunk data
-size_t head_bytes(int nseg)
+// Reduce workspace: SegCtl (256 B) | cstart over the segments (+1) | cseg
+// over the chunks | FP64 partials over the chunks
+size_t sum_bytes(unsigned long long n, int nseg)
 {
-  const size_t npos = (size_t)NCLASS * classes_m(nseg);
-  return (256 + (size_t)NCLASS * CTR_STRIDE * 8 + (npos + 1) * 8 + 255) & ~(size_t)255;
-}
-
-void carve(void* ws, SegCtl** ctl, unsigned long long** ctr, long long** cstart, void** data,
-           int nseg)
-{
-  char* p = static_cast<char*>(ws);
-  *ctl = reinterpret_cast<SegCtl*>(p);
-  *ctr = reinterpret_cast<unsigned long long*>(p + 256);
-  *cstart = reinterpret_cast<long long*>(p + 256 + (size_t)NCLASS * CTR_STRIDE * 8);
-  *data = p + head_bytes(nseg);
+  const size_t ch = max_chunks(n, nseg, SUM_CH);
+  return 256 + pad256(((size_t)nseg + 1) * 8) + pad256(ch * 4) + ch * 8;
 }
 
 // flat Scan workspace: tile ticket (256 B) | error word (256 B) | tfirst over
 // the tiles (+1), padded to 256 | two 64-bit status words per tile
 long long flat_tiles(unsigned long long n) { return (long long)((n + FTILE - 1) / FTILE); }
-size_t flat_table_bytes(long long ntiles) { return ((size_t)(ntiles + 1) * 4 + 255) & ~(size_t)255; }
+size_t flat_table_bytes(long long ntiles) { return pad256((size_t)(ntiles + 1) * 4); }
 size_t flat_bytes(unsigned long long n)
 {
   const long long t = flat_tiles(n);
@@ -557,7 +536,7 @@ size_t flat_bytes(unsigned long long n)
 
 extern "C" size_t skl_internal_segmented_workspace(unsigned long long n, int nseg)
 {
-  const size_t sum = head_bytes(nseg) + max_chunks(n, nseg, SUM_CH) * 8;
+  const size_t sum = sum_bytes(n, nseg);
   const size_t scan = flat_bytes(n);
   return sum > scan ? sum : scan;
 }
@@ -566,19 +545,17 @@ extern "C" int skl_internal_segmented_sum(float* out, const float* in, unsigned 
                                           const int64_t* offsets, int nseg, void* ws,
                                           hipStream_t s)
 {
-  SegCtl* ctl;
-  unsigned long long* ctr;
-  long long* cstart;
-  void* data;
-  carve(ws, &ctl, &ctr, &cstart, &data, nseg);
-  const int m = classes_m(nseg);
-  seg_prologue<<<1, 1024, 0, s>>>(offsets, nseg, m, n, SUM_CH, cstart, ctr, ctl);
-  const size_t chunks = max_chunks(n, nseg, SUM_CH);
-  const size_t cap = (size_t)device_cus() * 8;
-  const unsigned grid = (unsigned)(chunks < cap ? chunks : cap);
-  seg_sum_kernel<<<grid, WG, 0, s>>>(in, offsets, m, cstart, static_cast<double*>(data), ctl);
+  const size_t cap = max_chunks(n, nseg, SUM_CH);
+  if (cap > 0x7FFFFFFEull) return SKL_ERR_INVALID;
+  char* p = static_cast<char*>(ws);
+  SegCtl* ctl = reinterpret_cast<SegCtl*>(p);
+  long long* cstart = reinterpret_cast<long long*>(p + 256);
+  int* cseg = reinterpret_cast<int*>(p + 256 + pad256(((size_t)nseg + 1) * 8));
+  double* partials = reinterpret_cast<double*>(reinterpret_cast<char*>(cseg) + pad256(cap * 4));
+  seg_sum_prologue<<<1, 1024, 0, s>>>(offsets, nseg, n, (long long)cap, cstart, cseg, ctl);
+  seg_sum_kernel<<<(unsigned)cap, WG, 0, s>>>(in, offsets, cstart, cseg, partials, ctl);
   seg_combine_kernel<<<(unsigned)((nseg + WG / 64 - 1) / (WG / 64)), WG, 0, s>>>(
-      cstart, nseg, m, static_cast<const double*>(data), out, ctl);
+      cstart, nseg, partials, out, ctl);
   return hipGetLastError() == hipSuccess ? SKL_OK : SKL_ERR_HIP;
 }
 

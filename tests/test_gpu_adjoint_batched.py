@@ -259,6 +259,29 @@ def test_segmented_scan_layouts(skl, case, inclusive):
     assert np.all(got[~inside] == 7.0)
 
 
+SUM_CHUNK = 16384  # elements per chunk of the segmented Reduce
+
+
+@pytest.mark.parametrize("case", ["dense_tiny", "chunk_multiples"])
+def test_segmented_reduce_layouts(skl, case):
+    """Runs of empty and one-element segments, and segments whose lengths are
+    exact multiples of the chunk (full chunks only) or one off, starting at
+    offsets that are not multiples of 4 (the full-chunk path reads
+    dword-aligned packs)."""
+    rng = np.random.default_rng(7 if case == "dense_tiny" else 8)
+    if case == "dense_tiny":
+        lens = rng.integers(0, 4, 30000)
+    else:
+        C = SUM_CHUNK
+        lens = np.array([1, C, 2 * C, C - 1, C + 1, 0, 3 * C, 5, 7 * C + 2], np.int64)
+    off = np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)
+    x = oracle.gen_f32_sym(0x5040, int(off[-1]))
+    r = skl.Reduce("float(float)", "float func(float x, float y){ return x+y; }")
+    got = np.asarray(r.segmented(skl.Vector(x), off)).astype(np.float64)
+    ref = oracle.segmented_sum_f64(x, off)
+    assert np.all(np.abs(got - ref) <= np.spacing(np.abs(ref).astype(np.float32)) + 1e-12)
+
+
 def test_segmented_scan_repeated_and_resized(skl):
     """The tile ticket and status words are reused across calls: alternate
     sizes on one skeleton, every result checked."""
