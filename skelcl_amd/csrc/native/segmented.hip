@@ -34,57 +34,67 @@ struct SegCtl {
 };
 
 // Reduce prologue, one work-group of 1024 threads: validates the offsets,
-// cuts segment s (length L) into ceil(L / SUM_CH) chunks, writes
-// cstart[s] = its first chunk (cstart[nseg] = total) and cseg[c] = the
-// segment of chunk c (c < cap).
+// cuts segment s (length L) into ceil(L / SUM_CH) chunks and writes
+// cstart[s] = its first chunk (cstart[nseg] = total; the total is 0 after an
+// error or beyond cap chunks).  Thread t owns the segments
+// [t * per, (t + 1) * per): it reads their offsets (loads independent of one
+// another, so they are in flight together), one block scan of the per-thread
+// chunk counts places them, and a second pass over the same offsets writes.
 __global__ __launch_bounds__(1024) void seg_sum_prologue(const int64_t* __restrict__ off,
                                                          int nseg, unsigned long long n,
                                                          long long cap,
                                                          long long* __restrict__ cstart,
-                                                         int* __restrict__ cseg, SegCtl* ctl)
+                                                         SegCtl* ctl)
 {
   __shared__ long long wsum[1024 / 64];
-  __shared__ long long carry;
   __shared__ unsigned err;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  if (threadIdx.x == 0) {
-    carry = 0;
+  const int per = (nseg + 1023) / 1024;
+  const int s0 = (int)threadIdx.x * per;
+  const int s1 = s0 + per < nseg ? s0 + per : nseg;
+  if (threadIdx.x == 0)
     err = (off[0] < 0 || off[nseg] < 0 || (unsigned long long)off[nseg] > n) ? 2u : 0u;
-  }
   __syncthreads();
-  for (int base = 0; base < nseg; base += 1024) {
-    const int s = base + (int)threadIdx.x;
-    long long cnt = 0;
-    if (s < nseg) {
-      const long long len = off[s + 1] - off[s];
-      if (len < 0) atomicOr(&err, 1u);
-      else cnt = (len + SUM_CH - 1) / SUM_CH;
-    }
-    // inclusive scan of the 1024 counts: wave scans, then the wave totals
-    long long v = cnt;
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-      const long long t = __shfl_up(v, o, 64);
-      if (lane >= o) v += t;
-    }
-    if (lane == 63) wsum[wv] = v;
-    __syncthreads();
-    long long pre = carry;
-    for (int w = 0; w < wv; ++w) pre += wsum[w];
-    const long long first = pre + v - cnt;
-    if (s < nseg) {
-      cstart[s] = first;
-      for (long long c = first; c < first + cnt && c < cap; ++c) cseg[c] = s;
-    }
-    __syncthreads();
-    if (threadIdx.x == 1023) carry = first + cnt;
-    __syncthreads();
+  long long tot = 0;
+  bool bad = false;
+#pragma unroll 8
+  for (int s = s0; s < s1; ++s) {
+    const long long len = off[s + 1] - off[s];
+    if (len < 0) bad = true;
+    else tot += (len + SUM_CH - 1) / SUM_CH;
   }
-  if (threadIdx.x == 0) {
-    cstart[nseg] = carry;
-    ctl->total = err ? 0 : (unsigned long long)carry;
+  if (bad) atomicOr(&err, 1u);
+  long long v = tot;  // inclusive scan over the threads
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const long long t = __shfl_up(v, o, 64);
+    if (lane >= o) v += t;
+  }
+  if (lane == 63) wsum[wv] = v;
+  __syncthreads();
+  long long c = v - tot;
+  for (int w = 0; w < wv; ++w) c += wsum[w];
+  for (int s = s0; s < s1; ++s) {
+    const long long len = off[s + 1] - off[s];
+    cstart[s] = c;
+    c += len < 0 ? 0 : (len + SUM_CH - 1) / SUM_CH;
+  }
+  if (threadIdx.x == 1023) {
+    cstart[nseg] = c;
+    ctl->total = err || (unsigned long long)c > (unsigned long long)cap ? 0 : (unsigned long long)c;
     ctl->error = err;
   }
+}
+
+// one wave per segment: cseg[c] = s over the segment's chunks
+__global__ __launch_bounds__(256) void seg_sum_fill(const long long* __restrict__ cstart,
+                                                    int nseg, int* __restrict__ cseg,
+                                                    const SegCtl* __restrict__ ctl)
+{
+  const int s = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (s >= nseg || ctl->total == 0) return;
+  const long long c1 = cstart[s + 1];
+  for (long long c = cstart[s] + (threadIdx.x & 63); c < c1; c += 64) cseg[c] = s;
 }
 
 __device__ __forceinline__ double block_sum_double(double v, double* red)
@@ -552,7 +562,8 @@ extern "C" int skl_internal_segmented_sum(float* out, const float* in, unsigned 
   long long* cstart = reinterpret_cast<long long*>(p + 256);
   int* cseg = reinterpret_cast<int*>(p + 256 + pad256(((size_t)nseg + 1) * 8));
   double* partials = reinterpret_cast<double*>(reinterpret_cast<char*>(cseg) + pad256(cap * 4));
-  seg_sum_prologue<<<1, 1024, 0, s>>>(offsets, nseg, n, (long long)cap, cstart, cseg, ctl);
+  seg_sum_prologue<<<1, 1024, 0, s>>>(offsets, nseg, n, (long long)cap, cstart, ctl);
+  seg_sum_fill<<<(unsigned)((nseg + 3) / 4), 256, 0, s>>>(cstart, nseg, cseg, ctl);
   seg_sum_kernel<<<(unsigned)cap, WG, 0, s>>>(in, offsets, cstart, cseg, partials, ctl);
   seg_combine_kernel<<<(unsigned)((nseg + WG / 64 - 1) / (WG / 64)), WG, 0, s>>>(
       cstart, nseg, partials, out, ctl);
