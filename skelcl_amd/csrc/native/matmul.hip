@@ -24,7 +24,6 @@
 
 #include "skelcl_skeletons.h"
 #include "ld_util.h"
-#include "work_ticket.h"
 
 namespace {
 
@@ -724,7 +723,7 @@ static int device_cus()
 // Workspace sizes (bytes) of the MFMA forms for an M x N x K product.
 extern "C" size_t skl_internal_matmul_workspace(size_t M, size_t N, size_t K, int precision)
 {
-  const size_t Mp = round_up(M, 128), Np = round_up(N, 128);
+  const size_t Np = round_up(N, 128);
   if (precision == SKL_PREC_BF16X3) {
     const size_t Kp = round_up(K, 64);
     return 3 * (round_up(M, 256) + Np) * Kp * 2 + 256;  // A rows in pairs of blocks
@@ -740,7 +739,7 @@ extern "C" int skl_internal_matmul(float* C, const float* A, const float* B, int
                                    int K, int ta, int tb, int precision, void* ws,
                                    hipStream_t s)
 {
-  const int Mp = (int)round_up(M, 128), Np = (int)round_up(N, 128);
+  const int Np = (int)round_up(N, 128);
   if (precision == SKL_PREC_BF16X3) {
     const int Kp = (int)round_up(K, 64);
     const int Ma = (int)round_up(M, 256);  // A rows: whole pairs of 128-row blocks
@@ -810,42 +809,69 @@ extern "C" int skl_internal_matmul(float* C, const float* A, const float* B, int
 //   BF16X3  the FP32 operand slices are split into three bf16 terms with the
 //           hardware round-to-nearest conversion while they are staged in
 //           LDS (no pre-pass; the next slice is loaded during the MFMAs),
-//           then the six MFMA products of gemm_bf16x3_kernel.
+//           then the six MFMA products of gemm_bf16x3_kernel.  Products
+//           with K and N multiples of 4 go to a variant that stages with
+//           16-byte loads and is held to 256 VGPRs (two work-groups per
+//           CU); the others to one with element loads.  The prologue numbers
+//           the tiles of each class separately.
 namespace {
 
 struct BatchCtl {
-  unsigned long long total;
+  unsigned long long total;      // tiles of all products (FP32 kernel)
+  unsigned long long total_vec;  // tiles of products with K % 4 == 0 and N % 4 == 0
+  unsigned long long total_gen;  // tiles of the other products
 };
 
+__device__ __forceinline__ bool batch_vec(const skl_gemm_desc& g)
+{
+  return (g.K & 3) == 0 && (g.N & 3) == 0;
+}
+
+// tstart[0 .. count]: first tile of each product over all products;
+// tstart[count + 1 + p] / tstart[2 (count + 1) + p]: the same numbering
+// over the products of one class (the others count no tiles).
 __global__ __launch_bounds__(1024) void batch_prologue(const skl_gemm_desc* __restrict__ d,
                                                        int count, long long* __restrict__ tstart,
                                                        BatchCtl* ctl)
 {
-  __shared__ long long part[1024];
-  __shared__ long long carry;
-  if (threadIdx.x == 0) carry = 0;
+  __shared__ long long part[3][1024];
+  __shared__ long long carry[3];
+  if (threadIdx.x < 3) carry[threadIdx.x] = 0;
   __syncthreads();
   for (int base = 0; base < count; base += 1024) {
     const int p = base + threadIdx.x;
     long long t = 0;
-    if (p < count && d[p].M > 0 && d[p].N > 0)
+    bool v = false;
+    if (p < count && d[p].M > 0 && d[p].N > 0) {
       t = (long long)((d[p].M + BM - 1) / BM) * ((d[p].N + BN - 1) / BN);
-    part[threadIdx.x] = t;
+      v = batch_vec(d[p]);
+    }
+    const long long tc[3] = {t, v ? t : 0, v ? 0 : t};
+#pragma unroll
+    for (int c = 0; c < 3; ++c) part[c][threadIdx.x] = tc[c];
     __syncthreads();
     for (int s = 1; s < 1024; s <<= 1) {
-      long long v = threadIdx.x >= (unsigned)s ? part[threadIdx.x - s] : 0;
+      long long w[3];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) w[c] = threadIdx.x >= (unsigned)s ? part[c][threadIdx.x - s] : 0;
       __syncthreads();
-      part[threadIdx.x] += v;
+#pragma unroll
+      for (int c = 0; c < 3; ++c) part[c][threadIdx.x] += w[c];
       __syncthreads();
     }
-    if (p < count) tstart[p] = carry + part[threadIdx.x] - t;
+    if (p < count)
+#pragma unroll
+      for (int c = 0; c < 3; ++c)
+        tstart[c * (count + 1) + p] = carry[c] + part[c][threadIdx.x] - tc[c];
     __syncthreads();
-    if (threadIdx.x == 1023) carry += part[1023];
+    if (threadIdx.x < 3) carry[threadIdx.x] += part[threadIdx.x][1023];
     __syncthreads();
   }
+  if (threadIdx.x < 3) tstart[threadIdx.x * (count + 1) + count] = carry[threadIdx.x];
   if (threadIdx.x == 0) {
-    tstart[count] = carry;
-    ctl->total = (unsigned long long)carry;
+    ctl->total = (unsigned long long)carry[0];
+    ctl->total_vec = (unsigned long long)carry[1];
+    ctl->total_gen = (unsigned long long)carry[2];
   }
 }
 
@@ -891,16 +917,18 @@ __device__ __forceinline__ void split3_hw(float x, __bf16& h0, __bf16& h1, __bf1
 constexpr int BKS = 32;               // k per slice
 constexpr int BLD = BKS + 8;          // bf16 per LDS row (padding vs bank conflicts)
 
-__global__ __launch_bounds__(256) void batch_bf16x3_kernel(const skl_gemm_desc* __restrict__ d,
-                                                           int count,
-                                                           const long long* __restrict__ tstart,
-                                                           const BatchCtl* __restrict__ ctl)
+// VEC: products with K % 4 == 0 and N % 4 == 0 only (16-byte staging loads,
+// two waves per SIMD); otherwise the others (element loads)
+template <bool VEC>
+__device__ __forceinline__ void batch_bf16x3_body(const skl_gemm_desc* __restrict__ d, int count,
+                                                  const long long* __restrict__ tstart,
+                                                  const BatchCtl* __restrict__ ctl)
 {
   __shared__ __attribute__((aligned(16))) __bf16 As[3][BM][BLD];
   __shared__ __attribute__((aligned(16))) __bf16 Bs[3][BN][BLD];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = wave >> 1, wn = wave & 1;
-  const long long total = (long long)ctl->total;
+  const long long total = (long long)(VEC ? ctl->total_vec : ctl->total_gen);
   for (long long t = blockIdx.x; t < total; t += gridDim.x) {
     const int p = find_item(tstart, count, t);
     const skl_gemm_desc g = d[p];
@@ -915,14 +943,39 @@ __global__ __launch_bounds__(256) void batch_bf16x3_kernel(const skl_gemm_desc* 
         acc0[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
         acc1[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
       }
-    // per thread and slice: 4 groups of 4 consecutive k of A (128 x 32) and
-    // of B (32 x 128).  The prefetch loads go straight into ra / rb from
-    // clamped indices (a select right after a load makes the compiler wait
-    // for it); the out-of-range elements are zeroed when the slice is
-    // written to LDS, each group's three bf16 terms as one 8-byte store per
-    // plane.
+    // Per thread and slice, A: 4 groups of 4 consecutive k (row r, k kq..kq+3).
+    // B: when K and N are multiples of 4 (vec), a 4 x 4 block (k 4kb..4kb+3,
+    // n 4nb..4nb+3) read as 4 float4 rows and transposed in registers;
+    // otherwise 4 groups of 4 consecutive k of one column.  With vec every
+    // 16-byte pack lies wholly inside or outside the operand.  The prefetch
+    // loads go straight into ra / rb from clamped indices (a select right
+    // after a load makes the compiler wait for it); out-of-range elements
+    // are zeroed when the slice is written to LDS, each group's three bf16
+    // terms as one 8-byte store per plane.
+    typedef float f4a __attribute__((ext_vector_type(4), aligned(4)));
+    constexpr bool vec = VEC;
+    const int vkb = tid >> 5, vnb = tid & 31;
     float ra[4][4], rb[4][4];
     auto gload = [&](int k0) {
+      if constexpr (vec) {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const int e = tid + 256 * j;
+          const int gr = m0 + (e >> 3), gk = k0 + (e & 7) * 4;
+          const f4a v = *reinterpret_cast<const __attribute__((address_space(1))) f4a*>(
+              as_global(g.A) + (gr < g.M && gk < g.K ? (size_t)gr * g.K + gk : 0));
+#pragma unroll
+          for (int i = 0; i < 4; ++i) ra[j][i] = v[i];
+        }
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const int gk = k0 + 4 * vkb + i, gc = n0 + 4 * vnb;
+          const f4a v = *reinterpret_cast<const __attribute__((address_space(1))) f4a*>(
+              as_global(g.B) + (gk < g.K && gc < g.N ? (size_t)gk * g.N + gc : 0));
+#pragma unroll
+          for (int c = 0; c < 4; ++c) rb[i][c] = v[c];
+        }
+      } else {
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
         const int e = tid + 256 * j;
@@ -935,6 +988,7 @@ __global__ __launch_bounds__(256) void batch_bf16x3_kernel(const skl_gemm_desc* 
           const int gkb = k0 + kb + i, gc = n0 + n;
           rb[j][i] = as_global(g.B)[gkb < g.K && gc < g.N ? (size_t)gkb * g.N + gc : 0];
         }
+      }
       }
     };
     auto put4 = [&](__bf16 (*plane)[BLD], int row, int k, const float* v, int part) {
@@ -954,17 +1008,33 @@ __global__ __launch_bounds__(256) void batch_bf16x3_kernel(const skl_gemm_desc* 
       for (int j = 0; j < 4; ++j) {
         const int e = tid + 256 * j;
         const int r = e >> 3, kq = (e & 7) * 4;
-        const int n = e & 127, kb = (e >> 7) * 4;
-        float va[4], vb[4];
+        float va[4];
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-          va[i] = m0 + r < g.M && k0 + kq + i < g.K ? ra[j][i] : 0.0f;
-          vb[i] = k0 + kb + i < g.K && n0 + n < g.N ? rb[j][i] : 0.0f;
+        for (int i = 0; i < 4; ++i) va[i] = m0 + r < g.M && k0 + kq + i < g.K ? ra[j][i] : 0.0f;
+#pragma unroll
+        for (int q = 0; q < 3; ++q) put4(As[q], r, kq, va, q);
+      }
+      if constexpr (vec) {
+        // column 4nb + c of the block: k 4kb..4kb+3
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+          float vb[4];
+#pragma unroll
+          for (int i = 0; i < 4; ++i)
+            vb[i] = k0 + 4 * vkb + i < g.K && n0 + 4 * vnb < g.N ? rb[i][c] : 0.0f;
+#pragma unroll
+          for (int q = 0; q < 3; ++q) put4(Bs[q], 4 * vnb + c, 4 * vkb, vb, q);
         }
+      } else {
 #pragma unroll
-        for (int q = 0; q < 3; ++q) {
-          put4(As[q], r, kq, va, q);
-          put4(Bs[q], n, kb, vb, q);
+        for (int j = 0; j < 4; ++j) {
+          const int e = tid + 256 * j;
+          const int n = e & 127, kb = (e >> 7) * 4;
+          float vb[4];
+#pragma unroll
+          for (int i = 0; i < 4; ++i) vb[i] = k0 + kb + i < g.K && n0 + n < g.N ? rb[j][i] : 0.0f;
+#pragma unroll
+          for (int q = 0; q < 3; ++q) put4(Bs[q], n, kb, vb, q);
         }
       }
     };
@@ -974,15 +1044,31 @@ __global__ __launch_bounds__(256) void batch_bf16x3_kernel(const skl_gemm_desc* 
       __syncthreads();
       if (k0 + BKS < g.K) gload(k0 + BKS);  // next slice in flight during the MFMAs
       const int fr = lane & 15, fk = (lane >> 4) * 8;
-      bf16x8 a[4][3], b[4][3];
+      // B fragments held for the slice, A fragments read per 16-row block:
+      // per (i, j) the same order of the six products as SKL_BF16X3_STEP
+      bf16x8 b[4][3];
 #pragma unroll
-      for (int i = 0; i < 4; ++i)
+      for (int j = 0; j < 4; ++j)
 #pragma unroll
-        for (int q = 0; q < 3; ++q) {
-          a[i][q] = *reinterpret_cast<const bf16x8*>(&As[q][wm * 64 + i * 16 + fr][fk]);
-          b[i][q] = *reinterpret_cast<const bf16x8*>(&Bs[q][wn * 64 + i * 16 + fr][fk]);
-        }
-      SKL_BF16X3_STEP(acc0, acc1, a, b);
+        for (int q = 0; q < 3; ++q)
+          b[j][q] = *reinterpret_cast<const bf16x8*>(&Bs[q][wn * 64 + j * 16 + fr][fk]);
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        bf16x8 a[3];
+#pragma unroll
+        for (int q = 0; q < 3; ++q)
+          a[q] = *reinterpret_cast<const bf16x8*>(&As[q][wm * 64 + i * 16 + fr][fk]);
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+          acc0[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[0], b[j][0], acc0[i][j], 0, 0, 0);
+        constexpr int pa[5] = {1, 0, 2, 0, 1}, pb[5] = {1, 2, 0, 1, 0};
+#pragma unroll
+        for (int t = 0; t < 5; ++t)
+#pragma unroll
+          for (int j = 0; j < 4; ++j)
+            acc1[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[pa[t]], b[j][pb[t]],
+                                                                 acc1[i][j], 0, 0, 0);
+      }
       __syncthreads();
     }
     const int cc = lane & 15, cr = (lane >> 4) * 4;
@@ -1000,11 +1086,26 @@ __global__ __launch_bounds__(256) void batch_bf16x3_kernel(const skl_gemm_desc* 
   }
 }
 
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void
+batch_bf16x3_vec_kernel(const skl_gemm_desc* __restrict__ d, int count,
+                        const long long* __restrict__ tstart, const BatchCtl* __restrict__ ctl)
+{
+  batch_bf16x3_body<true>(d, count, tstart, ctl);
+}
+
+__global__ __launch_bounds__(256) void batch_bf16x3_gen_kernel(const skl_gemm_desc* __restrict__ d,
+                                                               int count,
+                                                               const long long* __restrict__ tstart,
+                                                               const BatchCtl* __restrict__ ctl)
+{
+  batch_bf16x3_body<false>(d, count, tstart, ctl);
+}
+
 }  // namespace
 
 extern "C" size_t skl_internal_batched_workspace(int count)
 {
-  return 256 + (size_t)(count + 1) * 8;
+  return 256 + 3 * (size_t)(count + 1) * 8;
 }
 
 extern "C" int skl_internal_batched_gemm_ws(const skl_gemm_desc* descs, int count, int precision,
@@ -1019,9 +1120,10 @@ extern "C" int skl_internal_batched_gemm_ws(const skl_gemm_desc* descs, int coun
       cus = v;
   }
   batch_prologue<<<1, 1024, 0, s>>>(descs, count, tstart, ctl);
-  if (precision == SKL_PREC_BF16X3)
-    batch_bf16x3_kernel<<<cus * 2, 256, 0, s>>>(descs, count, tstart, ctl);
-  else
-    batch_f32_kernel<<<cus * 2, 256, 0, s>>>(descs, count, tstart, ctl);
+  if (precision == SKL_PREC_BF16X3) {
+    batch_bf16x3_vec_kernel<<<cus * 2, 256, 0, s>>>(descs, count, tstart + (count + 1), ctl);
+    batch_bf16x3_gen_kernel<<<cus, 256, 0, s>>>(descs, count, tstart + 2 * (count + 1), ctl);
+  } else
+    batch_f32_kernel<<<cus * 3, 256, 0, s>>>(descs, count, tstart, ctl);  // 3 fit per CU
   return st(hipGetLastError());
 }

@@ -355,6 +355,36 @@ def test_batched_matmul_ragged_shapes(skl, prec):
         assert np.linalg.norm(c - ref) / max(np.linalg.norm(ref), 1e-30) < 1e-6, (M, N)
 
 
+@pytest.mark.parametrize("prec", [1, 2])
+def test_batched_matmul_mixed_classes(skl, prec):
+    """One batch holding products with K and N multiples of 4 (16-byte
+    staging, partial 128x128 tiles included) and products without (element
+    staging): each class is numbered and run by its own kernel."""
+    from skelcl_amd import _lib
+    shapes = [(129, 260, 36), (3, 130, 7), (1, 4, 4), (200, 8, 64), (64, 64, 1),
+              (130, 132, 100), (0, 8, 8), (257, 12, 4)]
+    descs = (_lib.skl_gemm_desc * len(shapes))()
+    keep, refs = [], []
+    for i, (M, N, K) in enumerate(shapes):
+        A = oracle.gen_f32_sym(0x5200 + 2 * i, max(M * K, 1)).reshape(-1)[:M * K].reshape(M, K)
+        B = oracle.gen_f32_sym(0x5201 + 2 * i, K * N).reshape(K, N)
+        dA = skl.DeviceBuffer.from_array(A) if M * K else skl.DeviceBuffer(16)
+        dB = skl.DeviceBuffer.from_array(B)
+        dC = skl.DeviceBuffer(max(4 * M * N, 16))
+        keep += [dA, dB, dC]
+        descs[i].A, descs[i].B, descs[i].C = dA.value, dB.value, dC.value
+        descs[i].M, descs[i].N, descs[i].K = M, N, K
+        if M:
+            refs.append((dC, M, N, A.astype(np.float64) @ B.astype(np.float64)))
+    dd = skl.DeviceBuffer(C.sizeof(descs))
+    _lib.check(_lib.lib().skl_memcpy_h2d(dd.ptr, 0, C.cast(descs, C.c_void_p), C.sizeof(descs),
+                                         None, 1))
+    _lib.check(_lib.lib().skl_matmul_batched_f32(dd.ptr, len(shapes), prec, None))
+    for dC, M, N, ref in refs:
+        c = dC.to_numpy(np.float32, M * N).reshape(M, N)
+        assert np.linalg.norm(c - ref) / max(np.linalg.norm(ref), 1e-30) < 1e-6, (M, N)
+
+
 def test_graph_capture_zip_reduce(skl):
     """Zip->Reduce fused, captured into a graph and replayed: every replay
     equals the eager result (the reduce counter re-arms itself, no memset
