@@ -57,8 +57,10 @@ enum skl_skeleton_kind {
 enum skl_padding { SKL_PADDING_NEUTRAL = 0, SKL_PADDING_NEAREST = 1 };
 
 /* Precision form of the float paths (north-star extension):
- *  DEFAULT  Reduce/Scan(+) on float: compensated per-block partials with an
- *           FP64 cross-block combine; AllPairs (+,*) on float: plain FP32.
+ *  DEFAULT  Reduce(+) on float: compensated per-block partials with an
+ *           FP64 cross-block combine; Scan(+) on float: FP64 prefixes and
+ *           carries, one rounding per output; AllPairs (+,*) on float:
+ *           plain FP32.
  *  FP32     plain FP32 everywhere (user function, FP32 accumulation).
  *  BF16X3   AllPairs (+,*) on float: bf16x3 split on MFMA, FP32 accumulate.
  *  FP8      AllPairs (+,*) on float: FP8 e4m3 MFMA, per-128x128-tile scale. */

@@ -23,7 +23,7 @@
 // SKL_CARRY_T, skl_to_carry, SKL_CARRY_COMB, SKL_CARRY_ID, SKL_PREFIX_T,
 // skl_make_prefix(carry, accprefix), SKL_EMIT(prefix, acc),
 // skl_emit0(prefix).  For float addition (precision DEFAULT) the tile-local
-// values are compensated double-float pairs and the carry is FP64.
+// prefixes and the carry are FP64.
 // ---------------------------------------------------------------------------
 
 #define SKL_SCAN_AGG 1u

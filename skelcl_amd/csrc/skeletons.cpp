@@ -46,6 +46,10 @@ void skl_skeleton_s::apply_default_shape(int cus)
 
 int skl_skeleton_s::default_scan_j() const
 {
+  // float (+) Scan, FP64 prefixes: 16 packs per lane measured fastest (2^28
+  // and 577.5 M elements, forward and reversed) although the FP64 register
+  // tile is twice the float one
+  if (precise_add() && kind == SKL_SCAN) return 16;
   const size_t acc = precise_add() ? 2 * sizeof(float) : (types.empty() ? 4 : types[0].size);
   const size_t per_pack = (size_t)(V > 0 ? V : 1) * (acc > 0 ? acc : 1);
   size_t j = 256 / per_pack;
@@ -237,7 +241,26 @@ std::vector<Param> extras_of(const UserFunction& f, int lead)
 // accumulator hooks for Reduce / Scan
 void emit_reduce_scan_hooks(std::ostringstream& o, const skl_skeleton_s* s)
 {
-  if (s->precise_add()) {
+  if (s->precise_add() && s->kind == SKL_SCAN) {
+    // float (+) Scan: FP64 tile prefixes and carry (one conversion and one
+    // FP64 add per element, against about six FP32 operations for a
+    // compensated pair), every output one rounding of the FP64 prefix
+    o << R"(
+typedef double SKL_ACC_T;
+typedef double SKL_CARRY_T;
+typedef double SKL_PREFIX_T;
+__device__ __forceinline__ double skl_lift(const float& x) { return (double)x; }
+#define SKL_ACC_COMB(a, b) ((a) + (b))
+#define SKL_CARRY_COMB(a, b) ((a) + (b))
+__device__ __forceinline__ double skl_to_carry(const double& a) { return a; }
+__device__ __forceinline__ float skl_carry_out(const double& c) { return (float)c; }
+#define SKL_ACC_ID 0.0
+#define SKL_CARRY_ID 0.0
+#define skl_make_prefix(c, p) ((c) + (p))
+#define SKL_EMIT(P, q) ((float)((P) + (q)))
+__device__ __forceinline__ float skl_emit0(const double& P) { return (float)P; }
+)";
+  } else if (s->precise_add()) {
     o << R"(
 typedef skl_df SKL_ACC_T;
 typedef double SKL_CARRY_T;
