@@ -134,17 +134,52 @@ extern "C" __global__ __launch_bounds__(256) void skl_allpairs_general_lds(
                                          ((SKL_APG_B * (size_t)K * sizeof(SKL_TL) + 15) & ~(size_t)15));
   const int tid = threadIdx.x;
   const int m0 = blockIdx.y * SKL_APG_B, n0 = blockIdx.x * SKL_APG_B;
-  for (int e = tid; e < SKL_APG_B * K; e += 256) {
-    const int r = e / K, k = e - r * K;
-    SKL_TL z;
-    __builtin_memset(&z, 0, sizeof(z));
-    As[e] = skl_ld_or(A, (size_t)(m0 + r) * K + k, 0, m0 + r < M, z);
-  }
-  for (int e = tid; e < SKL_APG_B * K; e += 256) {
-    const int k = e / SKL_APG_B, c = e - k * SKL_APG_B;
-    SKL_TR z;
-    __builtin_memset(&z, 0, sizeof(z));
-    Bs[e] = skl_ld_or(B, (size_t)k * N + n0 + c, 0, n0 + c < N, z);
+  // 4-byte elements, K and N multiples of 4, 16-byte aligned operands: the
+  // 32 rows of A are one contiguous run of 32 K elements and B's 32 columns
+  // are 8 packs of 4 per k, so both tiles are copies of 16-byte packs (pack
+  // i of either tile is LDS pack i), 8 per thread in flight at a time.
+  const bool vec = sizeof(SKL_TL) == 4 && sizeof(SKL_TR) == 4 && (K & 3) == 0 &&
+                   (N & 3) == 0 && ((size_t)A & 15) == 0 && ((size_t)B & 15) == 0;
+  if (vec) {
+    typedef unsigned int u4 __attribute__((ext_vector_type(4)));
+    const int nv = 8 * K;                           // packs per tile
+    const long long alim = (long long)(M - m0) * K;  // valid A elements from row m0
+    const u4* ga = reinterpret_cast<const u4*>(A);
+    const u4* gb = reinterpret_cast<const u4*>(B);
+    for (int base = 0; base < nv; base += 256 * 8) {
+      u4 ra[8], rb[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int i = base + u * 256 + tid;
+        const bool aok = i < nv && 4LL * i < alim;
+        ra[u] = ga[aok ? ((size_t)m0 * K) / 4 + i : 0];
+        const int c = (i & 7) * 4;
+        const bool bok = i < nv && n0 + c < N;
+        rb[u] = gb[bok ? ((size_t)(i >> 3) * N + n0 + c) / 4 : 0];
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int i = base + u * 256 + tid;
+        if (i < nv) {
+          const u4 z = {0u, 0u, 0u, 0u};
+          reinterpret_cast<u4*>(As)[i] = 4LL * i < alim ? ra[u] : z;
+          reinterpret_cast<u4*>(Bs)[i] = n0 + (i & 7) * 4 < N ? rb[u] : z;
+        }
+      }
+    }
+  } else {
+    for (int e = tid; e < SKL_APG_B * K; e += 256) {
+      const int r = e / K, k = e - r * K;
+      SKL_TL z;
+      __builtin_memset(&z, 0, sizeof(z));
+      As[e] = skl_ld_or(A, (size_t)(m0 + r) * K + k, 0, m0 + r < M, z);
+    }
+    for (int e = tid; e < SKL_APG_B * K; e += 256) {
+      const int k = e / SKL_APG_B, c = e - k * SKL_APG_B;
+      SKL_TR z;
+      __builtin_memset(&z, 0, sizeof(z));
+      Bs[e] = skl_ld_or(B, (size_t)k * N + n0 + c, 0, n0 + c < N, z);
+    }
   }
   __syncthreads();
   const int tx = tid & 31, ty = tid >> 5;
@@ -162,6 +197,72 @@ extern "C" __global__ __launch_bounds__(256) void skl_allpairs_general_lds(
     SKL_TOUT v = SKL_CALL3(&Mm, &Nm, (unsigned)SKL_APG_DIM);
     const int r = m0 + lr, c = n0 + tx;
     if (r < M && c < N) C[(size_t)r * N + c] = v;
+  }
+}
+
+// Column-staged: 64 columns of right in LDS (K x 64), 64 rows per
+// work-group of 16 waves.  A wave computes one row at a time with one column
+// per lane, so the row's elements are wave-uniform: the compiler reads them
+// with scalar loads straight from left (the operands do not alias the
+// output: an output overlapping an input would race in any order), and the
+// only LDS traffic is the column element, half of what the form above reads.
+#define SKL_APG_COLS 64
+#define SKL_APG_ROWS 64
+extern "C" __global__ __launch_bounds__(1024) void skl_allpairs_general_cols(
+    const SKL_TL* __restrict__ A, const SKL_TR* __restrict__ B, SKL_TOUT* __restrict__ C, int M,
+    int N, int K SKL_EXTRA_PARAMS)
+{
+  SKL_EXTRA_SETUP
+  SKL_TR* Bs = reinterpret_cast<SKL_TR*>(skl_dyn_lds + SKL_APG_LDS_BASE);
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wv = (int)__builtin_amdgcn_readfirstlane((unsigned)(tid >> 6));
+  const int m0 = blockIdx.y * SKL_APG_ROWS, n0 = blockIdx.x * SKL_APG_COLS;
+  const bool vec = sizeof(SKL_TR) == 4 && (N & 3) == 0 && ((size_t)B & 15) == 0;
+  if (vec) {
+    // pack i: k = i / 16, columns 4 (i % 16) .. +3; LDS pack i
+    typedef unsigned int u4 __attribute__((ext_vector_type(4)));
+    const int nv = 16 * K;
+    const u4* gb = reinterpret_cast<const u4*>(B);
+    for (int base = 0; base < nv; base += 1024 * 4) {
+      u4 rb[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int i = base + u * 1024 + tid;
+        const int c = (i & 15) * 4;
+        const bool ok = i < nv && n0 + c < N;
+        rb[u] = gb[ok ? ((size_t)(i >> 4) * N + n0 + c) / 4 : 0];
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int i = base + u * 1024 + tid;
+        if (i < nv) {
+          const u4 z = {0u, 0u, 0u, 0u};
+          reinterpret_cast<u4*>(Bs)[i] = n0 + (i & 15) * 4 < N ? rb[u] : z;
+        }
+      }
+    }
+  } else {
+    for (int e = tid; e < SKL_APG_COLS * K; e += 1024) {
+      const int k = e >> 6, c = e & 63;
+      SKL_TR z;
+      __builtin_memset(&z, 0, sizeof(z));
+      Bs[e] = skl_ld_or(B, (size_t)k * N + n0 + c, 0, n0 + c < N, z);
+    }
+  }
+  __syncthreads();
+  rmatrix_t Nm;
+  Nm.data = Bs;
+  Nm.width = SKL_APG_COLS;
+  Nm.column = lane;
+  for (int i = 0; i < SKL_APG_ROWS / 16; ++i) {
+    const int r = m0 + wv + 16 * i;  // wave-uniform
+    if (r >= M) break;
+    lmatrix_t Mm;
+    Mm.data = A;
+    Mm.dimension = (unsigned)K;
+    Mm.row = r;
+    SKL_TOUT v = SKL_CALL3(&Mm, &Nm, (unsigned)SKL_APG_DIM);
+    if (n0 + lane < N) C[(size_t)r * N + n0 + lane] = v;
   }
 }
 

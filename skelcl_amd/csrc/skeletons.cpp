@@ -1100,7 +1100,15 @@ extern "C" int skl_allpairs_run(skl_skeleton s, void* out, const void* left, con
     if (gy > 65535) return fail(SKL_ERR_INVALID, "too many rows");
     return launch(k, (unsigned)((cols + 63) / 64), gy, 256, 1, 0, a, stream);
   }
-  // general form: stage rows and columns in LDS when they fit
+  // general form: 64 columns in LDS with the rows read as wave-uniform
+  // scalar loads, else 32 rows and 32 columns in LDS, when they fit
+  const size_t lds_cols = 64 * dim * s->types[2].size;
+  if (pk.dyn_lds == 0 && lds_cols <= 64 * 1024 && !s->simple) {
+    SKL_TRY(get_kernel(s, "dim" + std::to_string(dim), "skl_allpairs_general_cols", &k));
+    unsigned gy = (unsigned)((rows + 63) / 64);
+    if (gy > 65535) return fail(SKL_ERR_INVALID, "too many rows");
+    return launch(k, (unsigned)((cols + 63) / 64), gy, 1024, 1, lds_cols, a, stream);
+  }
   size_t lds = ((32 * dim * s->types[1].size + 15) & ~(size_t)15) + 32 * dim * s->types[2].size;
   if (pk.dyn_lds == 0 && lds <= 64 * 1024 && !s->simple) {
     SKL_TRY(get_kernel(s, "dim" + std::to_string(dim), "skl_allpairs_general_lds", &k));

@@ -104,8 +104,11 @@ def test_allpairs_plain_fp32_bit_exact_vs_fma_fold(skl):
     assert np.linalg.norm(out - ref) / np.linalg.norm(ref) < 1e-6
 
 
-@pytest.mark.parametrize("n,d", [(50, 20), (100, 256), (33, 300)])
+@pytest.mark.parametrize("n,d", [(50, 20), (100, 256), (33, 300), (70, 256), (130, 64)])
 def test_allpairs_general_sqdist(skl, n, d):
+    """d <= 256: 64 columns staged in LDS, rows read as wave-uniform scalar
+    loads (16-byte staging when n % 4 == 0, element staging otherwise, partial
+    64 x 64 tiles); d = 300: the form reading both operands from memory."""
     src = ("float func(lmatrix_t* row, rmatrix_t* col, const unsigned int dim) {"
            " float s = 0; for (unsigned k = 0; k < dim; ++k) {"
            " float t = getElementFromRow(row, k) - getElementFromColumn(col, k); s += t*t; }"
@@ -116,6 +119,23 @@ def test_allpairs_general_sqdist(skl, n, d):
     out = np.asarray(ap(skl.Matrix(P), skl.Matrix(Pt))).astype(np.float64)
     ref = oracle.allpairs_sqdist_f64(P, Pt)
     assert np.max(np.abs(out - ref)) <= 1e-5 * max(1.0, np.abs(ref).max())
+
+
+def test_allpairs_general_mixed_width_types(skl):
+    """float rows against double columns at d = 150: the columns no longer fit
+    the 64-column staging (64 x 150 doubles > 64 KB) but 32 rows and 32
+    columns do, which selects the form staging both in LDS."""
+    src = ("double func(lmatrix_t* row, rmatrix_t* col, const unsigned int dim) {"
+           " double s = 0; for (unsigned k = 0; k < dim; ++k) {"
+           " double t = getElementFromRow(row, k) - getElementFromColumn(col, k); s += t*t; }"
+           " return s; }")
+    ap = skl.AllPairs("double(float, double)", src)
+    n, d = 45, 150
+    P = oracle.gen_f32_sym(0x4005, n * d).reshape(n, d)
+    Q = oracle.gen_f32_sym(0x4006, d * 37).reshape(d, 37).astype(np.float64) * 0.5
+    out = np.asarray(ap(skl.Matrix(P), skl.Matrix(Q))).astype(np.float64)
+    ref = ((P.astype(np.float64)[:, :, None] - Q[None, :, :]) ** 2).sum(axis=1)
+    assert np.max(np.abs(out - ref)) <= 1e-12 * max(1.0, np.abs(ref).max())
 
 
 def test_allpairs_general_extra_arg_kat(skl):  # AllPairsTests.cpp:226-269
