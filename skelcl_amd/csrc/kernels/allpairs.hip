@@ -124,7 +124,9 @@ __device__ __forceinline__ SKL_TR getElementFromColumn(rmatrix_t* m, const unsig
 }
 
 #define SKL_APG_B 32
-// LDS-staged: 32 rows of left and 32 columns of right per work-group.
+// LDS-staged: 32 rows of left and 32 columns of right per work-group; taken
+// when 64 columns of right exceed 64 KB but this staging fits, which needs
+// right elements wider than left ones.
 extern "C" __global__ __launch_bounds__(256) void skl_allpairs_general_lds(
     const SKL_TL* A, const SKL_TR* B, SKL_TOUT* C, int M, int N, int K SKL_EXTRA_PARAMS)
 {
@@ -134,52 +136,18 @@ extern "C" __global__ __launch_bounds__(256) void skl_allpairs_general_lds(
                                          ((SKL_APG_B * (size_t)K * sizeof(SKL_TL) + 15) & ~(size_t)15));
   const int tid = threadIdx.x;
   const int m0 = blockIdx.y * SKL_APG_B, n0 = blockIdx.x * SKL_APG_B;
-  // 4-byte elements, K and N multiples of 4, 16-byte aligned operands: the
-  // 32 rows of A are one contiguous run of 32 K elements and B's 32 columns
-  // are 8 packs of 4 per k, so both tiles are copies of 16-byte packs (pack
-  // i of either tile is LDS pack i), 8 per thread in flight at a time.
-  const bool vec = sizeof(SKL_TL) == 4 && sizeof(SKL_TR) == 4 && (K & 3) == 0 &&
-                   (N & 3) == 0 && ((size_t)A & 15) == 0 && ((size_t)B & 15) == 0;
-  if (vec) {
-    typedef unsigned int u4 __attribute__((ext_vector_type(4)));
-    const int nv = 8 * K;                           // packs per tile
-    const long long alim = (long long)(M - m0) * K;  // valid A elements from row m0
-    const u4* ga = reinterpret_cast<const u4*>(A);
-    const u4* gb = reinterpret_cast<const u4*>(B);
-    for (int base = 0; base < nv; base += 256 * 8) {
-      u4 ra[8], rb[8];
-#pragma unroll
-      for (int u = 0; u < 8; ++u) {
-        const int i = base + u * 256 + tid;
-        const bool aok = i < nv && 4LL * i < alim;
-        ra[u] = ga[aok ? ((size_t)m0 * K) / 4 + i : 0];
-        const int c = (i & 7) * 4;
-        const bool bok = i < nv && n0 + c < N;
-        rb[u] = gb[bok ? ((size_t)(i >> 3) * N + n0 + c) / 4 : 0];
-      }
-#pragma unroll
-      for (int u = 0; u < 8; ++u) {
-        const int i = base + u * 256 + tid;
-        if (i < nv) {
-          const u4 z = {0u, 0u, 0u, 0u};
-          reinterpret_cast<u4*>(As)[i] = 4LL * i < alim ? ra[u] : z;
-          reinterpret_cast<u4*>(Bs)[i] = n0 + (i & 7) * 4 < N ? rb[u] : z;
-        }
-      }
-    }
-  } else {
-    for (int e = tid; e < SKL_APG_B * K; e += 256) {
-      const int r = e / K, k = e - r * K;
-      SKL_TL z;
-      __builtin_memset(&z, 0, sizeof(z));
-      As[e] = skl_ld_or(A, (size_t)(m0 + r) * K + k, 0, m0 + r < M, z);
-    }
-    for (int e = tid; e < SKL_APG_B * K; e += 256) {
-      const int k = e / SKL_APG_B, c = e - k * SKL_APG_B;
-      SKL_TR z;
-      __builtin_memset(&z, 0, sizeof(z));
-      Bs[e] = skl_ld_or(B, (size_t)k * N + n0 + c, 0, n0 + c < N, z);
-    }
+  // element loop: the column-staged kernel takes the 4-byte cases
+  for (int e = tid; e < SKL_APG_B * K; e += 256) {
+    const int r = e / K, k = e - r * K;
+    SKL_TL z;
+    __builtin_memset(&z, 0, sizeof(z));
+    As[e] = skl_ld_or(A, (size_t)(m0 + r) * K + k, 0, m0 + r < M, z);
+  }
+  for (int e = tid; e < SKL_APG_B * K; e += 256) {
+    const int k = e / SKL_APG_B, c = e - k * SKL_APG_B;
+    SKL_TR z;
+    __builtin_memset(&z, 0, sizeof(z));
+    Bs[e] = skl_ld_or(B, (size_t)k * N + n0 + c, 0, n0 + c < N, z);
   }
   __syncthreads();
   const int tx = tid & 31, ty = tid >> 5;
