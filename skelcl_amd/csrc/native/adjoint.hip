@@ -30,6 +30,37 @@ __global__ __launch_bounds__(256) void broadcast_kernel(T* __restrict__ dst,
 // A 64x4 work-group computes a 64x16 output tile; d out tile + halo in LDS.
 constexpr int TW = 64, TH = 16;
 
+// d in[p][q] under NEAREST padding for a pixel within R of the border: the
+// clamped views that land on row p (column q) from beyond the border add up
+// (g read from memory; ws = the (2R+1)^2 weights).
+__device__ float nearest_edge_value(const float* __restrict__ g, const float* ws, int p, int q,
+                                    int rows, int cols, int R)
+{
+  const int W = 2 * R + 1;
+  float acc = 0.0f;
+  for (int dy = -R; dy <= R; ++dy) {
+    // rows r in [0, rows) with clamp(r+dy) == p: exactly p - dy, extended
+    // down to 0 when p is the first row (r + dy <= 0) and up to rows - 1
+    // when it is the last (r + dy >= rows - 1); an empty range when ra > rb
+    int ra = p == 0 ? 0 : p - dy;
+    int rb = p == rows - 1 ? rows - 1 : p - dy;
+    if (ra < 0) ra = 0;
+    if (rb > rows - 1) rb = rows - 1;
+    for (int dx = -R; dx <= R; ++dx) {
+      int ca = q == 0 ? 0 : q - dx;
+      int cb = q == cols - 1 ? cols - 1 : q - dx;
+      if (ca < 0) ca = 0;
+      if (cb > cols - 1) cb = cols - 1;
+      if (ra > rb || ca > cb) continue;
+      float s = 0.0f;
+      for (int r = ra; r <= rb; ++r)
+        for (int c = ca; c <= cb; ++c) s += g[(size_t)r * cols + c];
+      acc = fmaf(ws[(dy + R) * W + dx + R], s, acc);
+    }
+  }
+  return acc;
+}
+
 __global__ __launch_bounds__(256) void stencil_transpose_kernel(
     float* __restrict__ din, const float* __restrict__ g, const float* __restrict__ w, int rows,
     int cols, int R, int nearest)
@@ -62,32 +93,104 @@ __global__ __launch_bounds__(256) void stencil_transpose_kernel(
         for (int dx = -R; dx <= R; ++dx)
           acc = fmaf(ws[(dy + R) * W + dx + R], tile[(lr + R - dy) * TWP + (tx + R - dx)], acc);
     } else {
-      for (int dy = -R; dy <= R; ++dy) {
-        // rows r with clamp(r+dy) == p
-        int ra = p - dy, rb = p - dy;
-        if (p == 0) ra = 0;                 // r + dy <= 0
-        if (p == rows - 1) rb = rows - 1;   // r + dy >= rows-1
-        if (ra < 0) ra = 0;
-        if (rb > rows - 1) rb = rows - 1;
-        if (p != 0 && (p - dy < 0)) continue;
-        if (p != rows - 1 && (p - dy > rows - 1)) continue;
-        for (int dx = -R; dx <= R; ++dx) {
-          int ca = q - dx, cb = q - dx;
-          if (q == 0) ca = 0;
-          if (q == cols - 1) cb = cols - 1;
-          if (ca < 0) ca = 0;
-          if (cb > cols - 1) cb = cols - 1;
-          if (q != 0 && (q - dx < 0)) continue;
-          if (q != cols - 1 && (q - dx > cols - 1)) continue;
-          float s = 0.0f;
-          for (int r = ra; r <= rb; ++r)
-            for (int c = ca; c <= cb; ++c) s += g[(size_t)r * cols + c];
-          acc = fmaf(ws[(dy + R) * W + dx + R], s, acc);
-        }
-      }
+      acc = nearest_edge_value(g, ws, p, q, rows, cols, R);
     }
     din[(size_t)p * cols + q] = acc;
   }
+}
+
+// Compile-time R (1..5; at 6 the 169 weights no longer stay in registers): a
+// 64x4 work-group computes a 64x64 output tile, 16
+// rows of one column per thread.  The (2R+1)^2 weights sit in VGPRs (read
+// from LDS once, so they never become spilled SGPRs); each tile row is read
+// once per thread into 2R+1 registers and feeds up to 2R+1 outputs.  Tile
+// rows go in descending order, so every output takes dy ascending and,
+// within it, dx ascending: the generic kernel's order of FMAs, bit for bit.
+// Every pixel gets the interior (zero outside) formula; under NEAREST the
+// border kernel below then rewrites the pixels within R of the border.
+constexpr int FTW = 64, FRB = 16, FTH = 4 * FRB;
+template <int R>
+__global__ __launch_bounds__(256) void stencil_transpose_fast(float* __restrict__ din,
+                                                              const float* __restrict__ g,
+                                                              const float* __restrict__ w,
+                                                              int rows, int cols)
+{
+  constexpr int W = 2 * R + 1, TWP = FTW + 2 * R, THP = FTH + 2 * R;
+  __shared__ float tile[THP * TWP];  // zero outside the matrix
+  __shared__ float ws[W * W];
+  for (int i = threadIdx.x; i < W * W; i += 256) ws[i] = w[i];
+  const int c0 = blockIdx.x * FTW, r0 = blockIdx.y * FTH;
+  for (int idx = threadIdx.x; idx < THP * TWP; idx += 256) {
+    const int tr = idx / TWP, tc = idx - tr * TWP;
+    const int gr = r0 - R + tr, gc = c0 - R + tc;
+    tile[idx] = ld_or(g, (size_t)gr * cols + gc, 0, gr >= 0 && gr < rows && gc >= 0 && gc < cols,
+                      0.0f);
+  }
+  __syncthreads();
+  float wr[W * W];
+#pragma unroll
+  for (int i = 0; i < W * W; ++i) wr[i] = ws[i];
+  const int tx = threadIdx.x & 63, lr0 = (threadIdx.x >> 6) * FRB;
+  float acc[FRB];
+#pragma unroll
+  for (int i = 0; i < FRB; ++i) acc[i] = 0.0f;
+  // output i (row lr0 + i) reads tile row lr0 + i + R - dy = lr0 + t
+#pragma unroll
+  for (int t = FRB + 2 * R - 1; t >= 0; --t) {
+    float v[W];  // v[j]: column tx + R - dx, dx = j - R
+#pragma unroll
+    for (int j = 0; j < W; ++j) v[j] = tile[(lr0 + t) * TWP + tx + 2 * R - j];
+#pragma unroll
+    for (int i = 0; i < FRB; ++i) {
+      const int dy = i + R - t;
+      if (dy < -R || dy > R) continue;
+#pragma unroll
+      for (int j = 0; j < W; ++j) acc[i] = fmaf(wr[(dy + R) * W + j], v[j], acc[i]);
+    }
+    // the accumulators are materialised here: otherwise each output's FMAs
+    // sink into its guarded store at the end and every tile row stays live
+#pragma unroll
+    for (int i = 0; i < FRB; ++i) asm volatile("" : "+v"(acc[i]));
+  }
+  const int q = c0 + tx;
+#pragma unroll
+  for (int i = 0; i < FRB; ++i) {
+    const int p = r0 + lr0 + i;
+    if (p < rows && q < cols) din[(size_t)p * cols + q] = acc[i];
+  }
+}
+
+// NEAREST padding: rewrites the pixels within R of the border (one thread
+// each) after the interior formula has written every pixel.  Bands: top
+// rows, bottom rows, then the left and right columns of the rows between.
+__global__ __launch_bounds__(256) void stencil_transpose_border(float* __restrict__ din,
+                                                                const float* __restrict__ g,
+                                                                const float* __restrict__ w,
+                                                                int rows, int cols, int R)
+{
+  const long long top = R < rows ? R : rows;
+  const long long bstart = rows - R > top ? rows - R : top;  // first bottom-band row
+  const long long bottom = rows - bstart, mid = bstart - top;
+  const long long left = R < cols ? R : cols;
+  const long long rstart = cols - R > left ? cols - R : left;  // first right-band column
+  const long long side = left + (cols - rstart);
+  const long long total = (top + bottom) * cols + mid * side;
+  const long long e = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (e >= total) return;
+  int p, q;
+  if (e < top * cols) {
+    p = (int)(e / cols);
+    q = (int)(e % cols);
+  } else if (e < (top + bottom) * cols) {
+    const long long e2 = e - top * cols;
+    p = (int)(bstart + e2 / cols);
+    q = (int)(e2 % cols);
+  } else {
+    const long long e3 = e - (top + bottom) * cols, k = e3 % side;
+    p = (int)(top + e3 / side);
+    q = (int)(k < left ? k : rstart + (k - left));
+  }
+  din[(size_t)p * cols + q] = nearest_edge_value(g, w, p, q, rows, cols, R);
 }
 
 }  // namespace
@@ -118,6 +221,29 @@ extern "C" int skl_internal_stencil_transpose(float* din, const float* dout, con
                                               hipStream_t s)
 {
   if (R < 0 || R > 32) return SKL_ERR_INVALID;
+  if (R >= 1 && R <= 5) {
+    dim3 fg((cols + FTW - 1) / FTW, (rows + FTH - 1) / FTH);
+    if (fg.y > 65535) return SKL_ERR_INVALID;
+    switch (R) {
+      case 1: stencil_transpose_fast<1><<<fg, 256, 0, s>>>(din, dout, w, rows, cols); break;
+      case 2: stencil_transpose_fast<2><<<fg, 256, 0, s>>>(din, dout, w, rows, cols); break;
+      case 3: stencil_transpose_fast<3><<<fg, 256, 0, s>>>(din, dout, w, rows, cols); break;
+      case 4: stencil_transpose_fast<4><<<fg, 256, 0, s>>>(din, dout, w, rows, cols); break;
+      default: stencil_transpose_fast<5><<<fg, 256, 0, s>>>(din, dout, w, rows, cols); break;
+    }
+    if (nearest) {
+      const long long top = R < rows ? R : rows;
+      const long long bstart = rows - R > top ? rows - R : top;
+      const long long left = R < cols ? R : cols;
+      const long long rstart = cols - R > left ? cols - R : left;
+      const long long total = (top + (rows - bstart)) * cols +
+                              (bstart - top) * (left + (cols - rstart));
+      if (total > 0)
+        stencil_transpose_border<<<(unsigned)((total + 255) / 256), 256, 0, s>>>(din, dout, w,
+                                                                                rows, cols, R);
+    }
+    return hipGetLastError() == hipSuccess ? SKL_OK : SKL_ERR_HIP;
+  }
   const size_t lds = ((size_t)(TH + 2 * R) * (TW + 2 * R) + (2 * R + 1) * (2 * R + 1)) * 4;
   dim3 grid((cols + TW - 1) / TW, (rows + TH - 1) / TH);
   if (grid.y > 65535) return SKL_ERR_INVALID;

@@ -105,6 +105,31 @@ def test_mapoverlap_stencil_backward(skl, padding):
         assert abs(fd - d[r, c]) < 1e-3 * max(1.0, abs(fd))
 
 
+def _stencil_src(R):
+    W = 2 * R + 1
+    return ("float func(input_matrix_t img, __global float* w) { float sum = 0.0f;"
+            f" for (int dy = -{R}; dy <= {R}; ++dy) for (int dx = -{R}; dx <= {R}; ++dx)"
+            f" sum += getData(img, dx, dy) * w[(dy + {R}) * {W} + (dx + {R})]; return sum; }}")
+
+
+@pytest.mark.parametrize("R", [1, 3, 5, 6])
+@pytest.mark.parametrize("shape", [(37, 70), (5, 9), (130, 64)])
+@pytest.mark.parametrize("padding", [0, 1])
+def test_mapoverlap_stencil_backward_radii(skl, R, shape, padding):
+    """Every pixel of the transposed stencil against the FP64 adjoint: radii
+    on the register-tiled form (1..5) and the generic one (6), NEUTRAL and
+    NEAREST, matrices smaller than 2R (overlapping border bands) and ones
+    spanning several 64 x 64 tiles."""
+    rows, cols = shape
+    w = oracle.gaussian_weights(R, 1.5)
+    W = oracle.gen_f32_sym(0x3010 + R, rows * cols).reshape(rows, cols)
+    m = skl.MapOverlap("float(float)", _stencil_src(R), R, padding, 0)
+    d = np.asarray(m.stencil_backward(skl.Matrix(W), skl.Vector(w.ravel()))).astype(np.float64)
+    pix = np.arange(rows * cols, dtype=np.int64)
+    ref = oracle.mapoverlap_stencil_adjoint_f64(W, w, R, pix, padding)
+    assert np.max(np.abs(d.ravel() - ref)) < 1e-5 * max(1.0, np.abs(ref).max())
+
+
 def test_allpairs_backward(skl):
     zip_ = skl.Zip("float(float, float)", "float func(float x, float y){ return x*y; }")
     red = skl.Reduce("float(float)", "float func(float x, float y){ return x+y; }")
