@@ -191,12 +191,13 @@ int skl_zip_reduce_run(skl_skeleton zip, skl_skeleton reduce, void* out,
 
 /* Batched segmented Reduce / Scan (float (+) only): nseg segments of the n
  * elements of `in`, described by device offsets[0..nseg] (int64, segment s
- * is offsets[s]..offsets[s+1]), all processed by one launch after a
- * one-work-group prologue.  Reduce writes out[s] (nseg floats; an empty
- * segment gives 0); Scan writes out[offsets[s] + k] (n floats).  Offsets are
- * validated on the device (non-decreasing, inside [0, n]); invalid offsets
- * make Reduce write NaN to every out[s] and Scan write nothing.  No
- * counterpart in the reference. */
+ * is offsets[s]..offsets[s+1]), processed by small prologue launches and one
+ * main launch (no host synchronisation).  Reduce writes out[s] (nseg floats;
+ * an empty segment gives 0); Scan writes out[offsets[s] + k] (n floats;
+ * elements outside [offsets[0], offsets[nseg]) are left as they are).
+ * Offsets are validated on the device (non-decreasing, inside [0, n]);
+ * invalid offsets make Reduce write NaN to every out[s] and Scan write
+ * nothing.  No counterpart in the reference. */
 int skl_reduce_segmented_run(skl_skeleton s, void* out, const void* in, size_t n,
                              const int64_t* offsets, int nseg,
                              skl_stream stream);
@@ -242,7 +243,8 @@ int skl_matmul_f32(float* C, const float* A, const float* B, size_t M,
                    size_t N, size_t K, int trans_a, int trans_b,
                    int precision, skl_stream stream);
 
-/* Many small AllPairs(+,*) products in one persistent launch.  Device arrays
+/* Many small AllPairs(+,*) products in one launch per kernel class (after a
+ * one-work-group prologue that numbers the tiles).  Device arrays
  * of `count` descriptors (pointers + shapes). */
 typedef struct {
   const float* A;
