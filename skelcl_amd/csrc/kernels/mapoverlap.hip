@@ -86,10 +86,15 @@ extern "C" __global__ __launch_bounds__(256) SKL_MO_ATTR void skl_mapoverlap(
     res[i] = SKL_CALL1(m);
   }
   if (c < cols) {
+    const int rb0 = r0 + ty * SKL_MO_RB;
+    if (rb0 + SKL_MO_RB <= rows) {
+      // every row in range (all but the last row tile): no per-row guard
 #pragma unroll
-    for (int i = 0; i < SKL_MO_RB; ++i) {
-      const int r = r0 + ty * SKL_MO_RB + i;
-      if (r < rows) out_[(size_t)r * cols + c] = res[i];
+      for (int i = 0; i < SKL_MO_RB; ++i) out_[(size_t)(rb0 + i) * cols + c] = res[i];
+    } else {
+#pragma unroll
+      for (int i = 0; i < SKL_MO_RB; ++i)
+        if (rb0 + i < rows) out_[(size_t)(rb0 + i) * cols + c] = res[i];
     }
   }
 }
