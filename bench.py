@@ -12,7 +12,7 @@ generated on the GPU):
 value = total algorithmic bytes / time (GB/s, 1e9 B/s).  The kernels run on
 one stream; HIP events on that stream time every launch.
 
-Prints ONE JSON line.  `--gpus N` with N > 1 exits: this build drives one GPU.
+Prints ONE JSON line.  `--gpus N` with N > 1 exits: the benchmark drives one GPU.
 """
 from __future__ import annotations
 
@@ -352,8 +352,8 @@ def main() -> int:
                     help="also time an identity Map (copy) leg, for counter calibration")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py drives one GPU; --gpus {args.gpus} is not supported by this build "
-              "(the multi-GPU layout is described in DESIGN.md)", file=sys.stderr)
+        print(f"bench.py drives one GPU; --gpus {args.gpus} is not supported: the library "
+              "runs every skeleton on one MI355X", file=sys.stderr)
         return 2
 
     import skelcl_amd as skelcl

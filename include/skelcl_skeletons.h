@@ -56,7 +56,7 @@ enum skl_skeleton_kind {
 
 enum skl_padding { SKL_PADDING_NEUTRAL = 0, SKL_PADDING_NEAREST = 1 };
 
-/* Precision form of the float paths (north-star extension):
+/* Precision form of the float paths (beyond the reference):
  *  DEFAULT  Reduce(+) on float: compensated per-block partials with an
  *           FP64 cross-block combine; Scan(+) on float: FP64 prefixes and
  *           carries, one rounding per output; AllPairs (+,*) on float:

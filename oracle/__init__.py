@@ -1,7 +1,8 @@
 """numpy front-end of the oracle (liboracle.so, built from oracle.c).
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, by ``__graft_entry__.smoke()``
-as the checker and by bench.py's ``cpu_baseline`` leg.  The product
+as the reference its result is compared with, and by bench.py's
+``cpu_baseline`` leg.  The product
 (``skelcl_amd``) never imports it.
 
 Every function here is a thin typed wrapper; the algorithms and their

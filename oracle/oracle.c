@@ -1,7 +1,7 @@
 /*
  * oracle.c -- CPU restatement of SkelCL's skeleton algorithms (the oracle).
  *
- * TEST INFRASTRUCTURE ONLY: used by tests/ as the parity checker and by
+ * TEST INFRASTRUCTURE ONLY: used by tests/ as the parity reference and by
  * bench.py as the CPU baseline.  The product never links it.
  *
  * Built with -ffp-contract=off so every float expression rounds exactly as

@@ -3,7 +3,7 @@
  *
  * TEST INFRASTRUCTURE ONLY.  Nothing in the product (skelcl_amd/, include/)
  * links, loads or calls this code.  It is used by tests/ as the parity
- * checker and by bench.py as the timed CPU baseline ("port"), never as the
+ * reference and by bench.py as the timed CPU baseline ("port"), never as the
  * thing measured on the GPU.
  *
  * Each function names the reference file:line whose behaviour it restates.

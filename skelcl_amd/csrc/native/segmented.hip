@@ -1,4 +1,4 @@
-// Batched segmented Reduce(+) / Scan(+) on float (north-star extension):
+// Batched segmented Reduce(+) / Scan(+) on float (beyond the reference):
 // many uneven segments, given by device offsets, in one main launch after a
 // small prologue launch.
 //

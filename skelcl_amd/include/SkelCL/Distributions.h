@@ -1,7 +1,7 @@
 // skelcl::distribution (Distributions.h:64-311, 365-400).  On one GPU every
 // distribution -- Single, Block, Copy, overlap -- is the whole container on
 // GPU 0, so these calls only record the request; the multi-GPU layouts are
-// not part of this build.
+// not part of this library.
 #pragma once
 
 namespace skelcl {

@@ -72,20 +72,20 @@ inline DeviceID device(size_t id) { return DeviceID{id}; }
 inline void init(DeviceProperties p = allDevices())
 {
   if (p.type() == device_type::CPU || p.type() == device_type::ACCELERATOR)
-    detail::abortWithError("this build runs on the MI355X GPU only");
+    detail::abortWithError("skelcl_amd runs on the MI355X GPU only");
   detail::check(skl_init((int)p.count(), SKL_DEVICE_ANY), "init");
 }
 
 inline void init(PlatformID p, DeviceID d)
 {
   if (p.id != 0 || d.id != 0)
-    detail::abortWithError("only platform 0 / device 0 exist on this build");
+    detail::abortWithError("skelcl_amd exposes only platform 0 / device 0");
   detail::check(skl_init(1, SKL_DEVICE_ANY), "init");
 }
 
 inline void terminate() { skl_terminate(); }
 
-// number of devices after init (DeviceList::size, one on this build)
+// number of devices after init (DeviceList::size: one, the MI355X)
 inline size_t deviceCount()
 {
   int n = 0;

@@ -1,5 +1,5 @@
 """Adjoints, batched segmented Reduce/Scan, batched small products, fused
-Zip->Reduce under graph capture (north-star extensions, no reference
+Zip->Reduce under graph capture (extensions beyond the reference, no reference
 counterpart).  Adjoints are checked against FP64 central finite differences
 and FP64 reference gradients."""
 import ctypes as C
