@@ -138,6 +138,21 @@ def test_allpairs_general_mixed_width_types(skl):
     assert np.max(np.abs(out - ref)) <= 1e-12 * max(1.0, np.abs(ref).max())
 
 
+@pytest.mark.parametrize("n_cols", [36, 70])
+def test_allpairs_general_int_exact(skl, n_cols):
+    """int rows and columns through the column-staged kernel: 16-byte staging
+    copies the 4-byte elements as raw bits (n_cols % 4 == 0), element staging
+    otherwise; the products are exact in int32."""
+    src = ("int func(lmatrix_t* row, rmatrix_t* col, const unsigned int dim) {"
+           " int s = 0; for (unsigned k = 0; k < dim; ++k)"
+           " s += getElementFromRow(row, k) * getElementFromColumn(col, k); return s; }")
+    ap = skl.AllPairs("int(int, int)", src)
+    A = oracle.gen_i32_below(0x4101, 67 * 64, 100).reshape(67, 64).astype(np.int32) - 50
+    B = oracle.gen_i32_below(0x4102, 64 * n_cols, 100).reshape(64, n_cols).astype(np.int32) - 50
+    out = np.asarray(ap(skl.Matrix(A), skl.Matrix(B)))
+    assert np.array_equal(out, (A.astype(np.int64) @ B.astype(np.int64)).astype(np.int32))
+
+
 def test_allpairs_general_extra_arg_kat(skl):  # AllPairsTests.cpp:226-269
     src = ("float func(lmatrix_t* row, rmatrix_t* col, const unsigned int dim, int start) {"
            "float res = 0; for (int i = start; i < dim; ++i) { res += getElementFromRow(row, i) *"
