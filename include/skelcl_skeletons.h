@@ -162,6 +162,20 @@ int skl_zip_run(skl_skeleton s, void* out, const void* left,
  * and commutative, as the reference's strided fold assumes. */
 int skl_reduce_run(skl_skeleton s, void* out, const void* in, size_t n,
                    const skl_arg* args, int nargs, skl_stream stream);
+/* skl_reduce_run without allocation.  No counterpart in the reference: for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * serving loops, harnesses that forbid hipMalloc).  Same kernel, grid and
+ * result as skl_reduce_run; its scratch (completion counters + one partial
+ * per work-group) is a 64 KiB static device array per GPU, held until the
+ * process ends, instead of the skeleton's own.  Each call clears the
+ * counters with a fill kernel on `stream`; calls that overlap on one GPU
+ * race.  Fails with SKL_ERR_INVALID when the launch shape needs more than
+ * 64 KiB (16640 bytes of counters + one partial per work-group: the default
+ * shape needs about 18.7 KB) or when the skeleton's kernel is not built (a
+ * created skeleton always holds it). */
+int skl_reduce_run_noalloc(skl_skeleton s, void* out, const void* in,
+                           size_t n, const skl_arg* args, int nargs,
+                           skl_stream stream);
 /* Scan: exclusive (inclusive = 0; the reference's form, ScanKernel.cl:
  * 142-153) or inclusive prefix under f with the identity; f must be
  * associative.  In-place (out == in) is allowed. */

@@ -126,6 +126,7 @@ SIGNATURES = {
     "skl_map_indexpoint_run": (_i, [_vp, _vp, _sz, _sz, _sz, _argp, _i, _vp]),
     "skl_zip_run": (_i, [_vp, _vp, _vp, _vp, _sz, _argp, _i, _vp]),
     "skl_reduce_run": (_i, [_vp, _vp, _vp, _sz, _argp, _i, _vp]),
+    "skl_reduce_run_noalloc": (_i, [_vp, _vp, _vp, _sz, _argp, _i, _vp]),
     "skl_scan_run": (_i, [_vp, _vp, _vp, _sz, _i, _argp, _i, _vp]),
     "skl_mapoverlap_run": (_i, [_vp, _vp, _vp, _sz, _sz, _argp, _i, _vp]),
     "skl_allpairs_run": (_i, [_vp, _vp, _vp, _vp, _sz, _sz, _sz, _argp, _i, _vp]),

@@ -81,10 +81,7 @@ void apply(skl_skeleton_s* s, const Shape& sh)
   s->scan_j = sh.scan_j;
   s->mo_rb = sh.mo_rb;
   s->max_blocks = cus * sh.blocks_per_cu;
-  if (changed) {
-    for (auto& kv : s->progs) skl_program_release(kv.second.program);
-    s->progs.clear();  // the shape is compiled into the kernels
-  }
+  if (changed) rebuild_programs(s);  // the shape is compiled into the kernels
 }
 
 std::vector<Shape> candidates(const skl_skeleton_s* s)

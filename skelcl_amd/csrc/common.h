@@ -31,6 +31,10 @@ std::string target_arch();
 
 std::string sha1_hex(const std::string& data);
 
+// Looks up the no-allocation scratch of `device` (noalloc.cpp); skl_init
+// calls it with the device current.
+int noalloc_setup(int device);
+
 // Logging level from SKELCL_LOG (0 = errors .. 3 = debug), as
 // pvsutil::Logger's severities (libraries/pvsutil/include/pvsutil/Logger.h).
 int log_level();

@@ -208,6 +208,7 @@ extern "C" int skl_init(int ndevices, int device_type)
   SKL_HIP_TRY(hipSetDevice(g_device));
   SKL_HIP_TRY(hipGetDeviceProperties(&g_props, g_device));
   SKL_HIP_TRY(hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking));
+  SKL_TRY(noalloc_setup(g_device));
   g_init = true;
   logmsg(2, "init: %s (%s), %d CUs", g_props.name, g_props.gcnArchName,
       g_props.multiProcessorCount);

@@ -57,3 +57,16 @@ struct skl_skeleton_s {
   // launch shape defaults per kind (unroll, max_blocks, scan_j, mo_rb)
   void apply_default_shape(int cus);
 };
+
+namespace skl {
+// Drops the generated programs and builds the first one for the current
+// launch shape and options (skeletons.cpp).
+int rebuild_programs(skl_skeleton_s* s);
+// Reduce with caller-provided scratch: no allocation and no build
+// (skeletons.cpp); the layout needs `counter_bytes` of zeroed counters at
+// the start of `total_bytes`.
+int reduce_scratch_need(skl_skeleton_s* s, size_t n, size_t* counter_bytes, size_t* total_bytes);
+int reduce_run_fixed_scratch(skl_skeleton_s* s, void* out, const void* in, size_t n,
+                             const skl_arg* args, int nargs, skl_stream stream, void* scratch,
+                             size_t scratch_bytes);
+}  // namespace skl

@@ -620,6 +620,17 @@ static std::string first_key(const skl_skeleton_s* s)
   return "main";
 }
 
+// Drops every generated program and builds the first one again: a skeleton
+// always holds the kernel of its current launch shape and options.
+int skl::rebuild_programs(skl_skeleton_s* s)
+{
+  for (auto& kv : s->progs) skl_program_release(kv.second.program);
+  s->progs.clear();
+  if (s->kind == SKL_ALLPAIRS_GENERAL) return SKL_OK;  // built per dim on first run
+  skl_program p;
+  return get_program(s, first_key(s), &p);
+}
+
 extern "C" int skl_skeleton_create(const skl_skeleton_desc* desc, skl_skeleton* out)
 {
   SKL_TRY(ensure_initialized());
@@ -672,9 +683,9 @@ extern "C" int skl_skeleton_set_work_group_size(skl_skeleton s, int wg)
 {
   if (!s) return fail(SKL_ERR_INVALID, "skeleton is NULL");
   if (wg <= 0 || wg > 1024) return fail(SKL_ERR_INVALID, "work-group size %d", wg);
+  if (wg == s->wg) return SKL_OK;
   s->wg = wg;
-  s->progs.clear();  // kernels are compiled for the work-group size
-  return SKL_OK;
+  return skl::rebuild_programs(s);  // kernels are compiled for the work-group size
 }
 
 extern "C" int skl_skeleton_set_precision(skl_skeleton s, int precision)
@@ -685,9 +696,8 @@ extern "C" int skl_skeleton_set_precision(skl_skeleton s, int precision)
   if (precision != s->precision) {
     const bool default_j = s->kind == SKL_SCAN && s->scan_j == s->default_scan_j();
     s->precision = precision;
-    for (auto& kv : s->progs) skl_program_release(kv.second.program);
-    s->progs.clear();
     if (default_j) s->scan_j = s->default_scan_j();
+    return skl::rebuild_programs(s);
   }
   return SKL_OK;
 }
@@ -892,26 +902,57 @@ static size_t carry_size(const skl_skeleton_s* s)
   return s->precise_add() ? sizeof(double) : s->types[0].size;
 }
 
-static int reduce_launch(skl_skeleton_s* s, const std::string& key, void* out,
-                         std::initializer_list<const void*> ins, size_t n, const ArgPack& pk,
-                         skl_stream stream, const skl_skeleton_s* zip = nullptr)
+// Reduce scratch: the top completion counter at 0, 64 sub-counters 256 bytes
+// apart, then one carry-sized partial per work-group.
+static constexpr size_t kReduceCounterBytes = 256 * 65;
+
+static int reduce_grid(const skl_skeleton_s* s, size_t n, unsigned* blocks, size_t* scratch)
 {
-  if (n == 0) return fail(SKL_ERR_INVALID, "Reduce of an empty input");
-  if (!out) return fail(SKL_ERR_INVALID, "NULL output buffer");
-  size_t csz = (carry_size(s) + 7) & ~(size_t)7;
-  unsigned long long npk = (n + s->V - 1) / s->V;
   if (s->wg > 64 && s->wg % 64 != 0)
     return fail(SKL_ERR_INVALID, "Reduce work-group size %d: use at most 64 or a multiple of 64",
                 s->wg);
-  unsigned blocks = blocks_for(npk, s->wg, (unsigned long long)s->max_blocks);
-  // scratch: top counter at 0, 64 sub-counters 256 bytes apart, partials
-  constexpr size_t kCounters = 256 * 65;
-  SKL_TRY(ensure_scratch(s, kCounters + (size_t)s->max_blocks * csz));
-  void* partials = static_cast<char*>(s->scratch) + kCounters;
-  void* ticket = s->scratch;
-  unsigned long long nn = n;
+  const size_t csz = (carry_size(s) + 7) & ~(size_t)7;
+  const unsigned long long npk = (n + s->V - 1) / s->V;
+  *blocks = blocks_for(npk, s->wg, (unsigned long long)s->max_blocks);
+  *scratch = kReduceCounterBytes + (size_t)*blocks * csz;
+  return SKL_OK;
+}
+
+// `fixed`: caller-provided scratch of `fixed_bytes` (no allocation, and the
+// kernel must already be built); NULL = the skeleton's own scratch.
+static int reduce_launch(skl_skeleton_s* s, const std::string& key, void* out,
+                         std::initializer_list<const void*> ins, size_t n, const ArgPack& pk,
+                         skl_stream stream, const skl_skeleton_s* zip = nullptr,
+                         void* fixed = nullptr, size_t fixed_bytes = 0)
+{
+  if (n == 0) return fail(SKL_ERR_INVALID, "Reduce of an empty input");
+  if (!out) return fail(SKL_ERR_INVALID, "NULL output buffer");
+  unsigned blocks = 0;
+  size_t need = 0;
+  SKL_TRY(reduce_grid(s, n, &blocks, &need));
+  void* base = fixed;
   skl_kernel k;
-  SKL_TRY(get_kernel(s, key, "skl_reduce", &k, zip));
+  if (fixed) {
+    if (need > fixed_bytes)
+      return fail(SKL_ERR_INVALID,
+                  "Reduce of %zu elements on %u work-groups needs %zu bytes of scratch; the "
+                  "fixed scratch holds %zu",
+                  n, blocks, need, fixed_bytes);
+    auto it = s->progs.find(key);
+    if (it == s->progs.end() || !it->second.program)
+      return fail(SKL_ERR_INVALID,
+                  "the Reduce kernel for this launch shape is not built yet (building loads a "
+                  "code object, which allocates): run skl_reduce_run once first");
+    SKL_TRY(skl_kernel_get(it->second.program, "skl_reduce", &k));
+  } else {
+    const size_t csz = (carry_size(s) + 7) & ~(size_t)7;
+    SKL_TRY(ensure_scratch(s, kReduceCounterBytes + (size_t)s->max_blocks * csz));
+    base = s->scratch;
+    SKL_TRY(get_kernel(s, key, "skl_reduce", &k, zip));
+  }
+  void* partials = static_cast<char*>(base) + kReduceCounterBytes;
+  void* ticket = base;
+  unsigned long long nn = n;
   std::vector<const void*> in(ins);
   std::vector<void*> a;
   for (auto& p : in) a.push_back((void*)&p);
@@ -936,6 +977,32 @@ extern "C" int skl_reduce_run(skl_skeleton s, void* out, const void* in, size_t 
                 s->V * s->types[0].size);
   return reduce_launch(s, "main", out, {in}, n, pk, stream);
 }
+
+namespace skl {
+int reduce_scratch_need(skl_skeleton s, size_t n, size_t* counter_bytes, size_t* total_bytes)
+{
+  SKL_TRY(check_kind(s, SKL_REDUCE));
+  unsigned blocks = 0;
+  SKL_TRY(reduce_grid(s, n ? n : 1, &blocks, total_bytes));
+  *counter_bytes = kReduceCounterBytes;
+  return SKL_OK;
+}
+
+int reduce_run_fixed_scratch(skl_skeleton s, void* out, const void* in, size_t n,
+                             const skl_arg* args, int nargs, skl_stream stream, void* scratch,
+                             size_t scratch_bytes)
+{
+  SKL_TRY(check_kind(s, SKL_REDUCE));
+  ArgPack pk;
+  SKL_TRY(marshal_extras(s->extras, args, nargs, &pk));
+  if (!in) return fail(SKL_ERR_INVALID, "NULL input buffer");
+  if (pk.dyn_lds) return fail(SKL_ERR_UNSUPPORTED, "Reduce user functions cannot take Local");
+  if (!aligned(in, s->V * s->types[0].size))
+    return fail(SKL_ERR_INVALID, "Reduce input must be aligned to %zu bytes",
+                s->V * s->types[0].size);
+  return reduce_launch(s, "main", out, {in}, n, pk, stream, nullptr, scratch, scratch_bytes);
+}
+}  // namespace skl
 
 extern "C" int skl_zip_reduce_run(skl_skeleton zip, skl_skeleton reduce, void* out,
                                   const void* left, const void* right, size_t n,
