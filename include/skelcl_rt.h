@@ -114,11 +114,15 @@ int skl_event_elapsed_ms(skl_event start, skl_event stop, float* ms);
  * skl_program_compile: hiprtc compile of `source` for the target arch
  *   (gfx950 unless SKELCL_ARCH overrides); needs no GPU.  *code is malloc'd
  *   (free with skl_free_string), *log likewise (may be NULL).
- * skl_program_build: compile (or fetch from the in-memory / on-disk code
- *   object cache keyed by SHA1(source | options | arch)) and load on GPU 0.
- *   Cache directory: $SKELCL_CACHE_DIR, else ~/.cache/skelcl_amd; disable
- *   with SKELCL_LOAD_BINARY=NO / SKELCL_SAVE_BINARY=NO (Program.cpp:139,256).
- *   SKELCL_PRINT_BUILD_LOG=YES prints the hiprtc log (Program.cpp:198). */
+ * skl_program_build: compile and load on GPU 0; a program built before in
+ *   this process is shared (in-memory cache).
+ * Both use the on-disk code-object cache only as the reference's binary
+ *   cache does (Program.cpp:139,256): a code object is read when
+ *   SKELCL_LOAD_BINARY=YES and written when SKELCL_SAVE_BINARY=YES, as
+ *   $SKELCL_CACHE_DIR (else ~/.cache/skelcl_amd) / SHA1(source | options |
+ *   arch | hiprtc version).co; a cached object the runtime rejects is
+ *   compiled afresh.  SKELCL_PRINT_BUILD_LOG=YES prints the hiprtc log
+ *   (Program.cpp:198). */
 int skl_program_compile(const char* source, const char* options, char** code,
                         size_t* code_size, char** log);
 int skl_program_build(const char* source, const char* options,

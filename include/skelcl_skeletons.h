@@ -63,7 +63,10 @@ enum skl_padding { SKL_PADDING_NEUTRAL = 0, SKL_PADDING_NEAREST = 1 };
  *           plain FP32.
  *  FP32     plain FP32 everywhere (user function, FP32 accumulation).
  *  BF16X3   AllPairs (+,*) on float: bf16x3 split on MFMA, FP32 accumulate.
- *  FP8      AllPairs (+,*) on float: FP8 e4m3 MFMA, per-128x128-tile scale. */
+ *  FP8      AllPairs (+,*) on float: FP8 e4m3 MFMA, per-128x128-tile scale.
+ * The AllPairs forms apply to float inputs and output with a zero Reduce
+ * identity; a non-zero identity starts each fold (AllPairsKernel.cl:63), so
+ * that contraction runs the in-order FP32 FMA fold at every setting. */
 enum skl_precision {
   SKL_PREC_DEFAULT = 0,
   SKL_PREC_FP32 = 1,

@@ -423,6 +423,10 @@ __global__ __launch_bounds__(256) void quant_fp8_kernel(const float* __restrict_
   }
   int ex = 0;
   if (red[0] > 0.0f) frexpf(red[0] / 448.0f, &ex);  // absmax/448 = m * 2^ex, m in [0.5, 1)
+  // 2^-127 is the smallest E8M0 scale (exponent field 0 of the float
+  // below); tinier tiles flush towards zero in e4m3 instead of making
+  // 1/scale overflow
+  ex = ex < -127 ? -127 : ex;
   const float scale = ldexpf(1.0f, ex);
   const float inv = ldexpf(1.0f, -ex);
   if (threadIdx.x == 0) scales[tr * (Cp / 128) + tc] = scale;

@@ -550,6 +550,61 @@ int rtc_compile(const std::string& src, const char* options,
   hiprtcDestroyProgram(&prog);
   return SKL_OK;
 }
+
+// Code-object cache on disk, as the reference's binary cache: a code object
+// is loaded only when SKELCL_LOAD_BINARY is "YES" and saved only when
+// SKELCL_SAVE_BINARY is "YES" (Program.cpp:139,256), under
+// $SKELCL_CACHE_DIR/<key>.co.  The key covers the source, the options, the
+// target and the hiprtc version, so a toolchain change never loads a stale
+// object.
+std::string program_key(const char* source, const char* options)
+{
+  int major = 0, minor = 0;
+  hiprtcVersion(&major, &minor);
+  return sha1_hex(std::string(source) + "\n//options: " + (options ? options : "") +
+                  "\n//arch: " + target_arch() + "\n//hiprtc: " + std::to_string(major) + "." +
+                  std::to_string(minor) + "\n//rt: 2");
+}
+
+bool cache_load(const std::string& key, std::string* code)
+{
+  if (!env_is("SKELCL_LOAD_BINARY", "YES")) return false;
+  const std::string path = cache_dir() + "/" + key + ".co";
+  std::ifstream f(path, std::ios::binary);
+  if (!f) return false;
+  std::stringstream ss;
+  ss << f.rdbuf();
+  *code = ss.str();
+  if (code->empty()) return false;
+  logmsg(3, "loaded code object %s", path.c_str());
+  return true;
+}
+
+void cache_save(const std::string& key, const std::string& code)
+{
+  if (!env_is("SKELCL_SAVE_BINARY", "YES")) return;
+  mkdirs(cache_dir());
+  const std::string path = cache_dir() + "/" + key + ".co";
+  const std::string tmp = path + ".tmp" + std::to_string(getpid());
+  std::ofstream f(tmp, std::ios::binary);
+  if (!f) return;
+  f.write(code.data(), (std::streamsize)code.size());
+  f.close();
+  if (rename(tmp.c_str(), path.c_str()) != 0) unlink(tmp.c_str());
+  else logmsg(3, "saved code object %s", path.c_str());
+}
+
+// Compiles `source`, or takes it from the disk cache; `from_cache` tells
+// which.
+int compile_cached(const char* source, const char* options, const std::string& key,
+                   std::string* code, std::string* log, bool* from_cache)
+{
+  *from_cache = cache_load(key, code);
+  if (*from_cache) return SKL_OK;
+  SKL_TRY(rtc_compile(source, options, code, log));
+  cache_save(key, *code);
+  return SKL_OK;
+}
 }  // namespace
 
 extern "C" int skl_program_compile(const char* source, const char* options,
@@ -559,7 +614,8 @@ extern "C" int skl_program_compile(const char* source, const char* options,
     return fail(SKL_ERR_INVALID,
                 "Tried to create program with empty user source.");
   std::string c, lg;
-  int st = rtc_compile(source, options, &c, &lg);
+  bool cached = false;
+  int st = compile_cached(source, options, program_key(source, options), &c, &lg, &cached);
   if (log) *log = dup_string(lg);
   if (st != SKL_OK) return st;
   if (code) {
@@ -579,9 +635,7 @@ extern "C" int skl_program_build(const char* source, const char* options,
     return fail(SKL_ERR_INVALID,
                 "Tried to create program with empty user source.");
   if (log) *log = nullptr;
-  std::string key = sha1_hex(std::string(source) + "\n//options: " +
-                             (options ? options : "") + "\n//arch: " +
-                             target_arch() + "\n//rt: 1");
+  const std::string key = program_key(source, options);
   {
     std::lock_guard<std::mutex> lk(g_prog_mutex);
     auto it = g_programs.find(key);
@@ -591,37 +645,28 @@ extern "C" int skl_program_build(const char* source, const char* options,
       return SKL_OK;
     }
   }
-  std::string code;
-  std::string path = cache_dir() + "/" + key + ".co";
-  bool loaded = false;
-  if (!env_is("SKELCL_LOAD_BINARY", "NO")) {
-    std::ifstream f(path, std::ios::binary);
-    if (f) {
-      std::stringstream ss;
-      ss << f.rdbuf();
-      code = ss.str();
-      loaded = !code.empty();
-      if (loaded) logmsg(3, "loaded code object %s", path.c_str());
-    }
-  }
-  if (!loaded) {
-    std::string lg;
-    int st = rtc_compile(source, options, &code, &lg);
-    if (log) *log = dup_string(lg);
-    if (st != SKL_OK) return st;
-    if (!env_is("SKELCL_SAVE_BINARY", "NO")) {
-      mkdirs(cache_dir());
-      std::string tmp = path + ".tmp" + std::to_string(getpid());
-      std::ofstream f(tmp, std::ios::binary);
-      if (f) {
-        f.write(code.data(), (std::streamsize)code.size());
-        f.close();
-        rename(tmp.c_str(), path.c_str());
-      }
-    }
-  }
+  std::string code, lg;
+  bool cached = false;
+  int st = compile_cached(source, options, key, &code, &lg, &cached);
+  if (log) *log = dup_string(lg);
+  if (st != SKL_OK) return st;
   hipModule_t mod;
   hipError_t e = hipModuleLoadData(&mod, code.data());
+  if (e != hipSuccess && cached) {
+    // a cached object this runtime rejects: compile afresh (and replace it)
+    logmsg(1, "cached code object %s rejected (%s): recompiling", key.c_str(),
+           hipGetErrorString(e));
+    code.clear();
+    lg.clear();
+    st = rtc_compile(source, options, &code, &lg);
+    if (log) {
+      if (*log) free(*log);
+      *log = dup_string(lg);
+    }
+    if (st != SKL_OK) return st;
+    cache_save(key, code);
+    e = hipModuleLoadData(&mod, code.data());
+  }
   if (e != hipSuccess)
     return fail(SKL_ERR_HIP, "hipModuleLoadData: %s", hipGetErrorString(e));
   auto* p = new skl_program_s;

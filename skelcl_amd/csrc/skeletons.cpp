@@ -3,6 +3,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cctype>
+#include <cstdlib>
 #include <cstring>
 #include <deque>
 #include <sstream>
@@ -612,12 +614,32 @@ static int setup_skeleton(const skl_skeleton_desc* d, skl_skeleton_s* s)
   return SKL_OK;
 }
 
+// AllPairs(+, *) with float output AND float inputs: the FMA form (and the
+// native products when the identity is zero).  Other input types take the
+// user functions' own fold.
+static bool float_addmul(const skl_skeleton_s* s)
+{
+  return s->kind == SKL_ALLPAIRS && s->op == UserOp::Add && s->op2 == UserOp::Mul &&
+         s->extras.empty() && s->types[0].name == "float" && s->types[1].name == "float" &&
+         s->types[2].name == "float";
+}
+
+// True when the identity text is a zero literal ("0", "0.0f", "-0.", "0e0").
+static bool identity_is_zero(const std::string& id)
+{
+  std::string t;
+  for (char c : id)
+    if (!isspace((unsigned char)c)) t += c;
+  if (!t.empty() && (t.back() == 'f' || t.back() == 'F')) t.pop_back();
+  if (t.empty()) return false;
+  char* end = nullptr;
+  const double v = strtod(t.c_str(), &end);
+  return end && *end == '\0' && v == 0.0;
+}
+
 static std::string first_key(const skl_skeleton_s* s)
 {
-  if (s->kind == SKL_ALLPAIRS && s->op == UserOp::Add && s->op2 == UserOp::Mul &&
-      s->extras.empty() && s->types[0].name == "float")
-    return "fma";
-  return "main";
+  return float_addmul(s) ? "fma" : "main";
 }
 
 // Drops every generated program and builds the first one again: a skeleton
@@ -1022,8 +1044,12 @@ extern "C" int skl_zip_reduce_run(skl_skeleton zip, skl_skeleton reduce, void* o
   if (!aligned(left, al) || !aligned(right, al))
     return fail(SKL_ERR_INVALID, "inputs must be aligned to %zu bytes", al);
   ArgPack pk;
-  char key[64];
-  snprintf(key, sizeof key, "fused:%p", (const void*)zip);
+  // the fused program is keyed on what the Zip contributes to its source,
+  // never on the Zip object (a released Zip's address can come back)
+  std::ostringstream z;
+  z << zip->src << "\n//func " << zip->func << "\n//options " << zip->options;
+  for (auto& t : zip->types) z << "\n//type " << t.name << " " << t.size;
+  const std::string key = "fused:" + sha1_hex(z.str());
   return reduce_launch(reduce, key, out, {left, right}, n, pk, stream, zip);
 }
 
@@ -1154,11 +1180,10 @@ extern "C" int skl_allpairs_run(skl_skeleton s, void* out, const void* left, con
   if (s->kind == SKL_ALLPAIRS) {
     std::string key = first_key(s);
     // the (+,*) contraction on float from a zero identity is a matmul: the
-    // native kernels (FP32 VALU with 8x8 register blocks, or the MFMA forms)
-    const bool zero_id = s->identity == "0" || s->identity == "0.0" || s->identity == "0.0f" ||
-                         s->identity == "0.f" || s->identity == "0.";
-    if (key == "fma" && (zero_id || (s->precision != SKL_PREC_DEFAULT &&
-                                     s->precision != SKL_PREC_FP32)))
+    // native kernels (FP32 VALU with 8x8 register blocks, or the MFMA forms).
+    // A non-zero identity starts the fold (AllPairsKernel.cl:63), so it
+    // takes the in-order FMA fold at every precision.
+    if (key == "fma" && identity_is_zero(s->identity))
       return skl_matmul_f32((float*)out, (const float*)left, (const float*)right, rows, cols,
                             dim, 0, 0, s->precision, stream);
     if (pk.dyn_lds) return fail(SKL_ERR_UNSUPPORTED, "AllPairs user functions cannot take Local");

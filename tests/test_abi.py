@@ -132,3 +132,45 @@ def test_segment_offsets_checked_on_host():
     for bad, n in (([0, 5, 4], 10), ([0, 11], 10), ([-1, 2], 10), ([], 10)):
         with pytest.raises(SkelError):
             skl._segment_offsets(bad, n)
+
+
+def _compile(src):
+    code, size, log = C.c_void_p(), C.c_size_t(), C.c_void_p()
+    _lib.check(_lib.lib().skl_program_compile(src, None, C.byref(code), C.byref(size),
+                                              C.byref(log)))
+    blob = C.string_at(code.value, size.value)
+    _lib.lib().skl_free_string(code)
+    _lib.take_string(log)
+    return blob
+
+
+def test_code_object_cache_is_opt_in(tmp_path, monkeypatch):
+    """SKELCL_SAVE_BINARY / SKELCL_LOAD_BINARY act only when they are "YES"
+    (src/Program.cpp:139,256): nothing is written or read otherwise; with
+    SAVE=YES one code object is written under SKELCL_CACHE_DIR; with
+    LOAD=YES a compile returns that file's bytes."""
+    src = b'extern "C" __global__ void k(int* o){ o[threadIdx.x] = 7; }'
+    monkeypatch.setenv("SKELCL_CACHE_DIR", str(tmp_path))
+    for v in (None, "NO", "yes", "1"):
+        for name in ("SKELCL_SAVE_BINARY", "SKELCL_LOAD_BINARY"):
+            if v is None:
+                monkeypatch.delenv(name, raising=False)
+            else:
+                monkeypatch.setenv(name, v)
+        _compile(src)
+        assert list(tmp_path.iterdir()) == [], v
+    monkeypatch.setenv("SKELCL_SAVE_BINARY", "YES")
+    monkeypatch.delenv("SKELCL_LOAD_BINARY")
+    real = _compile(src)
+    files = list(tmp_path.iterdir())
+    assert len(files) == 1 and files[0].suffix == ".co"
+    assert files[0].read_bytes() == real
+    # plant a marker: only LOAD=YES returns it
+    files[0].write_bytes(b"\x7fELF-cached-marker")
+    monkeypatch.delenv("SKELCL_SAVE_BINARY")
+    assert _compile(src) == real
+    monkeypatch.setenv("SKELCL_LOAD_BINARY", "YES")
+    assert _compile(src) == b"\x7fELF-cached-marker"
+    # another source has another key: compiled, not loaded
+    other = _compile(src.replace(b"7", b"8"))
+    assert other.startswith(b"\x7fELF") and other != b"\x7fELF-cached-marker"

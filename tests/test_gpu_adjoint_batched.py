@@ -468,3 +468,23 @@ def test_autotune_keeps_results_and_caches(skl, tmp_path, monkeypatch):
     a, b = oracle.gen_f32_sym(1, 12345), oracle.gen_f32_sym(2, 12345)
     assert np.array_equal(np.asarray(m(skl.Vector(a), skl.Vector(b))), a * b)
     _lib.check(L.skl_autotune(m.handle, 0, 0))  # back to defaults
+
+
+def test_fused_reduce_follows_a_new_zip(skl):
+    """One Reduce fused with a Zip, the Zip released, then a Zip with another
+    function (possibly at the released one's address): the fused program
+    belongs to the Zip's function, so each result matches its own function."""
+    import gc
+    n = 100003
+    x, y = oracle.gen_f32_sym(0x1001, n), oracle.gen_f32_sym(0x1002, n)
+    X, Y = skl.Vector(x), skl.Vector(y)
+    r = skl.Reduce("float(float)", "float func(float x, float y){ return x+y; }")
+    exact = {"x*y": (x.astype(np.float64) * y).sum(), "x-y": (x.astype(np.float64) - y).sum(),
+             "x+2*y": (x.astype(np.float64) + 2 * y.astype(np.float64)).sum()}
+    for _ in range(2):
+        for body, want in exact.items():
+            z = skl.Zip("float(float, float)", f"float func(float x, float y){{ return {body}; }}")
+            got = float(r.fused(z, X, Y)[0])
+            del z
+            gc.collect()
+            assert abs(got - want) <= 1e-5 * max(1.0, np.abs(x).sum()), body

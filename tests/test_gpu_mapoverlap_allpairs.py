@@ -256,3 +256,43 @@ def test_allpairs_skeleton_precision_dispatch(skl):
         ap.set_precision(p)
         out = np.asarray(ap(skl.Matrix(A), skl.Matrix(B))).astype(np.float64)
         assert np.linalg.norm(out - ref) / np.linalg.norm(ref) < tol, p
+
+
+def test_allpairs_addmul_int_inputs_take_the_fold(skl):
+    """AllPairs<float(int, int)> with (+, *): the inputs are ints, so the
+    float products must not run (they would read int bits as floats); the
+    user functions' own fold gives the exact integer sums."""
+    zip_ = skl.Zip("float(int, int)", "float func(int x, int y){ return x*y; }")
+    red = skl.Reduce("float(float)", "float func(float x, float y){ return x+y; }")
+    ap = skl.AllPairs("float(int, int)", reduce=red, zip=zip_)
+    A = oracle.gen_i32_below(0x4201, 70 * 90, 30).reshape(70, 90).astype(np.int32) - 15
+    B = oracle.gen_i32_below(0x4202, 90 * 50, 30).reshape(90, 50).astype(np.int32) - 15
+    out = np.asarray(ap(skl.Matrix(A), skl.Matrix(B)))
+    assert np.array_equal(out, (A.astype(np.int64) @ B.astype(np.int64)).astype(np.float32))
+
+
+@pytest.mark.parametrize("prec", [0, 1, 2, 3])
+def test_allpairs_nonzero_identity_starts_the_fold(skl, prec):
+    """A Reduce identity of 1 starts every fold (AllPairsKernel.cl:63), at
+    every precision setting: integer-valued inputs make the in-order fold
+    exact, so the result is 1 + A.B exactly."""
+    zip_ = skl.Zip("float(float, float)", "float func(float x, float y){ return x*y; }")
+    red = skl.Reduce("float(float)", "float func(float x, float y){ return x+y; }", "1")
+    ap = skl.AllPairs("float(float, float)", reduce=red, zip=zip_)
+    ap.set_precision(prec)
+    A, B = _ints(41, (96, 130)), _ints(42, (130, 70))
+    out = np.asarray(ap(skl.Matrix(A), skl.Matrix(B)))
+    assert np.array_equal(out, (1.0 + A.astype(np.float64) @ B).astype(np.float32))
+
+
+def test_matmul_fp8_tiny_tile_stays_finite(skl):
+    """A 128x128 tile of values near 1e-38 (below 448 * 2^-128) gets the
+    smallest E8M0 scale; its products flush towards zero, never NaN/inf."""
+    M = N = K = 256
+    A = oracle.gen_f32_sym(0x4301, M * K).reshape(M, K)
+    B = oracle.gen_f32_sym(0x4302, K * N).reshape(K, N)
+    A[:128, :128] *= np.float32(1e-38)
+    c = _matmul(skl, A, B, skl.PREC_FP8)
+    assert np.isfinite(c).all()
+    ref = oracle.allpairs_mm_f64(A, B)
+    assert np.linalg.norm(c - ref) / np.linalg.norm(ref) < 5e-2
