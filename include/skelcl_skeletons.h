@@ -116,6 +116,17 @@ typedef struct {
   unsigned int cols;
 } skl_arg;
 
+/* CommonDefinitions (src/Source.cpp:102-142, Macros.h:55-61): source text
+ * that every skeleton created afterwards places ahead of its element
+ * typedefs and user function -- the types, functions and macros of
+ * SKELCL_COMMON_DEFINITION / SKELCL_ADD_DEFINE.  level: 0 pragmas, 1 user
+ * definitions, 2 generated definitions; the text is emitted level by level in
+ * the order appended.  Needs no GPU and no skl_init (static initialisers
+ * register definitions before main). */
+int skl_common_definition_append(const char* text, int level);
+/* all appended text, level by level; free with skl_free_string */
+int skl_common_definitions(char** text);
+
 /* Skeleton construction: parses the user function, generates the fused
  * kernel source and builds it (hiprtc, cached).  Mirrors the reference
  * constructors' createAndBuildProgram (e.g. MapDef.h:168-214). */

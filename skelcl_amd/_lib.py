@@ -110,6 +110,8 @@ SIGNATURES = {
     "skl_generate_f32_unit": (_i, [_vp, _sz, C.c_uint64, C.c_float, _vp]),
     "skl_generate_i32_below": (_i, [_vp, _sz, C.c_uint64, C.c_uint32, _vp]),
     # skeletons
+    "skl_common_definition_append": (_i, [C.c_char_p, _i]),
+    "skl_common_definitions": (_i, [C.POINTER(_vp)]),
     "skl_skeleton_create": (_i, [C.POINTER(skl_skeleton_desc), C.POINTER(_vp)]),
     "skl_skeleton_release": (_i, [_vp]),
     "skl_skeleton_set_work_group_size": (_i, [_vp, _i]),

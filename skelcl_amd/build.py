@@ -94,12 +94,13 @@ def build(verbose: bool = False) -> str:
 
 
 def build_cpp_tests(verbose: bool = False) -> str:
-    """Compile tests/cpp (the reference's skeleton tests restated against the
-    C++ API) with g++ against libskelcl_amd.so; host code only."""
-    src = os.path.join(ROOT, "tests", "cpp", "skeleton_tests.cpp")
+    """Compile tests/cpp (the reference's skeleton, container and device
+    tests restated against the C++ API) with g++ against libskelcl_amd.so;
+    host code only."""
+    srcs = [os.path.join(ROOT, "tests", "cpp", f) for f in ("skeleton_tests.cpp", "api_tests.cpp")]
     out = os.path.join(ROOT, "tests", "cpp", "skelcl_cpp_tests")
     cmd = ["g++", "-std=c++17", "-O1", "-Wall", "-I", os.path.join(ROOT, "include"),
-           "-I", os.path.join(HERE, "include"), "-o", out, src, "-L", HERE, "-lskelcl_amd",
+           "-I", os.path.join(HERE, "include"), "-o", out, *srcs, "-L", HERE, "-lskelcl_amd",
            "-Wl,-rpath,$ORIGIN/../../skelcl_amd"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
@@ -109,7 +110,57 @@ def build_cpp_tests(verbose: bool = False) -> str:
     return out
 
 
+# The reference's example programs (examples/*/main.cpp), compiled unchanged
+# against this package's headers and library -- the drop-in check on the
+# reference's own callers.  They are read where they lie (never copied) with
+# the reference's pvsutil helper library (libraries/pvsutil/src/*.cpp, 8
+# plain C++ files) for their logging and command-line parsing; the binaries
+# go to tests/cpp/_ref_examples/ (git-ignored).  Nothing is built when the
+# reference tree is absent.
+REFERENCE = os.environ.get("SKELCL_REFERENCE", "/root/reference")
+REF_EXAMPLES = ("dot_product", "saxpy", "matrix_mult", "montecarlo", "gauss", "mandelbrot")
+REF_EXAMPLE_DIR = os.path.join(ROOT, "tests", "cpp", "_ref_examples")
+
+
+def build_reference_examples(verbose: bool = False) -> list[str]:
+    ex_dir = os.path.join(REFERENCE, "examples")
+    pvs = os.path.join(REFERENCE, "libraries", "pvsutil")
+    if not (os.path.isdir(ex_dir) and os.path.isdir(pvs)):
+        if verbose:
+            print(f"reference examples not built: {REFERENCE} is absent")
+        return []
+    objdir = os.path.join(ROOT, "_build", "pvsutil")
+    os.makedirs(objdir, exist_ok=True)
+    os.makedirs(REF_EXAMPLE_DIR, exist_ok=True)
+    inc_pvs = os.path.join(pvs, "include")
+    srcs = sorted(f for f in os.listdir(os.path.join(pvs, "src")) if f.endswith(".cpp"))
+
+    def run(cmd):
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+
+    objs = []
+    for f in srcs:
+        o = os.path.join(objdir, f[:-4] + ".o")
+        run(["g++", "-std=c++11", "-O1", "-c", "-I", inc_pvs, os.path.join(pvs, "src", f),
+             "-o", o])
+        objs.append(o)
+    built = []
+    for ex in REF_EXAMPLES:
+        out = os.path.join(REF_EXAMPLE_DIR, ex)
+        run(["g++", "-std=c++11", "-O1", "-I", os.path.join(ROOT, "include"),
+             "-I", os.path.join(HERE, "include"), "-I", inc_pvs,
+             os.path.join(ex_dir, ex, "main.cpp"), *objs, "-L", HERE, "-lskelcl_amd",
+             "-Wl,-rpath,$ORIGIN/../../../skelcl_amd", "-o", out])
+        built.append(out)
+    if verbose:
+        print(f"built {len(built)} reference examples in {REF_EXAMPLE_DIR}")
+    return built
+
+
 if __name__ == "__main__":
     build(verbose=True)
     build_cpp_tests(verbose=True)
+    build_reference_examples(verbose=True)
     sys.exit(0)

@@ -243,6 +243,21 @@ SKL_MATRIX_T(skl_float4, float4)
 SKL_MATRIX_T(skl_int4, int4)
 SKL_MATRIX_T(skl_double4, double4)
 #undef SKL_MATRIX_T
+// the names above are taken (Matrix<T>::deviceFunctions guards on these)
+#define char_MATRIX_T
+#define uchar_MATRIX_T
+#define short_MATRIX_T
+#define ushort_MATRIX_T
+#define int_MATRIX_T
+#define uint_MATRIX_T
+#define long_MATRIX_T
+#define ulong_MATRIX_T
+#define float_MATRIX_T
+#define double_MATRIX_T
+#define float2_MATRIX_T
+#define float4_MATRIX_T
+#define int4_MATRIX_T
+#define double4_MATRIX_T
 #define get(m, y, x) (m).data[(int)((y) * (m).col_count + (x))]
 #define set(m, y, x, v) (m).data[(int)((y) * (m).col_count + (x))] = (v)
 

@@ -18,6 +18,8 @@ struct skl_skeleton_s {
   int kind = 0;
   std::string src, func, src2, func2, identity, neutral;
   std::string deriv_src, deriv_func, deriv_func2, options;
+  std::string common;  // common definitions registered when it was created
+  std::string contract = "on";  // -ffp-contract of the user functions
   std::vector<TypeInfo> types;  // kind-specific order, see skelcl_skeletons.h
   int overlap = 1, padding = SKL_PADDING_NEAREST;
   int wg = 256;

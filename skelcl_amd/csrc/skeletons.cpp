@@ -7,6 +7,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <deque>
+#include <mutex>
 #include <sstream>
 
 #include "common.h"
@@ -23,6 +24,47 @@ extern "C" const char* skl_tmpl_mapoverlap;
 extern "C" const char* skl_tmpl_allpairs;
 
 using namespace skl;
+
+// ---- common definitions (CommonDefinitions, src/Source.cpp:102-142) --------
+namespace {
+struct CommonDefs {
+  std::mutex mu;
+  std::vector<std::string> level[3];
+};
+CommonDefs& common_defs()
+{
+  static CommonDefs d;  // registered from static initialisers: built on first use
+  return d;
+}
+std::string common_text()
+{
+  CommonDefs& d = common_defs();
+  std::lock_guard<std::mutex> lk(d.mu);
+  std::string t;
+  for (auto& lv : d.level)
+    for (auto& x : lv) t += x + "\n";
+  return t;
+}
+}  // namespace
+
+extern "C" int skl_common_definition_append(const char* text, int level)
+{
+  if (!text) return fail(SKL_ERR_INVALID, "common definition is NULL");
+  if (level < 0 || level > 2) return fail(SKL_ERR_INVALID, "unknown definition level %d", level);
+  CommonDefs& d = common_defs();
+  std::lock_guard<std::mutex> lk(d.mu);
+  d.level[level].push_back(text);
+  return SKL_OK;
+}
+
+extern "C" int skl_common_definitions(char** text)
+{
+  if (!text) return fail(SKL_ERR_INVALID, "text is NULL");
+  const std::string t = common_text();
+  *text = (char*)malloc(t.size() + 1);
+  memcpy(*text, t.c_str(), t.size() + 1);
+  return SKL_OK;
+}
 
 bool skl_skeleton_s::precise_add() const
 {
@@ -211,6 +253,11 @@ struct Gen {
   {
     o << skl_tmpl_prelude << "\n";
     o << "#define SKL_WG " << s->wg << "\n";
+    // the common definitions the skeleton was created with, ahead of the
+    // element typedefs that may name their types (Source.cpp:118-126)
+    if (!s->common.empty())
+      o << "#pragma clang force_cuda_host_device begin\n#line 1 \"common_definitions\"\n"
+        << s->common << "\n#pragma clang force_cuda_host_device end\n";
   }
   void typedefs(const std::vector<std::string>& scl)
   {
@@ -457,7 +504,7 @@ static int generate(const skl_skeleton_s* s, const std::string& key, std::string
 
 static std::string build_options(const skl_skeleton_s* s)
 {
-  std::string o = "-ffp-contract=on";
+  std::string o = "-ffp-contract=" + s->contract;
   if (!s->options.empty()) o += " " + s->options;
   return o;
 }
@@ -533,6 +580,7 @@ static int setup_skeleton(const skl_skeleton_desc* d, skl_skeleton_s* s)
   if (!d->source || !*d->source)
     return fail(SKL_ERR_INVALID, "Tried to create program with empty user source.");
   s->src = d->source;
+  s->common = common_text();
   s->func = str_or(d->func, "func");
   s->identity = str_or(d->identity, "0");
   s->neutral = str_or(d->neutral, "0");
@@ -550,6 +598,23 @@ static int setup_skeleton(const skl_skeleton_desc* d, skl_skeleton_s* s)
       return fail(SKL_ERR_INVALID, "unknown padding %d", s->padding);
   }
   if (d->build_options) s->options = d->build_options;
+  // floating-point contraction of the user functions: SKELCL_FP_CONTRACT
+  // (on | off | fast; default on), a build option's own -ffp-contract wins
+  s->contract = "on";
+  if (const char* e = getenv("SKELCL_FP_CONTRACT")) {
+    const std::string v = e;
+    if (v != "on" && v != "off" && v != "fast")
+      return fail(SKL_ERR_INVALID, "SKELCL_FP_CONTRACT=%s: use on, off or fast", e);
+    s->contract = v;
+  }
+  {
+    const size_t at = s->options.rfind("-ffp-contract=");
+    if (at != std::string::npos) {
+      std::string v = s->options.substr(at + 14);
+      v = v.substr(0, v.find_first_of(" \t"));
+      if (v == "on" || v == "off" || v == "fast") s->contract = v;
+    }
+  }
   int nt = expected_types(s->kind);
   if (d->ntypes != nt)
     return fail(SKL_ERR_INVALID, "%s needs %d type(s), got %d", kind_name(s->kind), nt,
@@ -637,9 +702,12 @@ static bool identity_is_zero(const std::string& id)
   return end && *end == '\0' && v == 0.0;
 }
 
+// Without contraction (the parity setting) AllPairs(+, *) on float runs the
+// user functions' own fold in k order, two roundings per step, which is what
+// a sequential host loop computes (AllPairsKernel.cl:89-91).
 static std::string first_key(const skl_skeleton_s* s)
 {
-  return float_addmul(s) ? "fma" : "main";
+  return float_addmul(s) && s->contract != "off" ? "fma" : "main";
 }
 
 // Drops every generated program and builds the first one again: a skeleton
@@ -1183,7 +1251,8 @@ extern "C" int skl_allpairs_run(skl_skeleton s, void* out, const void* left, con
     // native kernels (FP32 VALU with 8x8 register blocks, or the MFMA forms).
     // A non-zero identity starts the fold (AllPairsKernel.cl:63), so it
     // takes the in-order FMA fold at every precision.
-    if (key == "fma" && identity_is_zero(s->identity))
+    const bool product_form = s->precision == SKL_PREC_BF16X3 || s->precision == SKL_PREC_FP8;
+    if ((key == "fma" || (float_addmul(s) && product_form)) && identity_is_zero(s->identity))
       return skl_matmul_f32((float*)out, (const float*)left, (const float*)right, rows, cols,
                             dim, 0, 0, s->precision, stream);
     if (pk.dyn_lds) return fail(SKL_ERR_UNSUPPORTED, "AllPairs user functions cannot take Local");

@@ -56,6 +56,7 @@ public:
   {
     if (left.columnCount() != right.rowCount())  // AllPairsDef.h:127-129
       detail::abortWithError("AllPairs: left.columnCount() != right.rowCount()");
+    detail::allPairsDistributions(left, right);
     auto L = detail::makeArgs(std::forward<Args>(args)...);
     const void* lp = left.devicePointer();
     const void* rp = right.devicePointer();
@@ -66,6 +67,7 @@ public:
                                    nullptr),
                   "AllPairs");
     o.dataOnDeviceModified();
+    detail::adoptDistribution(o, left);
     L.finish();
     return o;
   }

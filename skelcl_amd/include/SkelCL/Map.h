@@ -66,6 +66,7 @@ public:
   template <typename... Args>
   Vector<Tout>& operator()(Out<Vector<Tout>> output, const Vector<Tin>& in, Args&&... args)
   {
+    detail::defaultDistribution<detail::BlockDistribution>(in);
     auto L = detail::makeArgs(std::forward<Args>(args)...);
     const void* src = in.devicePointer();
     Vector<Tout>& o = output.container();
@@ -76,6 +77,7 @@ public:
                     : (o.prepareOutput(in.size()), o.deviceOutputPointer());
     detail::check(skl_map_run(_h, dst, src, in.size(), L.data(), L.count(), nullptr), "Map");
     o.dataOnDeviceModified();
+    detail::adoptDistribution(o, in);
     L.finish();
     return o;
   }
@@ -89,6 +91,7 @@ public:
   template <typename... Args>
   Matrix<Tout>& operator()(Out<Matrix<Tout>> output, const Matrix<Tin>& in, Args&&... args)
   {
+    detail::defaultDistribution<detail::BlockDistribution>(in);
     auto L = detail::makeArgs(std::forward<Args>(args)...);
     const void* src = in.devicePointer();
     Matrix<Tout>& o = output.container();
@@ -97,6 +100,7 @@ public:
                               L.count(), nullptr),
                   "Map");
     o.dataOnDeviceModified();
+    detail::adoptDistribution(o, in);
     L.finish();
     return o;
   }
@@ -123,6 +127,7 @@ public:
   template <typename C, typename... Args>
   void operator()(const C& in, Args&&... args)
   {
+    detail::defaultDistribution<detail::BlockDistribution>(in);
     auto L = detail::makeArgs(std::forward<Args>(args)...);
     size_t n = elementCount(in);
     detail::check(skl_map_run(_h, nullptr, in.devicePointer(), n, L.data(), L.count(), nullptr),

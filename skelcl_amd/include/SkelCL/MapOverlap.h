@@ -21,7 +21,7 @@ public:
              detail::Padding padding = detail::Padding::NEAREST, Tin neutral_element = Tin(),
              const std::string& funcName = "func")
       : _src(source), _func(funcName), _range(overlap_range), _padding(padding),
-        _neutral(detail::valueString(neutral_element))
+        _neutral(detail::valueString(neutral_element)), _neutralValue(neutral_element)
   {
     skl_skeleton_desc d = blankDesc(SKL_MAPOVERLAP);
     d.source = _src.c_str();
@@ -47,6 +47,8 @@ public:
   {
     if (in.rowCount() == 0 || in.columnCount() == 0)
       detail::abortWithError("MapOverlap: empty matrix");
+    // MapOverlapDef.h:250-262: the input always gets this skeleton's overlap
+    in.setDistribution(detail::OLDistribution<Matrix<Tin>>(_range, _padding, _neutralValue));
     auto L = detail::makeArgs(std::forward<Args>(args)...);
     const void* src = in.devicePointer();
     Matrix<Tout>& o = output.container();
@@ -55,6 +57,7 @@ public:
                                      in.columnCount(), L.data(), L.count(), nullptr),
                   "MapOverlap");
     o.dataOnDeviceModified();
+    detail::adoptDistribution(o, in);
     L.finish();
     return o;
   }
@@ -81,6 +84,7 @@ private:
   unsigned int _range;
   detail::Padding _padding;
   std::string _neutral;
+  Tin _neutralValue;
 };
 
 }  // namespace skelcl

@@ -38,6 +38,7 @@ public:
   template <typename... Args>
   Vector<T>& operator()(Out<Vector<T>> output, const Vector<T>& in, Args&&... args)
   {
+    detail::defaultDistribution<detail::SingleDistribution>(in);
     auto L = detail::makeArgs(std::forward<Args>(args)...);
     const void* src = in.devicePointer();
     Vector<T>& o = output.container();
@@ -45,6 +46,7 @@ public:
     detail::check(skl_reduce_run(_h, dst, src, in.size(), L.data(), L.count(), nullptr),
                   "Reduce");
     o.dataOnDeviceModified();
+    detail::adoptDistribution(o, in);
     L.finish();
     return o;
   }

@@ -83,6 +83,7 @@ private:
   template <typename... Args>
   Vector<T>& run(Vector<T>& o, const Vector<T>& in, int incl, Args&&... args)
   {
+    detail::defaultDistribution<detail::SingleDistribution>(in);
     auto L = detail::makeArgs(std::forward<Args>(args)...);
     const void* src = in.devicePointer();
     void* dst = (static_cast<const void*>(&o) == static_cast<const void*>(&in))
@@ -91,6 +92,7 @@ private:
     detail::check(skl_scan_run(_h, dst, src, in.size(), incl, L.data(), L.count(), nullptr),
                   "Scan");
     o.dataOnDeviceModified();
+    detail::adoptDistribution(o, in);
     L.finish();
     return o;
   }

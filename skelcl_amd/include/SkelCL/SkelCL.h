@@ -1,17 +1,24 @@
 // skelcl_amd C++ API -- drop-in for SkelCL's public headers
 // (include/SkelCL/SkelCL.h:137-207, Vector.h, Matrix.h, Map.h, Zip.h,
 // Reduce.h, Scan.h, MapOverlap.h, AllPairs.h, Out.h, Local.h, Source.h,
-// Index*.h, detail/Types.h, Distributions.h).  Host code written against
-// SkelCL compiles unchanged; underneath, every call goes through the C-ABI
-// of libskelcl_amd.so (include/skelcl_rt.h, include/skelcl_skeletons.h) to
+// Index*.h, detail/Types.h, Distributions.h, detail/Device*.h,
+// detail/Util.h, detail/Macros.h).  Host code written against SkelCL
+// compiles unchanged; underneath, every call goes through the C-ABI of
+// libskelcl_amd.so (include/skelcl_rt.h, include/skelcl_skeletons.h) to
 // hand-written gfx950 kernels.  One GPU: device selection collapses to GPU 0
 // and every distribution is the whole container on it.
 #pragma once
 
+#include <cmath>
 #include <cstddef>
 #include <string>
 
 #include "detail/Runtime.h"
+#include "detail/Device.h"
+#include "detail/DeviceList.h"
+#include "detail/DeviceProperties.h"
+#include "detail/Macros.h"
+#include "detail/Util.h"
 #include "Types.h"
 #include "Source.h"
 #include "Local.h"
@@ -31,66 +38,34 @@
 
 namespace skelcl {
 
-namespace device_type {
-enum DeviceType { ANY = 0, CPU, GPU, ACCELERATOR, DEFAULT };
-}
+// the public name of the device type (SkelCL.h:137)
+typedef detail::Device::Type device_type;
 
-// DeviceProperties (detail/DeviceProperties.h): which devices init takes.
-class DeviceProperties {
-public:
-  DeviceProperties& nDevices(size_t n)
-  {
-    _n = n;
-    return *this;
-  }
-  DeviceProperties& deviceType(int t)
-  {
-    _type = t;
-    return *this;
-  }
-  size_t count() const { return _n; }
-  int type() const { return _type; }
+inline detail::DeviceProperties allDevices() { return detail::DeviceProperties::allDevices(); }
+inline detail::DeviceProperties nDevices(size_t n) { return detail::DeviceProperties::nDevices(n); }
+inline detail::PlatformID platform(size_t pID) { return detail::PlatformID(pID); }
+inline detail::DeviceID device(size_t dID) { return detail::DeviceID(dID); }
 
-private:
-  size_t _n = 0;  // 0 = all
-  int _type = device_type::ANY;
-};
-
-inline DeviceProperties allDevices() { return DeviceProperties(); }
-inline DeviceProperties nDevices(size_t n) { return DeviceProperties().nDevices(n); }
-
-struct PlatformID {
-  size_t id;
-};
-struct DeviceID {
-  size_t id;
-};
-inline PlatformID platform(size_t id) { return PlatformID{id}; }
-inline DeviceID device(size_t id) { return DeviceID{id}; }
-
-// skelcl::init (src/SkelCL.cpp:51-54): one GPU, GPU 0.
-inline void init(DeviceProperties p = allDevices())
+// skelcl::init (src/SkelCL.cpp:51-64): selects the GPU into globalDeviceList
+inline void init(detail::DeviceProperties properties = allDevices())
 {
-  if (p.type() == device_type::CPU || p.type() == device_type::ACCELERATOR)
-    detail::abortWithError("skelcl_amd runs on the MI355X GPU only");
-  detail::check(skl_init((int)p.count(), SKL_DEVICE_ANY), "init");
+  detail::globalDeviceList.init(properties);
 }
 
-inline void init(PlatformID p, DeviceID d)
+inline void init(detail::PlatformID pID, detail::DeviceID dID)
 {
-  if (p.id != 0 || d.id != 0)
-    detail::abortWithError("skelcl_amd exposes only platform 0 / device 0");
-  detail::check(skl_init(1, SKL_DEVICE_ANY), "init");
+  detail::globalDeviceList.init(pID, dID);
 }
 
-inline void terminate() { skl_terminate(); }
-
-// number of devices after init (DeviceList::size: one, the MI355X)
-inline size_t deviceCount()
+// skelcl::terminate (src/SkelCL.cpp:66-70): drops the device list; init may
+// be called again
+inline void terminate()
 {
-  int n = 0;
-  skl_device_count(&n);
-  return (size_t)n;
+  detail::globalDeviceList.clear();
+  skl_terminate();
 }
+
+// number of devices after init (globalDeviceList.size(): one, the MI355X)
+inline size_t deviceCount() { return detail::globalDeviceList.size(); }
 
 }  // namespace skelcl

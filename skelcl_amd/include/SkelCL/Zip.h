@@ -77,6 +77,7 @@ public:
                            Args&&... args)
   {
     if (l.size() != r.size()) detail::abortWithError("Zip: inputs differ in size");
+    detail::zipDistributions(l, r);
     auto L = detail::makeArgs(std::forward<Args>(args)...);
     const void* lp = l.devicePointer();
     const void* rp = r.devicePointer();
@@ -88,6 +89,7 @@ public:
                     : (o.prepareOutput(l.size()), o.deviceOutputPointer());
     detail::check(skl_zip_run(_h, dst, lp, rp, l.size(), L.data(), L.count(), nullptr), "Zip");
     o.dataOnDeviceModified();
+    detail::adoptDistribution(o, l);
     L.finish();
     return o;
   }
@@ -103,6 +105,7 @@ public:
                            Args&&... args)
   {
     if (!(l.size() == r.size())) detail::abortWithError("Zip: inputs differ in size");
+    detail::zipDistributions(l, r);
     auto L = detail::makeArgs(std::forward<Args>(args)...);
     const void* lp = l.devicePointer();
     const void* rp = r.devicePointer();
@@ -118,6 +121,7 @@ public:
     detail::check(
         skl_zip_run(_h, dst, lp, rp, l.size().elemCount(), L.data(), L.count(), nullptr), "Zip");
     o.dataOnDeviceModified();
+    detail::adoptDistribution(o, l);
     L.finish();
     return o;
   }
@@ -146,6 +150,7 @@ public:
   void operator()(const Vector<Tl>& l, const Vector<Tr>& r, Args&&... args)
   {
     if (l.size() != r.size()) detail::abortWithError("Zip: inputs differ in size");
+    detail::zipDistributions(l, r);
     auto L = detail::makeArgs(std::forward<Args>(args)...);
     detail::check(skl_zip_run(_h, nullptr, l.devicePointer(), r.devicePointer(), l.size(),
                               L.data(), L.count(), nullptr),

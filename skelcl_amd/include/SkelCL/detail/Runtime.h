@@ -14,6 +14,7 @@
 
 #include "skelcl_rt.h"
 #include "skelcl_skeletons.h"
+#include "Util.h"
 
 namespace skelcl {
 namespace detail {
@@ -81,7 +82,16 @@ private:
 }  // namespace detail
 
 // ---- element type names as the kernel source spells them ------------------
-template <typename T> struct TypeName;
+// Built-in names are spelled out; any other type (skelcl::float4, a user
+// struct from SKELCL_COMMON_DEFINITION) takes its demangled name without
+// namespaces, as the reference's typeToString (detail/Util.h:81-98).
+template <typename T> struct TypeName {
+  static const char* get()
+  {
+    static const std::string name = detail::util::typeToString<T>();
+    return name.c_str();
+  }
+};
 #define SKELCL_TYPE_NAME(T, NAME)                                              \
   template <> struct TypeName<T> {                                             \
     static const char* get() { return NAME; }                                  \

@@ -20,6 +20,7 @@
 #include "../Out.h"
 #include "../Source.h"
 #include "../Vector.h"
+#include "Distribution.h"
 #include "Runtime.h"
 
 namespace skelcl {
@@ -115,6 +116,74 @@ inline std::string valueString(const T& v)
   std::ostringstream o;
   o << std::setprecision(std::numeric_limits<double>::max_digits10) << v;
   return o.str();
+}
+
+// ---- the reference's distribution rules for skeleton inputs and outputs.
+// On one GPU they change no data placement; containers record them so that
+// host code reading distribution() sees what the reference would set.
+
+// an input without a distribution gets kind D (MapHelperDef.h:70-71,
+// ReduceDef.h:135-136, ScanDef.h:303-304)
+template <template <typename> class D, typename C>
+inline void defaultDistribution(const C& c)
+{
+  if (!c.distribution().isValid()) c.setDistribution(D<C>());
+}
+
+// the output takes the input's distribution (MapHelperDef.h:92,
+// ZipDef.h:268, ReduceDef.h:157, ScanDef.h:324, MapOverlapDef.h:275)
+template <typename O, typename I>
+inline void adoptDistribution(const O& o, const I& in)
+{
+  o.setDistribution(in.distribution());
+}
+
+// Zip (ZipDef.h:232-243): both unset or unequal -> Block; one unset -> the
+// other's
+template <typename L, typename R>
+inline void zipDistributions(const L& l, const R& r)
+{
+  const bool lv = l.distribution().isValid(), rv = r.distribution().isValid();
+  if (!lv && !rv) {
+    l.setDistribution(BlockDistribution<L>());
+    r.setDistribution(BlockDistribution<R>());
+  } else if (!lv) {
+    l.setDistribution(r.distribution());
+  } else if (!rv) {
+    r.setDistribution(l.distribution());
+  } else if (*cloneAndConvert<L>(r.distribution()) != l.distribution()) {
+    l.setDistribution(BlockDistribution<L>());
+    r.setDistribution(BlockDistribution<R>());
+  }
+}
+
+// AllPairs (AllPairsDef.h:346-410): by default left Block, right Copy; a
+// Single side makes the other Single, a Block right becomes Copy
+template <typename L, typename R>
+inline void allPairsDistributions(const L& l, const R& r)
+{
+  const auto& ld = l.distribution();
+  const auto& rd = r.distribution();
+  if (!ld.isValid() && !rd.isValid()) {
+    l.setDistribution(BlockDistribution<L>());
+    r.setDistribution(CopyDistribution<R>());
+  } else if (!ld.isValid()) {
+    if (rd.isSingle()) {
+      l.setDistribution(SingleDistribution<L>());
+    } else {
+      l.setDistribution(BlockDistribution<L>());
+      if (rd.isBlock()) r.setDistribution(CopyDistribution<R>());
+    }
+  } else if (!rd.isValid()) {
+    if (ld.isSingle()) r.setDistribution(SingleDistribution<R>());
+    else r.setDistribution(CopyDistribution<R>());
+  } else if (ld.isCopy() && rd.isSingle()) {
+    l.setDistribution(SingleDistribution<L>());
+  } else if (ld.isSingle() && rd.isBlock()) {
+    r.setDistribution(SingleDistribution<R>());
+  } else if (rd.isSingle() || rd.isBlock()) {
+    if (ld.isBlock() || ld.isCopy()) r.setDistribution(CopyDistribution<R>());
+  }
 }
 
 class SkeletonBase {

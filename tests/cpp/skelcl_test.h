@@ -79,6 +79,7 @@ inline int run_all(int argc, char** argv)
   do {                                                                         \
     if (!(c)) SKL_FAIL_AT(std::string(#c));                                    \
   } while (0)
+#define EXPECT_FALSE(c) EXPECT_TRUE(!(c))
 #define EXPECT_LE(a, b) EXPECT_TRUE((a) <= (b))
 #define EXPECT_NEAR(a, b, tol) EXPECT_TRUE(std::fabs((double)(a) - (double)(b)) <= (tol))
 #define ASSERT_EQ(a, b) EXPECT_EQ(a, b)
