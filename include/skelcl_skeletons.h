@@ -253,7 +253,10 @@ int skl_scan_backward(skl_skeleton s, void* d_in, const void* d_out,
 /* MapOverlap whose user function is a weighted stencil
  * sum_{dy,dx} w[(dy+R)(2R+1)+dx+R] * getData(img,dx,dy): the adjoint is the
  * transposed stencil of d_out under the same padding.  `weights` is the
- * device weight table the forward received. */
+ * device weight table the forward received.  The first call checks the
+ * user function by running it on a small seeded probe against that sum on
+ * the host (it allocates and synchronises once); any other user function
+ * gives SKL_ERR_UNSUPPORTED. */
 int skl_mapoverlap_stencil_backward(skl_skeleton s, void* d_in,
                                     const void* d_out, const float* weights,
                                     size_t rows, size_t cols,

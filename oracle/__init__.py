@@ -67,6 +67,8 @@ def lib():
                                                   _int, ctypes.c_float, _int]),
             "orc_mapoverlap_stencil_f64": (None, [_f32p, _f64p, _int, _int, _f32p, _int,
                                                   _int, ctypes.c_float, _i64p, _int]),
+            "orc_mapoverlap_stencil_f32_fma": (None, [_f32p, _f32p, _int, _int, _f32p, _int,
+                                                      _int, ctypes.c_float, _i64p, _int]),
             "orc_mapoverlap_stencil_adjoint_f64": (None, [_f32p, _f64p, _int, _int, _f32p,
                                                           _int, _int, _i64p, _int]),
             "orc_getdata_i32": (ctypes.c_int32, [_i32p, _int, _int, _int, _int, _int,
@@ -248,6 +250,19 @@ def mapoverlap_stencil_f64(img, w, R, pix, padding=NEUTRAL, neutral=0.0):
     out = np.empty(pix.size, np.float64)
     lib().orc_mapoverlap_stencil_f64(img, out, rows, cols, w.ravel(), R, padding,
                                      neutral, pix, pix.size)
+    return out
+
+
+def mapoverlap_stencil_f32_fma(img, w, R, pix, padding=NEUTRAL, neutral=0.0):
+    """The stencil as a contracting compiler evaluates `sum += getData * w`:
+    one fmaf per term in dy-outer, dx-inner order, at the pixels `pix`."""
+    img = _c(img, np.float32)
+    w = _c(w, np.float32)
+    pix = _c(pix, np.int64)
+    rows, cols = img.shape
+    out = np.empty(pix.size, np.float32)
+    lib().orc_mapoverlap_stencil_f32_fma(img, out, rows, cols, w.ravel(), R, padding,
+                                         neutral, pix, pix.size)
     return out
 
 

@@ -28,6 +28,14 @@
  *                                  FP64 otherwise.
  *   orc_scan_add_f64 / _adjoint    numpy cumsum; adjoint against FP64 central
  *                                  finite differences of sum(g*scan(x)).
+ *   orc_mapoverlap_stencil_f32_fma the same chain with every step rounded
+ *                                  exactly (Python Fractions, round to
+ *                                  nearest even f32) on random inputs.
+ *   orc_saxpy_f32 / _fma, orc_scale_f32, orc_zip_mul_f32
+ *                                  numpy float32 arithmetic (two roundings)
+ *                                  and exactly rounded a*x+y (Fractions).
+ *   orc_gen_loguniform             the committed batched_shapes.json segment
+ *                                  lengths (written from Python integers).
  *   orc_mapoverlap_*               MapOverlapTests.cpp:187-447 shift KATs
  *                                  (through orc_getdata_i32); a delta-image
  *                                  reproduces the flipped weight table; the
@@ -544,6 +552,25 @@ void orc_mapoverlap_stencil_f64(const float* in, double* out, int rows,
                                  neutral) *
              (double)w[(dy + R) * W + (dx + R)];
     out[k] = s;
+  }
+}
+
+/* The user function `sum += getData(img, dx, dy) * w[...]` as a compiler
+ * that contracts it evaluates it: one fmaf per term, dy outer, dx inner. */
+void orc_mapoverlap_stencil_f32_fma(const float* in, float* out, int rows,
+                                    int cols, const float* w, int R,
+                                    int padding, float neutral,
+                                    const int64_t* pix, int npix)
+{
+  int W = 2 * R + 1;
+  for (int k = 0; k < npix; ++k) {
+    int r = (int)(pix[k] / cols), c = (int)(pix[k] % cols);
+    float sum = 0.0f;
+    for (int dy = -R; dy <= R; ++dy)
+      for (int dx = -R; dx <= R; ++dx)
+        sum = fmaf(getdata_f32(in, rows, cols, r, c, dx, dy, padding, neutral),
+                   w[(dy + R) * W + (dx + R)], sum);
+    out[k] = sum;
   }
 }
 

@@ -69,6 +69,12 @@ void orc_mapoverlap_stencil_f64(const float* in, double* out, int rows,
                                 int cols, const float* w, int R, int padding,
                                 float neutral, const int64_t* pix,
                                 int npix);
+/* the stencil with one fmaf per term (a contracting compiler's form of
+ * `sum += getData(..) * w[..]`), at the listed pixels */
+void orc_mapoverlap_stencil_f32_fma(const float* in, float* out, int rows,
+                                    int cols, const float* w, int R,
+                                    int padding, float neutral,
+                                    const int64_t* pix, int npix);
 /* d(sum W*out)/d(in) for the linear stencil, FP64 */
 void orc_mapoverlap_stencil_adjoint_f64(const float* g, double* din,
                                         int rows, int cols, const float* w,

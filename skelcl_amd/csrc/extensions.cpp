@@ -3,7 +3,10 @@
 // segmented Reduce/Scan and batched small products.
 #include <hip/hip_runtime.h>
 
+#include <cmath>
+#include <cstdint>
 #include <mutex>
+#include <vector>
 
 #include "common.h"
 #include "skeleton.h"
@@ -171,6 +174,89 @@ extern "C" int skl_zip_backward(skl_skeleton s, void* d_left, void* d_right, con
   return skl_run_backward_zip(s, d_left, d_right, left, right, d_out, n, args, nargs, stream);
 }
 
+// The transposed stencil is the adjoint only when the forward user function
+// is out = sum_{dy,dx} w[(dy+R)(2R+1) + dx+R] * getData(img, dx, dy) over the
+// weight table it takes as its one extra argument.  That is checked once per
+// skeleton by running the forward on a small seeded image (border pixels
+// included, rows != cols) with seeded asymmetric weights and comparing with
+// that sum on the host; a transposed weight index, a missing term or a
+// non-linear term fails the check.  The probe allocates and synchronises,
+// so a capture of the adjoint needs one eager call first.
+namespace {
+int check_weighted_stencil(skl_skeleton_s* s, skl_stream stream)
+{
+  if (s->stencil_checked == 1) return SKL_OK;
+  auto reject = [&](const char* why) {
+    s->stencil_checked = -1;
+    return fail(SKL_ERR_UNSUPPORTED,
+                "stencil backward: the MapOverlap user function is not the weighted stencil "
+                "sum w[(dy+R)*(2R+1)+dx+R] * getData(img, dx, dy) (%s)",
+                why);
+  };
+  if (s->stencil_checked == -1) return reject("checked before");
+  if (s->extras.size() != 1 || s->extras[0].kind != ParamKind::Buffer)
+    return reject("it must take the weight table as its only extra argument");
+  const int R = s->overlap, W = 2 * R + 1;
+  const int rows = 2 * R + 3, cols = 2 * R + 6;
+  std::vector<float> img((size_t)rows * cols), w((size_t)W * W), out((size_t)rows * cols);
+  uint32_t z = 0x9E3779B9u;
+  auto next = [&]() {
+    z = z * 1664525u + 1013904223u;
+    return (float)((z >> 8) & 0xFFFF) / 65536.0f;
+  };
+  for (auto& v : img) v = 4.0f * next() - 2.0f;
+  for (auto& v : w) v = 0.25f + next();
+  void *dimg = nullptr, *dw = nullptr, *dout = nullptr;
+  int st = SKL_OK;
+  if (hipMalloc(&dimg, img.size() * 4) != hipSuccess || hipMalloc(&dw, w.size() * 4) != hipSuccess ||
+      hipMalloc(&dout, out.size() * 4) != hipSuccess) {
+    st = fail(SKL_ERR_HIP, "stencil backward: probe allocation failed");
+  }
+  hipStream_t hs = resolve_stream(stream);
+  if (st == SKL_OK &&
+      (hipMemcpyAsync(dimg, img.data(), img.size() * 4, hipMemcpyHostToDevice, hs) != hipSuccess ||
+       hipMemcpyAsync(dw, w.data(), w.size() * 4, hipMemcpyHostToDevice, hs) != hipSuccess))
+    st = fail(SKL_ERR_HIP, "stencil backward: probe upload failed");
+  if (st == SKL_OK) {
+    skl_arg a{SKL_ARG_BUFFER, dw, 0, 0};
+    st = skl_mapoverlap_run(s, dout, dimg, rows, cols, &a, 1, stream);
+  }
+  if (st == SKL_OK &&
+      (hipMemcpyAsync(out.data(), dout, out.size() * 4, hipMemcpyDeviceToHost, hs) != hipSuccess ||
+       hipStreamSynchronize(hs) != hipSuccess))
+    st = fail(SKL_ERR_HIP, "stencil backward: probe download failed");
+  if (dimg) hipFree(dimg);
+  if (dw) hipFree(dw);
+  if (dout) hipFree(dout);
+  if (st != SKL_OK) return st;
+  const bool nearest = s->padding == SKL_PADDING_NEAREST;
+  double wabs = 0.0;
+  for (float v : w) wabs += std::fabs((double)v);
+  for (int r = 0; r < rows; ++r)
+    for (int c = 0; c < cols; ++c) {
+      double ref = 0.0;
+      for (int dy = -R; dy <= R; ++dy)
+        for (int dx = -R; dx <= R; ++dx) {
+          int rr = r + dy, cc = c + dx;
+          double v = 0.0;
+          if (nearest) {
+            rr = rr < 0 ? 0 : (rr >= rows ? rows - 1 : rr);
+            cc = cc < 0 ? 0 : (cc >= cols ? cols - 1 : cc);
+            v = img[(size_t)rr * cols + cc];
+          } else if (rr >= 0 && rr < rows && cc >= 0 && cc < cols) {
+            v = img[(size_t)rr * cols + cc];
+          }
+          ref += v * (double)w[(size_t)(dy + R) * W + (dx + R)];
+        }
+      const double got = out[(size_t)r * cols + c];
+      if (!(std::fabs(got - ref) <= 1e-4 * 2.0 * wabs))
+        return reject("the probe's forward differs from the weighted sum");
+    }
+  s->stencil_checked = 1;
+  return SKL_OK;
+}
+}  // namespace
+
 extern "C" int skl_mapoverlap_stencil_backward(skl_skeleton s, void* d_in, const void* d_out,
                                                const float* weights, size_t rows, size_t cols,
                                                skl_stream stream)
@@ -185,6 +271,7 @@ extern "C" int skl_mapoverlap_stencil_backward(skl_skeleton s, void* d_in, const
   if (rows == 0 || cols == 0) return SKL_OK;
   if (!d_in || !d_out || !weights) return fail(SKL_ERR_INVALID, "NULL buffer");
   if (rows > 0x7FFFFFFF || cols > 0x7FFFFFFF) return fail(SKL_ERR_INVALID, "matrix too large");
+  SKL_TRY(check_weighted_stencil(s, stream));
   return skl_internal_stencil_transpose((float*)d_in, (const float*)d_out, weights, (int)rows,
                                         (int)cols, s->overlap,
                                         s->padding == SKL_PADDING_NEAREST ? 1 : 0,

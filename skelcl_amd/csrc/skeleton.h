@@ -20,6 +20,7 @@ struct skl_skeleton_s {
   std::string deriv_src, deriv_func, deriv_func2, options;
   std::string common;  // common definitions registered when it was created
   std::string contract = "on";  // -ffp-contract of the user functions
+  int stencil_checked = 0;  // MapOverlap: 1 = a weighted stencil, -1 = not, 0 = unknown
   std::vector<TypeInfo> types;  // kind-specific order, see skelcl_skeletons.h
   int overlap = 1, padding = SKL_PADDING_NEAREST;
   int wg = 256;

@@ -545,6 +545,48 @@ TEST(AutodiffTest, FunctionPairs)
   auto grads = MatMulFn::backward(c4, mm, dC);
   EXPECT_EQ(10.0f, grads.first[0][0]);   // sum_j dC[i][j] * B[k][j] = 5 * 2
   EXPECT_EQ(4.0f, grads.second[2][4]);   // sum_i A[i][k] * dC[i][j] = 4 * 1
+  // Zip: f(a, b) = a*b + 3a, df/da = b + 3, df/db = a
+  skelcl::Zip<float(float, float)> z("float func(float a, float b){ return a*b + 3.0f*a; }",
+                                     "func",
+                                     "float da(float a, float b){ return b + 3.0f; }\n"
+                                     "float db(float a, float b){ return a; }",
+                                     "da", "db");
+  skelcl::Vector<float> a(300), b(300), gz(300);
+  for (size_t i = 0; i < 300; ++i) {
+    a[i] = 0.5f * (float)i;
+    b[i] = 2.0f - 0.25f * (float)i;
+    gz[i] = (float)(i % 7) - 3.0f;
+  }
+  Context c5;
+  skelcl::Vector<float> zz = ZipFn<float>::forward(c5, z, a, b);
+  EXPECT_EQ(a[10] * b[10] + 3.0f * a[10], zz[10]);
+  auto zg = ZipFn<float>::backward(c5, z, gz);
+  for (size_t i = 0; i < 300; ++i) {
+    EXPECT_EQ(gz[i] * (b[i] + 3.0f), zg.first[i]);
+    EXPECT_EQ(gz[i] * a[i], zg.second[i]);
+  }
+  // Stencil: a 3x3 weighted sum under NEUTRAL zero padding; its adjoint is
+  // the transposed stencil (each weight reaches the mirrored neighbour)
+  skelcl::MapOverlap<float(float)> st(
+      "float func(input_matrix_t m, __global float* w) { float s = 0.0f;"
+      " for (int dy = -1; dy <= 1; ++dy) for (int dx = -1; dx <= 1; ++dx)"
+      " s += getData(m, dx, dy) * w[(dy + 1) * 3 + (dx + 1)]; return s; }",
+      1, skelcl::detail::Padding::NEUTRAL, 0.0f);
+  skelcl::Vector<float> w(9);
+  for (size_t i = 0; i < 9; ++i) w[i] = (float)(i + 1);
+  skelcl::Matrix<float> img(skelcl::MatrixSize{6, 7}, 1.0f);
+  Context c6;
+  skelcl::Matrix<float> so = StencilFn::forward(c6, st, img, w);
+  EXPECT_EQ(45.0f, so[2][3]);                        // interior: sum of all weights
+  EXPECT_EQ(5.0f + 6.0f + 8.0f + 9.0f, so[0][0]);    // corner: only dy, dx >= 0
+  skelcl::Matrix<float> dy(skelcl::MatrixSize{6, 7}, 0.0f);
+  dy[3][3] = 1.0f;  // gradient of out[3][3] alone
+  skelcl::Matrix<float> dimg = StencilFn::backward(c6, st, dy);
+  // out[3][3] reads img[3+dy][3+dx] with weight w[(dy+1)*3 + dx+1]
+  EXPECT_EQ(1.0f, dimg[2][2]);
+  EXPECT_EQ(6.0f, dimg[3][4]);
+  EXPECT_EQ(9.0f, dimg[4][4]);
+  EXPECT_EQ(0.0f, dimg[0][0]);
 }
 
 int main(int argc, char** argv)

@@ -488,3 +488,120 @@ def test_fused_reduce_follows_a_new_zip(skl):
             del z
             gc.collect()
             assert abs(got - want) <= 1e-5 * max(1.0, np.abs(x).sum()), body
+
+
+# ------------------------------------------- adjoint rigour (FD + guards)
+def test_zip_backward_vs_fd(skl):
+    """A non-linear Zip: the GPU adjoint against the FP64 reference gradient
+    and FP64 central finite differences of loss = sum(g * f(x, y))."""
+    src = "float func(float x, float y){ return x*y + sinf(x)*y*y; }"
+    dsrc = ("float dl(float x, float y){ return y + cosf(x)*y*y; }\n"
+            "float dr(float x, float y){ return x + 2.0f*sinf(x)*y; }")
+    z = skl.Zip("float(float, float)", src, deriv_source=dsrc, deriv_func="dl", deriv_func2="dr")
+    n = 5003
+    x, y, g = (oracle.gen_f32_sym(s, n).astype(np.float32) for s in (0x6001, 0x6002, 0x6003))
+    dl, dr = (np.asarray(v).astype(np.float64) for v in z.backward(skl.Vector(x), skl.Vector(y),
+                                                                      skl.Vector(g)))
+    xd, yd, gd = x.astype(np.float64), y.astype(np.float64), g.astype(np.float64)
+    assert np.max(np.abs(dl - gd * (yd + np.cos(xd) * yd * yd))) < 1e-5
+    assert np.max(np.abs(dr - gd * (xd + 2 * np.sin(xd) * yd))) < 1e-5
+    f = lambda a, b: a * b + np.sin(a) * b * b
+    h = 1e-5
+    for i in (0, 17, 2500, n - 1):
+        fdl = gd[i] * (f(xd[i] + h, yd[i]) - f(xd[i] - h, yd[i])) / (2 * h)
+        fdr = gd[i] * (f(xd[i], yd[i] + h) - f(xd[i], yd[i] - h)) / (2 * h)
+        assert abs(fdl - dl[i]) < 1e-5 * max(1.0, abs(fdl)), i
+        assert abs(fdr - dr[i]) < 1e-5 * max(1.0, abs(fdr)), i
+
+
+def test_reduce_backward_vs_fd_of_the_gpu_forward(skl):
+    """Reduce(+) is linear, so central differences of the GPU forward itself
+    with h = 1 give the gradient up to FP32 rounding of the sums: every
+    sampled d loss / d x_i equals the broadcast upstream value."""
+    n = 100003
+    x = oracle.gen_f32_sym(0x6004, n)
+    up = np.float32(0.75)
+    r = skl.Reduce("float(float)", "float func(float x, float y){ return x+y; }")
+    d = np.asarray(r.backward(skl.Vector(np.array([up], np.float32)), n))
+    loss = lambda v: float(up) * float(r(skl.Vector(v))[0])
+    for i in (0, 1, 4099, n - 1):
+        xp, xm = x.copy(), x.copy()
+        xp[i] += np.float32(1.0)
+        xm[i] -= np.float32(1.0)
+        fd = (loss(xp) - loss(xm)) / 2.0
+        assert abs(fd - float(d[i])) < 2e-5 * max(1.0, abs(fd)), (i, fd, d[i])
+
+
+def test_allpairs_backward_vs_fd_of_the_gpu_forward(skl):
+    """The (+,*) product is bilinear: central differences of the GPU forward
+    of loss = sum(dC * (A.B)) with h = 1 at sampled entries of A and B match
+    dA = dC.B^T and dB = A^T.dC up to FP32 rounding."""
+    zip_ = skl.Zip("float(float, float)", "float func(float x, float y){ return x*y; }")
+    red = skl.Reduce("float(float)", "float func(float x, float y){ return x+y; }")
+    ap = skl.AllPairs("float(float, float)", reduce=red, zip=zip_)
+    M, K, N = 70, 130, 90
+    A = oracle.gen_f32_sym(0x6005, M * K).reshape(M, K)
+    B = oracle.gen_f32_sym(0x6006, K * N).reshape(K, N)
+    dC = oracle.gen_f32_sym(0x6007, M * N).reshape(M, N)
+    dA, dB = (np.asarray(v).astype(np.float64)
+              for v in ap.backward(skl.Matrix(A), skl.Matrix(B), skl.Matrix(dC)))
+    dC64 = dC.astype(np.float64)
+    loss = lambda a, b: float((dC64 * np.asarray(ap(skl.Matrix(a), skl.Matrix(b)))).sum())
+    for (i, k) in ((0, 0), (69, 129), (33, 7)):
+        ap_, am = A.copy(), A.copy()
+        ap_[i, k] += 1.0
+        am[i, k] -= 1.0
+        fd = (loss(ap_, B) - loss(am, B)) / 2.0
+        assert abs(fd - dA[i, k]) < 1e-4 * max(1.0, abs(fd)), (i, k, fd, dA[i, k])
+    for (k, j) in ((0, 0), (129, 89), (64, 45)):
+        bp, bm = B.copy(), B.copy()
+        bp[k, j] += 1.0
+        bm[k, j] -= 1.0
+        fd = (loss(A, bp) - loss(A, bm)) / 2.0
+        assert abs(fd - dB[k, j]) < 1e-4 * max(1.0, abs(fd)), (k, j, fd, dB[k, j])
+
+
+@pytest.mark.parametrize("src,why", [
+    # weights indexed transposed: not the stencil of w (w is asymmetric)
+    ("float func(input_matrix_t img, __global float* w) { float s = 0.0f;"
+     " for (int dy = -2; dy <= 2; ++dy) for (int dx = -2; dx <= 2; ++dx)"
+     " s += getData(img, dx, dy) * w[(dx + 2) * 5 + (dy + 2)]; return s; }", "transposed"),
+    # a non-linear term
+    ("float func(input_matrix_t img, __global float* w) { float s = 0.0f;"
+     " for (int dy = -2; dy <= 2; ++dy) for (int dx = -2; dx <= 2; ++dx)"
+     " { float v = getData(img, dx, dy); s += v * v * w[(dy + 2) * 5 + (dx + 2)]; } return s; }",
+     "square"),
+    # a term left out
+    ("float func(input_matrix_t img, __global float* w) { float s = 0.0f;"
+     " for (int dy = -2; dy <= 2; ++dy) for (int dx = -2; dx < 2; ++dx)"
+     " s += getData(img, dx, dy) * w[(dy + 2) * 5 + (dx + 2)]; return s; }", "dropped"),
+    # another extra argument
+    ("float func(input_matrix_t img, __global float* w, float k) {"
+     " return k * getData(img, 0, 0) * w[12]; }", "extra"),
+])
+def test_stencil_backward_rejects_other_user_functions(skl, src, why):
+    from skelcl_amd import _lib
+    m = skl.MapOverlap("float(float)", src, 2, 0, 0)
+    W = oracle.gen_f32_sym(0x6010, 30 * 40).reshape(30, 40)
+    w = oracle.gen_f32_sym(0x6011, 25)
+    with pytest.raises(_lib.SkelError) as e:
+        m.stencil_backward(skl.Matrix(W), skl.Vector(w))
+    assert e.value.status == _lib.SKL_ERR_UNSUPPORTED, why
+    assert "not the weighted stencil" in str(e.value)
+
+
+@pytest.mark.parametrize("padding", [0, 1])
+def test_stencil_backward_accepts_the_weighted_stencil(skl, padding):
+    """The same form with the weight index written another way is accepted
+    (the probe checks values, not spelling)."""
+    src = ("float func(input_matrix_t img, __global float* w) { float s = 0.0f;"
+           " for (int i = 0; i < 25; ++i) s += w[i] * getData(img, i % 5 - 2, i / 5 - 2);"
+           " return s; }")
+    m = skl.MapOverlap("float(float)", src, 2, padding, 0)
+    rows, cols = 30, 40
+    W = oracle.gen_f32_sym(0x6012, rows * cols).reshape(rows, cols)
+    w = oracle.gen_f32_sym(0x6013, 25).reshape(5, 5)
+    d = np.asarray(m.stencil_backward(skl.Matrix(W), skl.Vector(w.ravel()))).astype(np.float64)
+    pix = np.arange(rows * cols, dtype=np.int64)
+    ref = oracle.mapoverlap_stencil_adjoint_f64(W, w, 2, pix, padding)
+    assert np.max(np.abs(d.ravel() - ref)) < 1e-5 * max(1.0, np.abs(ref).max())

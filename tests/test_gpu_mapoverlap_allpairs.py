@@ -48,18 +48,34 @@ def test_mapoverlap_neutral_nonsquare(skl):
 @pytest.mark.parametrize("rows,cols,padding", [(64, 64, 0), (100, 37, 1), (257, 513, 0),
                                                (1024, 1000, 1)])
 def test_mapoverlap_gauss_parity(skl, rows, cols, padding):
-    """11x11 weighted stencil: bit-exact vs the oracle evaluating the same
-    user loop with fused multiply-adds; |err| vs FP64 <= 1e-5 relative."""
+    """11x11 weighted stencil, every pixel: bit-exact vs the oracle's fmaf
+    chain (the user loop as the default contraction evaluates it); within
+    1e-5 relative of FP64."""
     w = oracle.gaussian_weights(5, 2.0)
     img = oracle.gen_f32_unit(0x3001, rows * cols, 255.0).reshape(rows, cols)
     m = skl.MapOverlap("float(float)", GAUSS_SRC, 5, padding, 0)
+    out = np.asarray(m(skl.Matrix(img), skl.Vector(w.ravel()))).ravel()
+    pix = np.arange(rows * cols, dtype=np.int64)
+    fma = oracle.mapoverlap_stencil_f32_fma(img, w, 5, pix, padding, 0.0)
+    assert np.array_equal(out.view(np.uint32), fma.view(np.uint32))
+    sub = pix[::97]
+    ref = oracle.mapoverlap_stencil_f64(img, w, 5, sub, padding)
+    assert np.max(np.abs(out[sub] - ref) / np.maximum(1.0, np.abs(ref))) < 1e-5
+
+
+@pytest.mark.parametrize("padding", [0, 1])
+def test_mapoverlap_gauss_contraction_off_bit_exact(skl, padding):
+    """With the parity setting (-ffp-contract=off) the stencil is bit-exact
+    vs the oracle's two-rounding form (each product, then each sum, rounded
+    as written)."""
+    rows, cols = 300, 257
+    w = oracle.gaussian_weights(5, 2.0)
+    img = oracle.gen_f32_unit(0x3004, rows * cols, 255.0).reshape(rows, cols)
+    m = skl.MapOverlap("float(float)", GAUSS_SRC, 5, padding, 0,
+                       build_options="-ffp-contract=off")
     out = np.asarray(m(skl.Matrix(img), skl.Vector(w.ravel())))
-    pix = np.arange(0, rows * cols, 97, dtype=np.int64)
-    ref = oracle.mapoverlap_stencil_f64(img, w, 5, pix, padding)
-    got = out.ravel()[pix].astype(np.float64)
-    assert np.max(np.abs(got - ref) / np.maximum(1.0, np.abs(ref))) < 1e-5
     o32 = oracle.mapoverlap_stencil_f32(img, w, 5, padding)
-    assert np.max(np.abs(out - o32)) <= 1e-3  # fma vs separate rounding
+    assert np.array_equal(out.view(np.uint32), o32.view(np.uint32))
 
 
 # ------------------------------------------------------------------ AllPairs

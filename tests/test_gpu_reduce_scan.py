@@ -35,8 +35,9 @@ def test_reduce_long_kat(skl):  # ReduceTests.cpp:123-136
     assert r(skl.Vector(np.ones(100000000, np.int32)))[0] == 100000000
 
 
-@pytest.mark.parametrize("n", list(range(1, 256)) + list(range(8192, 8448, 17)))
-def test_reduce_sizes_kat(skl, n):  # ReduceTests.cpp:138-170 (ones, as float)
+@pytest.mark.parametrize("n", list(range(1, 256)))
+def test_reduce_sizes_kat(skl, n):  # ReduceTests.cpp:138-153 (ones, as float); 155-170
+    # (every size 8192..8447) is test_gpu_full_size.py::test_reduce_all_sizes_8192_to_8447
     r = skl.Reduce("float(float)", ADD_F)
     assert r(skl.Vector(np.ones(n, np.float32)))[0] == n
 

@@ -283,3 +283,95 @@ def test_segmented():
         seg = x[off[k]:off[k + 1]].astype(np.float64)
         assert abs(s[k] - seg.sum()) < 1e-12
         assert np.allclose(sc[off[k]:off[k + 1]], np.cumsum(seg), atol=1e-12)
+
+
+# ---- exactly rounded references (Python Fractions) ------------------------
+from fractions import Fraction  # noqa: E402
+
+
+def _round_f32(q):
+    """The float32 nearest the rational q, ties to even: float(q) is the
+    nearest double, so the answer is within one float32 step of it."""
+    f = np.float32(float(q))
+    best = None
+    for c in (np.nextafter(f, np.float32(-np.inf)), f, np.nextafter(f, np.float32(np.inf))):
+        d = abs(Fraction(float(c)) - q)
+        key = (d, int(np.array(c, np.float32).view(np.uint32)) & 1)
+        if best is None or key < best[0]:
+            best = (key, c)
+    return np.float32(best[1])
+
+
+def _fma_f32(a, b, c):
+    return _round_f32(Fraction(float(a)) * Fraction(float(b)) + Fraction(float(c)))
+
+
+def test_round_f32_matches_numpy_on_sums():
+    """The exact rounding agrees with float32 addition, which IEEE rounds
+    correctly (a check of the helper itself)."""
+    a, b = oracle.gen_f32_sym(71, 300), oracle.gen_f32_sym(72, 300)
+    for x, y in zip(a, b):
+        assert _round_f32(Fraction(float(x)) + Fraction(float(y))) == np.float32(x + y)
+
+
+@pytest.mark.parametrize("padding,R", [(0, 2), (1, 2), (0, 1), (1, 3)])
+def test_stencil_fma_form_is_the_exact_fma_chain(padding, R):
+    """orc_mapoverlap_stencil_f32_fma is sum = fma(getData(dx, dy), w, sum)
+    with dy outer and dx inner, every step rounded once: rebuilt here step by
+    step with exact rational arithmetic on a non-square image, all pixels
+    (the border ones included)."""
+    rows, cols = 9, 13
+    img = (oracle.gen_f32_unit(0x3501, rows * cols, 255.0)).reshape(rows, cols)
+    w = oracle.gen_f32_sym(0x3502, (2 * R + 1) ** 2).reshape(2 * R + 1, 2 * R + 1)
+    pix = np.arange(rows * cols, dtype=np.int64)
+    got = oracle.mapoverlap_stencil_f32_fma(img, w, R, pix, padding, 0.0)
+    for p in range(rows * cols):
+        r, c = divmod(p, cols)
+        s = np.float32(0.0)
+        for dy in range(-R, R + 1):
+            for dx in range(-R, R + 1):
+                rr, cc = r + dy, c + dx
+                if padding == 1:
+                    rr, cc = min(max(rr, 0), rows - 1), min(max(cc, 0), cols - 1)
+                    v = img[rr, cc]
+                else:
+                    v = img[rr, cc] if 0 <= rr < rows and 0 <= cc < cols else np.float32(0)
+                s = _fma_f32(v, w[dy + R, dx + R], s)
+        assert got[p] == s, (p, got[p], s)
+    # and it differs from the two-rounding form somewhere (so the test can
+    # tell the forms apart)
+    two = oracle.mapoverlap_stencil_f32(img, w, R, padding, 0.0)
+    assert not np.array_equal(got, two.ravel())
+
+
+def test_map_zip_helpers_against_numpy_and_exact_fma():
+    """saxpy_f32 / scale_f32 / zip_mul_f32 round each operation once, as
+    numpy float32 does; saxpy_f32_fma rounds a*x+y once (exact rationals)."""
+    x, y = oracle.gen_f32_sym(0x2003, 5000), oracle.gen_f32_sym(0x2004, 5000)
+    a = np.float32(2.5)
+    assert np.array_equal(oracle.saxpy_f32(a, x, y), (a * x + y).astype(np.float32))
+    assert np.array_equal(oracle.saxpy_f32(a, x, y, nthreads=4), oracle.saxpy_f32(a, x, y))
+    assert np.array_equal(oracle.scale_f32(a, x), (a * x).astype(np.float32))
+    assert np.array_equal(oracle.zip_mul_f32(x, y), (x * y).astype(np.float32))
+    a3 = np.float32(1.0 / 3.0)  # a*x is inexact, so one and two roundings differ
+    fma = oracle.saxpy_f32_fma(a3, x, y)
+    for i in range(0, 5000, 7):
+        assert fma[i] == _fma_f32(a3, x[i], y[i]), i
+    assert not np.array_equal(fma, oracle.saxpy_f32(a3, x, y))
+
+
+def test_loguniform_generator_matches_the_committed_fixture():
+    """The batched configuration's 4096 segment lengths, written from Python
+    integers by tests/golden/make_fixtures.py, are what orc_gen_loguniform
+    draws from seed 0x5001 (floor(10^(3 + 3 d)))."""
+    import json
+    import os
+    with open(os.path.join(os.path.dirname(__file__), "golden", "batched_shapes.json")) as f:
+        fx = json.load(f)
+    got = oracle.gen_loguniform(0x5001, 4096, 3.0, 3.0)
+    assert np.array_equal(got, np.array(fx["segment_lengths"], np.int64))
+    assert got.min() >= 1000 and got.max() < 1000000
+    # the product shapes: 64 * (1 + U{0..15}) from seed 0x5003, three draws each
+    u = ((oracle.gen_u64(0x5003, 3 * 512) >> np.uint64(32)) * np.uint64(16)) >> np.uint64(32)
+    shapes = (64 * (1 + u.astype(np.int64))).reshape(512, 3)
+    assert np.array_equal(shapes, np.array(fx["gemm_shapes"], np.int64))
