@@ -145,7 +145,15 @@ int skl_skeleton_extra_count(skl_skeleton s, int* count);
 /* Reads the definition of `func` in `source` (no GPU needed) and returns one
  * line per item: "return <type>" then "param <kind> <size> <type>|<name>"
  * with kind as enum skl_arg_kind.  What stooling's parameterTypeNames /
- * transferParameters inspect (libraries/stooling/src/SourceCode.cpp). */
+ * transferParameters inspect (libraries/stooling/src/SourceCode.cpp).
+ * Then "op <add|mul|max|min|other>", "work_item <0|1>" and "stencil <N>":
+ * N >= 0 when the function is the weighted stencil of radius N
+ *   float f(input_matrix_t m, __global float* w) { float s = 0.0f;
+ *     for (int dy = -N; dy <= N; ++dy) for (int dx = -N; dx <= N; ++dx)
+ *       s += getData(m, dx, dy) * w[(dy + N) * (2N+1) + (dx + N)];
+ *     return s; }
+ * (any names, token for token otherwise), which a float MapOverlap runs on
+ * its native stencil kernel for N <= 5, with the same results; else -1. */
 int skl_describe_user_function(const char* source, const char* func,
                                char** description);
 int skl_skeleton_extra_info(skl_skeleton s, int index, int* kind, size_t* size,

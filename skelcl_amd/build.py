@@ -35,7 +35,9 @@ TEMPLATES = ["prelude", "map", "reduce", "scan", "mapoverlap", "allpairs"]
 # the default (AGPR) form the register allocator gives each MFMA a result
 # register other than its accumulator input and copies the accumulators
 # round the k-loop (hundreds of v_accvgpr moves per slice).
-EXTRA = {"matmul.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form=1"]}
+EXTRA = {"matmul.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form=1"],
+         # the weighted-stencil forward fuses only where it calls fmaf
+         "stencil.hip": ["-ffp-contract=off"]}
 
 
 def _embed() -> str:

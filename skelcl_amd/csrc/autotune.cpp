@@ -81,7 +81,9 @@ void apply(skl_skeleton_s* s, const Shape& sh)
   s->scan_j = sh.scan_j;
   s->mo_rb = sh.mo_rb;
   s->max_blocks = cus * sh.blocks_per_cu;
-  if (changed) rebuild_programs(s);  // the shape is compiled into the kernels
+  // the shape is compiled into the kernels; the native stencil kernel takes
+  // its rows per thread at launch and never runs the generated one
+  if (changed && s->stencil_radius < 0) rebuild_programs(s);
 }
 
 std::vector<Shape> candidates(const skl_skeleton_s* s)
@@ -99,7 +101,7 @@ std::vector<Shape> candidates(const skl_skeleton_s* s)
       for (int j : {2, 4, 8, 16}) c.push_back(Shape{d.unroll, d.blocks_per_cu, j, d.mo_rb});
       break;
     case SKL_MAPOVERLAP:
-      for (int rb : {1, 2, 4, 8}) c.push_back(Shape{d.unroll, d.blocks_per_cu, d.scan_j, rb});
+      for (int rb : {1, 2, 4, 6, 8}) c.push_back(Shape{d.unroll, d.blocks_per_cu, d.scan_j, rb});
       break;
     default:
       c.push_back(d);

@@ -3,6 +3,7 @@
 #include "fusion.h"
 
 #include <cctype>
+#include <cstdlib>
 #include <map>
 
 namespace skl {
@@ -343,6 +344,146 @@ UserOp classify_binary(const UserFunction& f)
       is("fminf(" + a + "," + b + ")") || is(a + "<" + b + "?" + a + ":" + b))
     return UserOp::Min;
   return UserOp::Other;
+}
+
+int match_weighted_stencil(const std::string& src, const UserFunction& f)
+{
+  for (auto& t : tokenize(strip_comments(src)))
+    if (t.kind == Tok::Preproc) return -1;
+  if (collapse_ws(f.return_type) != "float" || f.params.size() != 2) return -1;
+  const Param& pi = f.params[0];
+  const Param& pw = f.params[1];
+  if (collapse_ws(pi.type_text) != "input_matrix_t" || pi.name.empty()) return -1;
+  if (pw.kind != ParamKind::Buffer || collapse_ws(pw.pointee) != "float" || pw.name.empty())
+    return -1;
+  for (auto& t : tokenize(pw.type_text))
+    if (t.text != "__global" && t.text != "global" && t.text != "const" && t.text != "float" &&
+        t.text != "*")
+      return -1;
+  const std::vector<Tok> tk = tokenize(f.body);
+  size_t at = 0;
+  auto lit = [&](const char* x) {
+    if (at < tk.size() && tk[at].text == x) {
+      ++at;
+      return true;
+    }
+    return false;
+  };
+  auto seq = [&](std::initializer_list<const char*> xs) {
+    const size_t save = at;
+    for (const char* x : xs)
+      if (!lit(x)) {
+        at = save;
+        return false;
+      }
+    return true;
+  };
+  auto ident = [&](std::string* out) {
+    if (at < tk.size() && tk[at].kind == Tok::Ident) {
+      *out = tk[at++].text;
+      return true;
+    }
+    return false;
+  };
+  auto number = [&](long* out) {
+    if (at >= tk.size() || tk[at].kind != Tok::Number) return false;
+    const std::string& x = tk[at].text;
+    for (char c : x)
+      if (!isdigit((unsigned char)c)) return false;
+    *out = std::strtol(x.c_str(), nullptr, 10);
+    ++at;
+    return true;
+  };
+  auto zero = [&]() {
+    if (at >= tk.size() || tk[at].kind != Tok::Number) return false;
+    std::string x = tk[at].text;
+    if (!x.empty() && (x.back() == 'f' || x.back() == 'F')) x.pop_back();
+    if (x.empty() || x.find_first_not_of("0.") != std::string::npos ||
+        x.find_first_of("0") == std::string::npos)
+      return false;
+    ++at;
+    return true;
+  };
+  std::string S, Y, X, I, Wn;
+  long n = -1, m = 0;
+  // float S = 0.0f ;
+  if (!lit("float") || !ident(&S) || !lit("=") || !zero() || !lit(";")) return -1;
+  // for ( int V = - N ; V <= N ; ++V | V++ )
+  auto loop = [&](std::string* v) {
+    long a, b;
+    std::string v2, v3;
+    if (!seq({"for", "(", "int"}) || !ident(v) || !lit("=") || !lit("-") || !number(&a) ||
+        !lit(";") || !ident(&v2) || v2 != *v || !lit("<=") || !number(&b) || !lit(";"))
+      return false;
+    if (a != b || (n >= 0 && a != n)) return false;
+    n = a;
+    if (lit("++")) {
+      if (!ident(&v3) || v3 != *v) return false;
+    } else if (!ident(&v3) || v3 != *v || !lit("++")) {
+      return false;
+    }
+    return lit(")");
+  };
+  if (!loop(&Y)) return -1;
+  const bool b1 = lit("{");
+  if (!loop(&X)) return -1;
+  const bool b2 = lit("{");
+  // S += P ;  or  S = S + P ;
+  std::string s2;
+  if (!ident(&s2) || s2 != S) return -1;
+  if (!lit("+=")) {
+    std::string s3;
+    if (!lit("=") || !ident(&s3) || s3 != S || !lit("+")) return -1;
+  }
+  auto get = [&]() {
+    std::string g, i, x, y;
+    if (!ident(&g) || g != "getData" || !lit("(") || !ident(&i) || !lit(",") || !ident(&x) ||
+        !lit(",") || !ident(&y) || !lit(")"))
+      return false;
+    return i == pi.name && x == X && y == Y;
+  };
+  auto plus_n = [&](const std::string& v, bool paren) {
+    std::string u;
+    long k;
+    if (paren && !lit("(")) return false;
+    if (!ident(&u) || u != v || !lit("+") || !number(&k) || k != n) return false;
+    return !paren || lit(")");
+  };
+  auto weight = [&]() {
+    std::string wn;
+    if (!ident(&wn) || wn != pw.name || !lit("[")) return false;
+    const size_t save = at;
+    bool ok = plus_n(Y, true) && lit("*") && number(&m);
+    if (!ok) {
+      at = save;
+      ok = number(&m) && lit("*") && plus_n(Y, true);
+    }
+    if (!ok || m != 2 * n + 1 || !lit("+")) return false;
+    const size_t save2 = at;
+    if (!plus_n(X, true)) {
+      at = save2;
+      if (!plus_n(X, false)) return false;
+    }
+    return lit("]");
+  };
+  const size_t pstart = at;
+  if (!(get() && lit("*") && weight())) {
+    at = pstart;
+    if (!(weight() && lit("*") && get())) return -1;
+  }
+  if (!lit(";")) return -1;
+  if (b2 && !lit("}")) return -1;
+  if (b1 && !lit("}")) return -1;
+  std::string s4;
+  if (!lit("return") || !ident(&s4) || s4 != S || !lit(";") || at != tk.size()) return -1;
+  // distinct names, none a keyword the pattern relies on
+  const std::string names[] = {S, X, Y, pi.name, pw.name};
+  for (int a = 0; a < 5; ++a) {
+    if (names[a] == "getData" || names[a] == "float" || names[a] == "int") return -1;
+    for (int b = a + 1; b < 5; ++b)
+      if (names[a] == names[b]) return -1;
+  }
+  return (int)n;
 }
 
 bool uses_work_item_builtins(const std::string& src)

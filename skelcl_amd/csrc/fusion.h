@@ -48,6 +48,21 @@ bool parse_user_function(const std::string& src, const std::string& name,
 // Recognises `return a+b;`-style bodies of two-parameter functions.
 UserOp classify_binary(const UserFunction& f);
 
+// Recognises the weighted-stencil MapOverlap user function, token for token:
+//   float F(input_matrix_t IMG, __global float* W) {
+//     float S = 0.0f;
+//     for (int Y = -N; Y <= N; ++Y)
+//       for (int X = -N; X <= N; ++X)
+//         S += getData(IMG, X, Y) * W[(Y + N) * (2N+1) + (X + N)];
+//     return S;
+//   }
+// (any names; braces around the loop bodies optional; the product's
+// operands in either order; `S = S + ...` for `+=`; `Y++` for `++Y`; a zero
+// literal of any spelling; `const` on the weights).  Returns N, or -1 for any
+// other source -- also for one with preprocessor lines, which could change
+// what the tokens mean.
+int match_weighted_stencil(const std::string& src, const UserFunction& f);
+
 // True when the source calls work-item functions, barriers or uses __local,
 // i.e. it needs the reference's one-element-per-work-item launch.
 bool uses_work_item_builtins(const std::string& src);

@@ -1,0 +1,207 @@
+// MapOverlap forward for user functions of the weighted-stencil form
+//   float func(input_matrix_t img, __global float* w) {
+//     float sum = 0.0f;
+//     for (int dy = -R; dy <= R; ++dy)
+//       for (int dx = -R; dx <= R; ++dx)
+//         sum += getData(img, dx, dy) * w[(dy + R) * (2R+1) + (dx + R)];
+//     return sum;
+//   }
+// which the skeleton layer recognises token for token.  Every output takes
+// the same sequence of operations as that loop: dy ascending, dx ascending,
+// one fused multiply-add per term from 0.0f (or, without contraction, a
+// rounded product and a rounded sum: this file is built with
+// -ffp-contract=off and fuses only through fmaf), so the results are
+// bit-identical to the generic fused kernel's.
+//
+// The generic kernel runs the loop as written for several rows per thread;
+// the compiler then keeps all (2R+1)^2 weights live in scalar registers,
+// which do not hold 121 of them, and spends a third of its instructions
+// moving spilled weights back (v_readlane) -- the kernel is bound by VALU
+// issue.  Here a work-group of 64 x 4 threads computes a 64 x 4RB tile, RB
+// rows of one column per thread, streaming the tile rows: at step t a thread
+// reads tile row t of its window (2R+1 values) once and feeds it to every
+// output whose window holds it (weight row t - i for output i), and weight
+// row t enters at step t.  So only RB weight rows are live at a time, in
+// scalar registers that the FMAs read directly.
+//
+// The tile is staged row by row: wave v loads tile rows v, v+4, ..., each
+// lane one column (and the 2R halo columns past 64 in a second load), from
+// clamped coordinates (NEAREST) with the neutral value outside (NEUTRAL),
+// every load in flight before the LDS writes.
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "skelcl_rt.h"
+
+#ifndef SKL_STENCIL_LAG
+#define SKL_STENCIL_LAG 1
+#endif
+
+namespace {
+
+template <int R, int RB, bool NEUTRAL, bool FUSED>
+__global__ __launch_bounds__(256) void stencil_forward_kernel(const float* __restrict__ in,
+                                                              float* __restrict__ out,
+                                                              const float* __restrict__ w,
+                                                              int rows, int cols, float neutral)
+{
+  constexpr int W = 2 * R + 1;
+  constexpr int TW = 64, TH = 4 * RB, TWP = TW + 2 * R, THP = TH + 2 * R;
+  constexpr int NR = (THP + 3) / 4;  // tile rows per wave
+  __shared__ float tile[THP * TWP];
+  const int lane = threadIdx.x & 63;
+  // uniform per wave: row addresses are scalar and loads/stores take a
+  // scalar base with a 32-bit lane offset
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int c0 = blockIdx.x * TW, r0 = blockIdx.y * TH;
+
+  // ---- stage the tile
+  const int gc0 = c0 - R + lane, gc1 = gc0 + 64;
+  const int cc0 = gc0 < 0 ? 0 : (gc0 >= cols ? cols - 1 : gc0);
+  const int cc1 = gc1 < 0 ? 0 : (gc1 >= cols ? cols - 1 : gc1);
+  const bool halo = lane < 2 * R;
+  float a[NR], b[NR];
+#pragma unroll
+  for (int k = 0; k < NR; ++k) {
+    const int tr = wv + 4 * k < THP ? wv + 4 * k : THP - 1;
+    const int gr = r0 - R + tr;
+    const int cr = gr < 0 ? 0 : (gr >= rows ? rows - 1 : gr);
+    const float* row = in + (size_t)cr * (size_t)cols;
+    a[k] = row[(unsigned)cc0];
+    b[k] = halo ? row[(unsigned)cc1] : 0.0f;
+    if (NEUTRAL) {
+      if (gr != cr || gc0 != cc0) a[k] = neutral;
+      if (gr != cr || gc1 != cc1) b[k] = neutral;
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < NR; ++k) {
+    const int tr = wv + 4 * k;
+    if (tr < THP) {
+      tile[tr * TWP + lane] = a[k];
+      if (halo) tile[tr * TWP + 64 + lane] = b[k];
+    }
+  }
+  __syncthreads();
+
+  // ---- RB outputs per thread, tile rows streamed
+  float acc[RB];
+#pragma unroll
+  for (int i = 0; i < RB; ++i) acc[i] = 0.0f;
+  float wr[W][W];
+  const char* win = (const char*)(tile + (wv * RB) * TWP + lane);
+  // The loads of step t (weight row t, tile row t) take byte offsets that are
+  // opaque to the compiler and become available once every accumulator has
+  // finished step t - LAG.  Without that the scheduler runs one output's
+  // chain far ahead of the others (they are independent), and the weight
+  // rows the lagging outputs still need overflow the scalar registers; and
+  // an opaque tile offset costs one address add per step instead of one per
+  // LDS read.  So the outputs advance together, about RB + LAG weight rows
+  // are live, in scalar registers that the FMAs read directly.  (Constant
+  // address space: the weight loads stay scalar.)
+  constexpr int LAG = SKL_STENCIL_LAG;
+  unsigned zt[LAG + 1], zw[LAG + 1];
+#pragma unroll
+  for (int q = 0; q < LAG; ++q) {
+    zt[q] = q * TWP * 4;
+    zw[q] = q * W * 4;
+    asm volatile("" : "+s"(zt[q]), "+s"(zw[q]));
+  }
+#pragma unroll
+  for (int t = 0; t < RB + 2 * R; ++t) {
+    const int q = t % (LAG + 1);
+    if (t < W) {
+      const __attribute__((address_space(4))) float* wrow =
+          (const __attribute__((address_space(4))) float*)((unsigned long long)w + zw[q]);
+#pragma unroll
+      for (int j = 0; j < W; ++j) wr[t][j] = wrow[j];
+    }
+    float v[W];
+    const float* wt = (const float*)(win + zt[q]);
+#pragma unroll
+    for (int j = 0; j < W; ++j) v[j] = wt[j];
+#pragma unroll
+    for (int i = 0; i < RB; ++i) {
+      const int k = t - i;  // weight row (dy + R) of output i at this step
+      if (k < 0 || k >= W) continue;
+#pragma unroll
+      for (int j = 0; j < W; ++j)
+        acc[i] = FUSED ? __builtin_fmaf(v[j], wr[k][j], acc[i]) : acc[i] + v[j] * wr[k][j];
+    }
+    const int n = (t + LAG) % (LAG + 1);
+    zt[n] = (t + LAG) * TWP * 4;
+    zw[n] = (t + LAG) * W * 4;
+#pragma unroll
+    for (int i = 0; i < RB; ++i) asm volatile("" : "+s"(zt[n]), "+s"(zw[n]) : "v"(acc[i]));
+  }
+  const int c = c0 + lane;
+  if (c < cols) {
+#pragma unroll
+    for (int i = 0; i < RB; ++i) {
+      const int r = r0 + wv * RB + i;
+      if (r < rows) (out + (size_t)r * (size_t)cols)[(unsigned)c] = acc[i];
+    }
+  }
+}
+
+template <int R, int RB>
+hipError_t launch_rb(float* out, const float* in, const float* w, int rows, int cols,
+                     bool neutral_pad, float neutral, bool fused, hipStream_t s)
+{
+  dim3 grid((cols + 63) / 64, (rows + 4 * RB - 1) / (4 * RB)), block(256);
+  if (neutral_pad) {
+    if (fused)
+      hipLaunchKernelGGL((stencil_forward_kernel<R, RB, true, true>), grid, block, 0, s, in, out,
+                         w, rows, cols, neutral);
+    else
+      hipLaunchKernelGGL((stencil_forward_kernel<R, RB, true, false>), grid, block, 0, s, in,
+                         out, w, rows, cols, neutral);
+  } else {
+    if (fused)
+      hipLaunchKernelGGL((stencil_forward_kernel<R, RB, false, true>), grid, block, 0, s, in,
+                         out, w, rows, cols, neutral);
+    else
+      hipLaunchKernelGGL((stencil_forward_kernel<R, RB, false, false>), grid, block, 0, s, in,
+                         out, w, rows, cols, neutral);
+  }
+  return hipGetLastError();
+}
+
+template <int R>
+hipError_t launch_r(float* out, const float* in, const float* w, int rows, int cols,
+                    bool neutral_pad, float neutral, bool fused, int rb, hipStream_t s)
+{
+  switch (rb) {
+    case 4: return launch_rb<R, 4>(out, in, w, rows, cols, neutral_pad, neutral, fused, s);
+    case 6: return launch_rb<R, 6>(out, in, w, rows, cols, neutral_pad, neutral, fused, s);
+    case 8: return launch_rb<R, 8>(out, in, w, rows, cols, neutral_pad, neutral, fused, s);
+  }
+  return hipErrorInvalidValue;
+}
+
+}  // namespace
+
+// Largest radius the stencil kernel is instantiated for.
+extern "C" int skl_internal_stencil_max_radius() { return 5; }
+
+// rb: rows per thread (4, 6 or 8).  Returns SKL_OK, or SKL_ERR_UNSUPPORTED
+// for a radius or rb without an instantiation.
+extern "C" int skl_internal_stencil_forward(float* out, const float* in, const float* w,
+                                            int rows, int cols, int R, int neutral_pad,
+                                            float neutral, int fused, int rb, hipStream_t s)
+{
+  if (rows <= 0 || cols <= 0) return rows < 0 || cols < 0 ? SKL_ERR_INVALID : SKL_OK;
+  hipError_t e = hipErrorInvalidValue;
+  switch (R) {
+    case 0: e = launch_r<0>(out, in, w, rows, cols, neutral_pad, neutral, fused, rb, s); break;
+    case 1: e = launch_r<1>(out, in, w, rows, cols, neutral_pad, neutral, fused, rb, s); break;
+    case 2: e = launch_r<2>(out, in, w, rows, cols, neutral_pad, neutral, fused, rb, s); break;
+    case 3: e = launch_r<3>(out, in, w, rows, cols, neutral_pad, neutral, fused, rb, s); break;
+    case 4: e = launch_r<4>(out, in, w, rows, cols, neutral_pad, neutral, fused, rb, s); break;
+    case 5: e = launch_r<5>(out, in, w, rows, cols, neutral_pad, neutral, fused, rb, s); break;
+    default: return SKL_ERR_UNSUPPORTED;
+  }
+  if (e == hipErrorInvalidValue) return SKL_ERR_UNSUPPORTED;
+  return e == hipSuccess ? SKL_OK : SKL_ERR_HIP;
+}

@@ -21,6 +21,10 @@ struct skl_skeleton_s {
   std::string common;  // common definitions registered when it was created
   std::string contract = "on";  // -ffp-contract of the user functions
   int stencil_checked = 0;  // MapOverlap: 1 = a weighted stencil, -1 = not, 0 = unknown
+  // MapOverlap whose user function is the weighted stencil of this radius
+  // (match_weighted_stencil), run by the native stencil kernel; -1 = not
+  int stencil_radius = -1;
+  float stencil_neutral = 0.0f;  // the neutral value, as the generated code rounds it
   std::vector<TypeInfo> types;  // kind-specific order, see skelcl_skeletons.h
   int overlap = 1, padding = SKL_PADDING_NEAREST;
   int wg = 256;

@@ -21,6 +21,10 @@ extern "C" const char* skl_tmpl_map;
 extern "C" const char* skl_tmpl_reduce;
 extern "C" const char* skl_tmpl_scan;
 extern "C" const char* skl_tmpl_mapoverlap;
+extern "C" int skl_internal_stencil_max_radius();
+extern "C" int skl_internal_stencil_forward(float* out, const float* in, const float* w,
+                                            int rows, int cols, int R, int neutral_pad,
+                                            float neutral, int fused, int rb, hipStream_t s);
 extern "C" const char* skl_tmpl_allpairs;
 
 using namespace skl;
@@ -571,6 +575,55 @@ static std::vector<void*> with_extras(std::initializer_list<void*> fixed, const 
 // ---------------------------------------------------------------------------
 // creation
 // ---------------------------------------------------------------------------
+// A literal as the generated code rounds it to float: a decimal with an
+// f suffix rounds once, to float; without, to double and then to float.
+static bool float_literal(const std::string& text, float* out)
+{
+  std::string t;
+  for (char c : text)
+    if (!isspace((unsigned char)c)) t += c;
+  bool f = !t.empty() && (t.back() == 'f' || t.back() == 'F');
+  if (f) t.pop_back();
+  if (t.empty() || t.find_first_not_of("0123456789.eE+-") != std::string::npos) return false;
+  char* end = nullptr;
+  if (f) {
+    *out = std::strtof(t.c_str(), &end);
+  } else {
+    *out = (float)std::strtod(t.c_str(), &end);
+  }
+  return end && *end == '\0';
+}
+
+// A float MapOverlap whose user function is the weighted stencil, compiled
+// with nothing that could change its arithmetic (no macros in the common
+// definitions, no build options but -ffp-contract), runs on the native
+// stencil kernel, which performs every output's operations in the loop's
+// order (native/stencil.hip): same results as the generated kernel.
+static void detect_native_stencil(skl_skeleton_s* s)
+{
+  s->stencil_radius = -1;
+  if (s->types.size() != 2 || s->types[0].name != "float" || s->types[1].name != "float")
+    return;
+  if (s->common.find('#') != std::string::npos) return;
+  {
+    std::string o = s->options;
+    size_t at;
+    while ((at = o.find("-ffp-contract=")) != std::string::npos) {
+      size_t e = o.find_first_of(" \t", at);
+      o.erase(at, e == std::string::npos ? std::string::npos : e - at);
+    }
+    if (o.find_first_not_of(" \t") != std::string::npos) return;
+  }
+  const int n = match_weighted_stencil(s->src, s->uf);
+  if (n < 0 || n > s->overlap || n > skl_internal_stencil_max_radius()) return;
+  float neutral = 0.0f;
+  if (s->padding == SKL_PADDING_NEUTRAL && !float_literal(s->neutral, &neutral)) return;
+  s->stencil_radius = n;
+  s->stencil_neutral = neutral;
+  // the transposed stencil of this radius is the adjoint
+  if (n == s->overlap) s->stencil_checked = 1;
+}
+
 static int setup_skeleton(const skl_skeleton_desc* d, skl_skeleton_s* s)
 {
   if (!d) return fail(SKL_ERR_INVALID, "descriptor is NULL");
@@ -647,6 +700,7 @@ static int setup_skeleton(const skl_skeleton_desc* d, skl_skeleton_s* s)
   }
   s->op = classify_binary(s->uf);
   s->simple = uses_work_item_builtins(s->src);
+  if (s->kind == SKL_MAPOVERLAP) detect_native_stencil(s);
   if (d->deriv_source && *d->deriv_source) {
     s->deriv_src = d->deriv_source;
     s->deriv_func = str_or(d->deriv_func, "dfunc");
@@ -829,6 +883,7 @@ extern "C" int skl_describe_user_function(const char* source, const char* func,
   const char* ops[] = {"other", "add", "mul", "max", "min"};
   o << "op " << ops[(int)classify_binary(f)] << "\n";
   o << "work_item " << (uses_work_item_builtins(source) ? 1 : 0) << "\n";
+  o << "stencil " << match_weighted_stencil(source, f) << "\n";
   std::string s = o.str();
   *description = (char*)malloc(s.size() + 1);
   memcpy(*description, s.c_str(), s.size() + 1);
@@ -1218,6 +1273,19 @@ extern "C" int skl_mapoverlap_run(skl_skeleton s, void* out, const void* in, siz
   if (rows > 0x7FFFFFFF || cols > 0x7FFFFFFF || rows * cols > (1ull << 40))
     return fail(SKL_ERR_INVALID, "matrix too large");
   int r = (int)rows, c = (int)cols;
+  if (s->stencil_radius >= 0) {
+    // rows per thread: the tuned value, on the instantiated 4 / 6 / 8
+    const int rb = s->mo_rb >= 8 ? 8 : (s->mo_rb >= 6 ? 6 : 4);
+    if ((rows + 4 * rb - 1) / (4 * rb) > 65535)
+      return fail(SKL_ERR_INVALID, "matrix has too many rows");
+    if (!args[0].value) return fail(SKL_ERR_INVALID, "NULL weight buffer");
+    const int st = skl_internal_stencil_forward(
+        (float*)out, (const float*)in, (const float*)args[0].value, r, c, s->stencil_radius,
+        s->padding == SKL_PADDING_NEUTRAL ? 1 : 0, s->stencil_neutral,
+        s->contract == "off" ? 0 : 1, rb, resolve_stream(stream));
+    if (st != SKL_OK) return fail(st, "MapOverlap: stencil kernel launch failed");
+    return SKL_OK;
+  }
   skl_kernel k;
   SKL_TRY(get_kernel(s, "main", "skl_mapoverlap", &k));
   auto a = with_extras({&in, &out, &r, &c}, pk);

@@ -27,7 +27,68 @@ def describe(src, func="func"):
             out["op"] = rest
         elif key == "work_item":
             out["work_item"] = rest == "1"
+        elif key == "stencil":
+            out["stencil"] = int(rest)
     return out
+
+
+STENCIL = """float func(input_matrix_t img, __global float* w) {
+  float sum = 0.0f;
+  for (int dy = -5; dy <= 5; ++dy)
+    for (int dx = -5; dx <= 5; ++dx)
+      sum += getData(img, dx, dy) * w[(dy + 5) * 11 + (dx + 5)];
+  return sum;
+}"""
+
+
+def test_weighted_stencil_recognised():
+    """The weighted-stencil form (any names, either operand order, braces,
+    `s = s + p`, postfix increments, const weights, zero spellings) is
+    recognised with its radius; each listed change to the arithmetic or
+    the window is not."""
+    assert describe(STENCIL)["stencil"] == 5
+    same = [
+        STENCIL.replace("sum", "acc").replace("img", "m").replace("dy", "y").replace("dx", "x"),
+        STENCIL.replace("getData(img, dx, dy) * w[(dy + 5) * 11 + (dx + 5)]",
+                        "w[(dy + 5) * 11 + (dx + 5)] * getData(img, dx, dy)"),
+        STENCIL.replace("(dy + 5) * 11", "11 * (dy + 5)").replace("+ (dx + 5)]", "+ dx + 5]"),
+        STENCIL.replace("sum += ", "sum = sum + "),
+        STENCIL.replace("++dy", "dy++").replace("++dx", "dx++"),
+        STENCIL.replace("__global float* w", "const __global float* w"),
+        STENCIL.replace("0.0f", "0.f"),
+        STENCIL.replace("0.0f", "0"),
+        STENCIL.replace("    for (int dx", "  {  for (int dx").replace("  return", "  }\n  return"),
+        "/* lead */ " + STENCIL.replace("return sum;", "return sum; // out"),
+    ]
+    for src in same:
+        assert describe(src)["stencil"] == 5, src
+    three = (STENCIL.replace("-5", "-3").replace("<= 5", "<= 3").replace("+ 5)", "+ 3)")
+             .replace("* 11", "* 7"))
+    assert describe(three)["stencil"] == 3
+    zero = (STENCIL.replace("-5", "-0").replace("<= 5", "<= 0").replace("+ 5)", "+ 0)")
+            .replace("* 11", "* 1"))
+    assert describe(zero)["stencil"] == 0
+    other = [
+        STENCIL.replace("float sum = 0.0f", "float sum = 1.0f"),
+        STENCIL.replace("sum += ", "sum -= "),
+        STENCIL.replace("getData(img, dx, dy)", "getData(img, dy, dx)"),
+        STENCIL.replace("(dy + 5) * 11 + (dx + 5)", "(dx + 5) * 11 + (dy + 5)"),
+        STENCIL.replace("* 11", "* 12"),
+        STENCIL.replace("<= 5; ++dx", "<= 4; ++dx"),
+        STENCIL.replace("for (int dy = -5; dy <= 5; ++dy)\n    for (int dx = -5; dx <= 5; ++dx)",
+                        "for (int dx = -5; dx <= 5; ++dx)\n    for (int dy = -5; dy <= 5; ++dy)"),
+        STENCIL.replace("return sum;", "return sum * 2.0f;"),
+        STENCIL.replace("float func", "double func"),
+        STENCIL.replace("__global float* w", "__global double* w"),
+        STENCIL.replace("__global float* w", "__constant float* w"),
+        "#define SCALE 1\n" + STENCIL,
+        STENCIL.replace("__global float* w)", "__global float* w, int k)"),
+        STENCIL.replace("for (int dx", "for (int dy"),  # the inner loop shadows dy
+        STENCIL.replace("w[", "img["),
+    ]
+    for src in other:
+        assert describe(src)["stencil"] == -1, src
+    assert describe("float func(float a, float b) { return a + b; }")["stencil"] == -1
 
 
 # --- TransferParametersTest.cpp:46-117: the parameters from index k are the
