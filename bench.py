@@ -32,6 +32,14 @@ HBM_PEAK_GBS = 8000.0
 N = 1 << 28
 SEED_SCAN, SEED_RED, SEED_X, SEED_Y = 0x2001, 0x2002, 0x2003, 0x2004
 SAXPY_A = 2.5
+# the 11x11 weighted stencil of the MapOverlap configuration
+GAUSS_SRC = """float func(input_matrix_t img, __global float* w) {
+  float sum = 0.0f;
+  for (int dy = -5; dy <= 5; ++dy)
+    for (int dx = -5; dx <= 5; ++dx)
+      sum += getData(img, dx, dy) * w[(dy + 5) * 11 + (dx + 5)];
+  return sum;
+}"""
 
 
 def progress(msg: str) -> None:
@@ -107,231 +115,249 @@ def run_extras(skelcl, L, ck, stream, args):
     import oracle
     st = stream.handle
     ex = {}
+    only = set(args.only.split(",")) if args.only else None
+
+    def want(name):
+        return only is None or name in only
+
+    add = skelcl.Reduce("float(float)", "float func(float x, float y){ return x+y; }", "0")
 
     # ---- configuration 1: dot product N=2^20 (Zip x*y then Reduce +), int Reduce
-    n1 = 1 << 20
-    x1, y1 = oracle.gen_f32_sym(0x1001, n1), oracle.gen_f32_sym(0x1002, n1)
-    X1, Y1 = skelcl.DeviceBuffer.from_array(x1), skelcl.DeviceBuffer.from_array(y1)
-    T1, o1 = skelcl.DeviceBuffer(4 * n1), skelcl.DeviceBuffer(16)
-    mult = skelcl.Zip("float(float, float)", "float func(float x, float y){ return x*y; }")
-    add = skelcl.Reduce("float(float)", "float func(float x, float y){ return x+y; }", "0")
-    addi = skelcl.Reduce("int(int)", "int func(int x, int y){ return x+y; }", "0")
-    k1 = oracle.gen_i32_below(0x1003, n1, 1000)
-    K1 = skelcl.DeviceBuffer.from_array(k1)
+    if want("dot"):
+        n1 = 1 << 20
+        x1, y1 = oracle.gen_f32_sym(0x1001, n1), oracle.gen_f32_sym(0x1002, n1)
+        X1, Y1 = skelcl.DeviceBuffer.from_array(x1), skelcl.DeviceBuffer.from_array(y1)
+        T1, o1 = skelcl.DeviceBuffer(4 * n1), skelcl.DeviceBuffer(16)
+        mult = skelcl.Zip("float(float, float)", "float func(float x, float y){ return x*y; }")
+        addi = skelcl.Reduce("int(int)", "int func(int x, int y){ return x+y; }", "0")
+        k1 = oracle.gen_i32_below(0x1003, n1, 1000)
+        K1 = skelcl.DeviceBuffer.from_array(k1)
 
-    def dot_unfused():
-        ck(L.skl_zip_run(mult.handle, T1.ptr, X1.ptr, Y1.ptr, n1, None, 0, st))
-        ck(L.skl_reduce_run(add.handle, o1.ptr, T1.ptr, n1, None, 0, st))
+        def dot_unfused():
+            ck(L.skl_zip_run(mult.handle, T1.ptr, X1.ptr, Y1.ptr, n1, None, 0, st))
+            ck(L.skl_reduce_run(add.handle, o1.ptr, T1.ptr, n1, None, 0, st))
 
-    def dot_fused():
-        ck(L.skl_zip_reduce_run(mult.handle, add.handle, o1.ptr, X1.ptr, Y1.ptr, n1, st))
+        def dot_fused():
+            ck(L.skl_zip_reduce_run(mult.handle, add.handle, o1.ptr, X1.ptr, Y1.ptr, n1, st))
 
-    t_unf = _time(stream, dot_unfused, reps=20)
-    got_unf = float(o1.to_numpy(np.float32, 1)[0])
-    t_fus = _time(stream, dot_fused, reps=20)
-    got_fus = float(o1.to_numpy(np.float32, 1)[0])
-    # graph-captured fused pipeline
-    g = C.c_void_p()
-    ck(L.skl_graph_begin_capture(st))
-    dot_fused()
-    ck(L.skl_graph_end_capture(st, C.byref(g)))
-    t_graph = _time(stream, lambda: ck(L.skl_graph_launch(g, st)), reps=20)
-    got_graph = float(o1.to_numpy(np.float32, 1)[0])
-    ck(L.skl_graph_destroy(g))
-    t_int = _time(stream, lambda: ck(L.skl_reduce_run(addi.handle, o1.ptr, K1.ptr, n1, None, 0, st)),
-                  reps=20)
-    got_int = int(o1.to_numpy(np.int32, 1)[0])
-    ref = oracle.dot_f64(x1, y1)
-    ex["dot_2^20"] = {
-        "unfused_ms": round(t_unf, 4), "fused_ms": round(t_fus, 4), "graph_ms": round(t_graph, 4),
-        "fused_GBps": round(8 * n1 / (t_fus * 1e-3) / 1e9, 1),
-        "err_vs_fp64": {"unfused": abs(got_unf - ref), "fused": abs(got_fus - ref),
-                        "graph": abs(got_graph - ref),
-                        "cpu_reference_order": abs(float(oracle.reduce_add_f32_reforder(
-                            oracle.zip_mul_f32(x1, y1))) - ref)},
-        "int_reduce_ms": round(t_int, 4),
-        "int_reduce_exact": bool(np.uint32(got_int & 0xFFFFFFFF)
-                                 == oracle.reduce_add_u32_reforder(k1)),
-    }
-    progress("dot_2^20 done")
+        t_unf = _time(stream, dot_unfused, reps=20)
+        got_unf = float(o1.to_numpy(np.float32, 1)[0])
+        t_fus = _time(stream, dot_fused, reps=20)
+        got_fus = float(o1.to_numpy(np.float32, 1)[0])
+        # graph-captured fused pipeline
+        g = C.c_void_p()
+        ck(L.skl_graph_begin_capture(st))
+        dot_fused()
+        ck(L.skl_graph_end_capture(st, C.byref(g)))
+        t_graph = _time(stream, lambda: ck(L.skl_graph_launch(g, st)), reps=20)
+        got_graph = float(o1.to_numpy(np.float32, 1)[0])
+        ck(L.skl_graph_destroy(g))
+        t_int = _time(stream, lambda: ck(L.skl_reduce_run(addi.handle, o1.ptr, K1.ptr, n1, None,
+                                                          0, st)), reps=20)
+        got_int = int(o1.to_numpy(np.int32, 1)[0])
+        ref = oracle.dot_f64(x1, y1)
+        ex["dot_2^20"] = {
+            "unfused_ms": round(t_unf, 4), "fused_ms": round(t_fus, 4),
+            "graph_ms": round(t_graph, 4),
+            "fused_GBps": round(8 * n1 / (t_fus * 1e-3) / 1e9, 1),
+            "err_vs_fp64": {"unfused": abs(got_unf - ref), "fused": abs(got_fus - ref),
+                            "graph": abs(got_graph - ref),
+                            "cpu_reference_order": abs(float(oracle.reduce_add_f32_reforder(
+                                oracle.zip_mul_f32(x1, y1))) - ref)},
+            "int_reduce_ms": round(t_int, 4),
+            "int_reduce_exact": bool(np.uint32(got_int & 0xFFFFFFFF)
+                                     == oracle.reduce_add_u32_reforder(k1)),
+        }
+        progress("dot_2^20 done")
 
     # ---- host buffers handed over: saxpy 2^26 through the Vector API
-    # (upload, kernel, download; pageable numpy memory), wall clock
-    n2 = 1 << 26
-    hx, hy = oracle.gen_f32_sym(SEED_X, n2), oracle.gen_f32_sym(SEED_Y, n2)
-    sax_v = skelcl.Zip("float(float, float)",
-                       "float func(float x, float y, float a){ return a*x+y; }")
-    np.asarray(sax_v(skelcl.Vector(hx[:1024]), skelcl.Vector(hy[:1024]), np.float32(SAXPY_A)))
-    walls = []
-    for _ in range(3):
-        t0 = time.perf_counter()
-        res = np.asarray(sax_v(skelcl.Vector(hx), skelcl.Vector(hy), np.float32(SAXPY_A)))
-        walls.append(time.perf_counter() - t0)
-    w = min(walls)
-    ex["saxpy_2^26_host_buffers"] = {
-        "wall_ms": round(w * 1e3, 3), "GBps_algorithmic": round(12 * n2 / w / 1e9, 1),
-        "bit_exact_vs_fma": bool(np.array_equal(res, oracle.saxpy_f32_fma(SAXPY_A, hx, hy))),
-    }
-    progress("saxpy_2^26_host_buffers done")
-    del hx, hy, res
+    if want("saxpy_host"):
+        # (upload, kernel, download; pageable numpy memory), wall clock
+        n2 = 1 << 26
+        hx, hy = oracle.gen_f32_sym(SEED_X, n2), oracle.gen_f32_sym(SEED_Y, n2)
+        sax_v = skelcl.Zip("float(float, float)",
+                           "float func(float x, float y, float a){ return a*x+y; }")
+        np.asarray(sax_v(skelcl.Vector(hx[:1024]), skelcl.Vector(hy[:1024]), np.float32(SAXPY_A)))
+        walls = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            res = np.asarray(sax_v(skelcl.Vector(hx), skelcl.Vector(hy), np.float32(SAXPY_A)))
+            walls.append(time.perf_counter() - t0)
+        w = min(walls)
+        ex["saxpy_2^26_host_buffers"] = {
+            "wall_ms": round(w * 1e3, 3), "GBps_algorithmic": round(12 * n2 / w / 1e9, 1),
+            "bit_exact_vs_fma": bool(np.array_equal(res, oracle.saxpy_f32_fma(SAXPY_A, hx, hy))),
+        }
+        progress("saxpy_2^26_host_buffers done")
+        del hx, hy, res
 
     # ---- configuration 3: MapOverlap 8192^2 Gaussian R=5 NEUTRAL + adjoint
-    R, S = 5, 8192
-    w = oracle.gaussian_weights(R, 2.0)
-    img = skelcl.DeviceBuffer(4 * S * S)
-    Wb = skelcl.DeviceBuffer(4 * S * S)
-    ck(L.skl_generate_f32_unit(img.ptr, S * S, 0x3001, 255.0, st))
-    ck(L.skl_generate_f32_sym(Wb.ptr, S * S, 0x3002, st))
-    outb, dinb = skelcl.DeviceBuffer(4 * S * S), skelcl.DeviceBuffer(4 * S * S)
-    wv = skelcl.DeviceBuffer.from_array(w.ravel())
-    gauss = skelcl.MapOverlap("float(float)", """float func(input_matrix_t img, __global float* w) {
-  float sum = 0.0f;
-  for (int dy = -5; dy <= 5; ++dy)
-    for (int dx = -5; dx <= 5; ++dx)
-      sum += getData(img, dx, dy) * w[(dy + 5) * 11 + (dx + 5)];
-  return sum;
-}""", R, skelcl.NEUTRAL, 0)
-    warg, nw, keep, _ = gauss._marshal((wv,))
-    if args.autotune:
-        ck(L.skl_autotune_args(gauss.handle, S * S, warg, nw))
-    t_mo = _time(stream, lambda: ck(L.skl_mapoverlap_run(gauss.handle, outb.ptr, img.ptr, S, S,
-                                                         warg, nw, st)), reps=5)
-    t_adj = _time(stream, lambda: ck(L.skl_mapoverlap_stencil_backward(
-        gauss.handle, dinb.ptr, Wb.ptr, wv.ptr, S, S, st)), reps=5)
-    himg = oracle.gen_f32_unit(0x3001, S * S, 255.0).reshape(S, S)
-    rng = np.random.default_rng(0x3003)
-    pix = rng.integers(0, S * S, 64).astype(np.int64)
-    o = outb.to_numpy(np.float32)
-    ref = oracle.mapoverlap_stencil_f64(himg, w, R, pix, 0)
-    hW = oracle.gen_f32_sym(0x3002, S * S).reshape(S, S)
-    dref = oracle.mapoverlap_stencil_adjoint_f64(hW, w, R, pix, 0)
-    d = dinb.to_numpy(np.float32)
-    flop = 2 * (2 * R + 1) ** 2 * S * S
-    ex["mapoverlap_8192^2_r5"] = {
-        "forward_ms": round(t_mo, 4), "forward_GBps": round(8 * S * S / (t_mo * 1e-3) / 1e9, 1),
-        "forward_hbm_frac": round(8 * S * S / (t_mo * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
-        "forward_TFLOPs": round(flop / (t_mo * 1e-3) / 1e12, 2),
-        "forward_fp32_frac_of_157.3": round(flop / (t_mo * 1e-3) / 1e12 / 157.3, 4),
-        "adjoint_ms": round(t_adj, 4),
-        "max_rel_err_vs_fp64": float(np.max(np.abs(o[pix] - ref) / np.maximum(1, np.abs(ref)))),
-        "adjoint_max_abs_err_vs_fp64": float(np.max(np.abs(d[pix] - dref))),
-    }
-    progress("mapoverlap_8192^2_r5 done")
-    del img, Wb, outb, dinb, o, d, himg, hW
+    if want("mapoverlap"):
+        R, S = 5, 8192
+        w = oracle.gaussian_weights(R, 2.0)
+        img = skelcl.DeviceBuffer(4 * S * S)
+        Wb = skelcl.DeviceBuffer(4 * S * S)
+        ck(L.skl_generate_f32_unit(img.ptr, S * S, 0x3001, 255.0, st))
+        ck(L.skl_generate_f32_sym(Wb.ptr, S * S, 0x3002, st))
+        outb, dinb = skelcl.DeviceBuffer(4 * S * S), skelcl.DeviceBuffer(4 * S * S)
+        wv = skelcl.DeviceBuffer.from_array(w.ravel())
+        gauss = skelcl.MapOverlap("float(float)", GAUSS_SRC, R, skelcl.NEUTRAL, 0)
+        # the same user function behind a preprocessor line: not recognised
+        # as the weighted stencil, so it runs the generated MapOverlap kernel
+        gauss_gen = skelcl.MapOverlap("float(float)", "#define GENERIC 1\n" + GAUSS_SRC, R,
+                                      skelcl.NEUTRAL, 0)
+        warg, nw, keep, _ = gauss._marshal((wv,))
+        if args.autotune:
+            ck(L.skl_autotune_args(gauss.handle, S * S, warg, nw))
+        t_gen = _time(stream, lambda: ck(L.skl_mapoverlap_run(gauss_gen.handle, outb.ptr, img.ptr,
+                                                              S, S, warg, nw, st)), reps=5)
+        o_gen = outb.to_numpy(np.float32)
+        t_mo = _time(stream, lambda: ck(L.skl_mapoverlap_run(gauss.handle, outb.ptr, img.ptr, S, S,
+                                                             warg, nw, st)), reps=5)
+        t_adj = _time(stream, lambda: ck(L.skl_mapoverlap_stencil_backward(
+            gauss.handle, dinb.ptr, Wb.ptr, wv.ptr, S, S, st)), reps=5)
+        himg = oracle.gen_f32_unit(0x3001, S * S, 255.0).reshape(S, S)
+        rng = np.random.default_rng(0x3003)
+        pix = rng.integers(0, S * S, 64).astype(np.int64)
+        o = outb.to_numpy(np.float32)
+        ref = oracle.mapoverlap_stencil_f64(himg, w, R, pix, 0)
+        hW = oracle.gen_f32_sym(0x3002, S * S).reshape(S, S)
+        dref = oracle.mapoverlap_stencil_adjoint_f64(hW, w, R, pix, 0)
+        d = dinb.to_numpy(np.float32)
+        flop = 2 * (2 * R + 1) ** 2 * S * S
+        ex["mapoverlap_8192^2_r5"] = {
+            "forward_ms": round(t_mo, 4), "forward_GBps": round(8 * S * S / (t_mo * 1e-3) / 1e9, 1),
+            "forward_hbm_frac": round(8 * S * S / (t_mo * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+            "forward_TFLOPs": round(flop / (t_mo * 1e-3) / 1e12, 2),
+            "forward_fp32_frac_of_157.3": round(flop / (t_mo * 1e-3) / 1e12 / 157.3, 4),
+            # FP32 FMAs on the VALU without packed math: half the 157.3 peak
+            "forward_valu_fma_frac_of_78.6": round(flop / (t_mo * 1e-3) / 1e12 / 78.6, 4),
+            "generated_kernel_ms": round(t_gen, 4),
+            "bit_exact_vs_generated_kernel": bool(np.array_equal(o.view(np.uint32),
+                                                                 o_gen.view(np.uint32))),
+            "adjoint_ms": round(t_adj, 4),
+            "max_rel_err_vs_fp64": float(np.max(np.abs(o[pix] - ref) / np.maximum(1, np.abs(ref)))),
+            "adjoint_max_abs_err_vs_fp64": float(np.max(np.abs(d[pix] - dref))),
+        }
+        progress("mapoverlap_8192^2_r5 done")
+        del img, Wb, outb, dinb, o, o_gen, d, himg, hW
 
     # ---- configuration 4: AllPairs 8192^3 (FP32 / BF16X3 / FP8), sq-Euclidean
-    M = 8192
-    A = skelcl.DeviceBuffer(4 * M * M)
-    B = skelcl.DeviceBuffer(4 * M * M)
-    Cb = skelcl.DeviceBuffer(4 * M * M)
-    ck(L.skl_generate_f32_sym(A.ptr, M * M, 0x4001, st))
-    ck(L.skl_generate_f32_sym(B.ptr, M * M, 0x4002, st))
-    hA = oracle.gen_f32_sym(0x4001, M * M).reshape(M, M)
-    hB = oracle.gen_f32_sym(0x4002, M * M).reshape(M, M)
-    rows = np.sort(np.random.default_rng(7).choice(M, 32, replace=False)).astype(np.int32)
-    ref = oracle.allpairs_mm_f64(hA, hB, rows=rows)
-    flop = 2.0 * M * M * M
-    ap = {}
-    for name, prec, peak in (("fp32", skelcl.PREC_FP32, 157.3), ("bf16x3", skelcl.PREC_BF16X3, 2500.0),
-                             ("fp8", skelcl.PREC_FP8, 5000.0)):
-        t = _time(stream, lambda: ck(L.skl_matmul_f32(Cb.ptr, A.ptr, B.ptr, M, M, M, 0, 0, prec, st)),
-                  reps=3)
-        c = Cb.to_numpy(np.float32).reshape(M, M)[rows].astype(np.float64)
-        err = float(np.linalg.norm(c - ref) / np.linalg.norm(ref))
-        tf = flop / (t * 1e-3) / 1e12
-        mfma_work = flop * (6 if name == "bf16x3" else 1)
-        ap[name] = {"ms": round(t, 3), "TFLOPs_effective": round(tf, 1),
-                    "unit_utilisation": round(mfma_work / (t * 1e-3) / 1e12 / peak, 4),
-                    "normwise_err_vs_fp64": err}
-    ap["bf16x3_speedup_vs_fp32"] = round(ap["fp32"]["ms"] / ap["bf16x3"]["ms"], 2)
-    dC = skelcl.DeviceBuffer(4 * M * M)
-    ck(L.skl_generate_f32_sym(dC.ptr, M * M, 0x4004, st))
-    zip_ = skelcl.Zip("float(float, float)", "float func(float x, float y){ return x*y; }")
-    apk = skelcl.AllPairs("float(float, float)", reduce=add, zip=zip_,
-                          precision=skelcl.PREC_BF16X3)
-    t_bw = _time(stream, lambda: ck(L.skl_allpairs_backward(apk.handle, A.ptr, B.ptr, Cb.ptr,
-                                                            Cb.ptr, dC.ptr, M, M, M, st)),
-                 reps=2)
-    ap["backward_bf16x3_ms"] = round(t_bw, 3)
-    del A, B, Cb, dC, hA, hB
-    n, d = 16384, 256
-    P = skelcl.DeviceBuffer(4 * n * d)
-    ck(L.skl_generate_f32_sym(P.ptr, n * d, 0x4003, st))
-    hP = oracle.gen_f32_sym(0x4003, n * d).reshape(n, d)
-    Pt = skelcl.DeviceBuffer.from_array(np.ascontiguousarray(hP.T))
-    D = skelcl.DeviceBuffer(4 * n * n)
-    sq = skelcl.AllPairs("float(float, float)",
-                         "float func(lmatrix_t* row, rmatrix_t* col, const unsigned int dim) {"
-                         " float s = 0; for (unsigned k = 0; k < dim; ++k) {"
-                         " float t = getElementFromRow(row, k) - getElementFromColumn(col, k);"
-                         " s += t*t; } return s; }")
-    t_sq = _time(stream, lambda: ck(L.skl_allpairs_run(sq.handle, D.ptr, P.ptr, Pt.ptr, n, n, d,
-                                                       None, 0, st)), reps=3)
-    rows = np.array([0, 1, 4097, 16383], np.int32)
-    dref = oracle.allpairs_sqdist_f64(hP, np.ascontiguousarray(hP.T), rows=rows)
-    dg = D.to_numpy(np.float32).reshape(n, n)[rows]
-    ap["sqeuclid_16384x256"] = {"ms": round(t_sq, 3),
-                               "TFLOPs": round(3.0 * n * n * d / (t_sq * 1e-3) / 1e12, 1),
-                               "max_rel_err_vs_fp64": float(np.max(np.abs(dg - dref) /
-                                                                   np.maximum(1, dref)))}
-    ex["allpairs_8192^3"] = ap
-    progress("allpairs_8192^3 done")
-    del P, Pt, D
+    if want("allpairs"):
+        M = 8192
+        A = skelcl.DeviceBuffer(4 * M * M)
+        B = skelcl.DeviceBuffer(4 * M * M)
+        Cb = skelcl.DeviceBuffer(4 * M * M)
+        ck(L.skl_generate_f32_sym(A.ptr, M * M, 0x4001, st))
+        ck(L.skl_generate_f32_sym(B.ptr, M * M, 0x4002, st))
+        hA = oracle.gen_f32_sym(0x4001, M * M).reshape(M, M)
+        hB = oracle.gen_f32_sym(0x4002, M * M).reshape(M, M)
+        rows = np.sort(np.random.default_rng(7).choice(M, 32, replace=False)).astype(np.int32)
+        ref = oracle.allpairs_mm_f64(hA, hB, rows=rows)
+        flop = 2.0 * M * M * M
+        ap = {}
+        for name, prec, peak in (("fp32", skelcl.PREC_FP32, 157.3), ("bf16x3", skelcl.PREC_BF16X3, 2500.0),
+                                 ("fp8", skelcl.PREC_FP8, 5000.0)):
+            t = _time(stream, lambda: ck(L.skl_matmul_f32(Cb.ptr, A.ptr, B.ptr, M, M, M, 0, 0, prec, st)),
+                      reps=3)
+            c = Cb.to_numpy(np.float32).reshape(M, M)[rows].astype(np.float64)
+            err = float(np.linalg.norm(c - ref) / np.linalg.norm(ref))
+            tf = flop / (t * 1e-3) / 1e12
+            mfma_work = flop * (6 if name == "bf16x3" else 1)
+            ap[name] = {"ms": round(t, 3), "TFLOPs_effective": round(tf, 1),
+                        "unit_utilisation": round(mfma_work / (t * 1e-3) / 1e12 / peak, 4),
+                        "normwise_err_vs_fp64": err}
+        ap["bf16x3_speedup_vs_fp32"] = round(ap["fp32"]["ms"] / ap["bf16x3"]["ms"], 2)
+        dC = skelcl.DeviceBuffer(4 * M * M)
+        ck(L.skl_generate_f32_sym(dC.ptr, M * M, 0x4004, st))
+        zip_ = skelcl.Zip("float(float, float)", "float func(float x, float y){ return x*y; }")
+        apk = skelcl.AllPairs("float(float, float)", reduce=add, zip=zip_,
+                              precision=skelcl.PREC_BF16X3)
+        t_bw = _time(stream, lambda: ck(L.skl_allpairs_backward(apk.handle, A.ptr, B.ptr, Cb.ptr,
+                                                                Cb.ptr, dC.ptr, M, M, M, st)),
+                     reps=2)
+        ap["backward_bf16x3_ms"] = round(t_bw, 3)
+        del A, B, Cb, dC, hA, hB
+        n, d = 16384, 256
+        P = skelcl.DeviceBuffer(4 * n * d)
+        ck(L.skl_generate_f32_sym(P.ptr, n * d, 0x4003, st))
+        hP = oracle.gen_f32_sym(0x4003, n * d).reshape(n, d)
+        Pt = skelcl.DeviceBuffer.from_array(np.ascontiguousarray(hP.T))
+        D = skelcl.DeviceBuffer(4 * n * n)
+        sq = skelcl.AllPairs("float(float, float)",
+                             "float func(lmatrix_t* row, rmatrix_t* col, const unsigned int dim) {"
+                             " float s = 0; for (unsigned k = 0; k < dim; ++k) {"
+                             " float t = getElementFromRow(row, k) - getElementFromColumn(col, k);"
+                             " s += t*t; } return s; }")
+        t_sq = _time(stream, lambda: ck(L.skl_allpairs_run(sq.handle, D.ptr, P.ptr, Pt.ptr, n, n, d,
+                                                           None, 0, st)), reps=3)
+        rows = np.array([0, 1, 4097, 16383], np.int32)
+        dref = oracle.allpairs_sqdist_f64(hP, np.ascontiguousarray(hP.T), rows=rows)
+        dg = D.to_numpy(np.float32).reshape(n, n)[rows]
+        ap["sqeuclid_16384x256"] = {"ms": round(t_sq, 3),
+                                   "TFLOPs": round(3.0 * n * n * d / (t_sq * 1e-3) / 1e12, 1),
+                                   "max_rel_err_vs_fp64": float(np.max(np.abs(dg - dref) /
+                                                                       np.maximum(1, dref)))}
+        ex["allpairs_8192^3"] = ap
+        progress("allpairs_8192^3 done")
+        del P, Pt, D
 
     # ---- configuration 5: batched segmented Reduce/Scan + 512 small products
-    import json as _json
-    with open(os.path.join(ROOT, "tests", "golden", "batched_shapes.json")) as f:
-        fx = _json.load(f)
-    lens = np.array(fx["segment_lengths"], np.int64)
-    off = np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)
-    tot = int(off[-1])
-    segx = skelcl.DeviceBuffer(4 * tot)
-    segy = skelcl.DeviceBuffer(4 * tot)
-    ck(L.skl_generate_f32_sym(segx.ptr, tot, 0x5002, st))
-    offb = skelcl.DeviceBuffer.from_array(off)
-    sums = skelcl.DeviceBuffer(4 * len(lens))
-    scan = skelcl.Scan("float(float)", "float func(float x, float y){ return x+y; }", "0.0f")
-    t_sr = _time(stream, lambda: ck(L.skl_reduce_segmented_run(add.handle, sums.ptr, segx.ptr,
-                                                               tot, offb.ptr, len(lens), st)), reps=3)
-    t_ss = _time(stream, lambda: ck(L.skl_scan_segmented_run(scan.handle, segy.ptr, segx.ptr,
-                                                             tot, offb.ptr, len(lens), 1, st)), reps=3)
-    t_sb = _time(stream, lambda: ck(L.skl_scan_backward(scan.handle, segy.ptr, segx.ptr, tot, 1,
-                                                        st)), reps=3)
-    shapes = fx["gemm_shapes"]
-    descs = (skelcl._lib.skl_gemm_desc * len(shapes))()
-    bufs = []
-    flops = 0.0
-    for i, (m, nn, k) in enumerate(shapes):
-        a_, b_, c_ = (skelcl.DeviceBuffer(4 * m * k), skelcl.DeviceBuffer(4 * k * nn),
-                      skelcl.DeviceBuffer(4 * m * nn))
-        ck(L.skl_generate_f32_sym(a_.ptr, m * k, 0x5004 + 2 * i, st))
-        ck(L.skl_generate_f32_sym(b_.ptr, k * nn, 0x5005 + 2 * i, st))
-        bufs += [a_, b_, c_]
-        descs[i].A, descs[i].B, descs[i].C = a_.value, b_.value, c_.value
-        descs[i].M, descs[i].N, descs[i].K = m, nn, k
-        flops += 2.0 * m * nn * k
-    dd = skelcl.DeviceBuffer(C.sizeof(descs))
-    ck(L.skl_memcpy_h2d(dd.ptr, 0, C.cast(descs, C.c_void_p), C.sizeof(descs), None, 1))
-    bt = {}
-    for name, prec in (("fp32", skelcl.PREC_FP32), ("bf16x3", skelcl.PREC_BF16X3)):
-        t = _time(stream, lambda: ck(L.skl_matmul_batched_f32(dd.ptr, len(shapes), prec, st)),
-                  reps=3)
-        bt[name] = {"ms": round(t, 3), "TFLOPs": round(flops / (t * 1e-3) / 1e12, 1)}
-    hs = sums.to_numpy(np.float32)
-    chk = [0, 1, 2, 100]
-    hx = {s: oracle.gen_f32_sym(0x5002, int(off[s + 1]))[off[s]:off[s + 1]] for s in chk}
-    ex["batched"] = {
-        "segments": len(lens), "elements": tot,
-        "segmented_reduce_ms": round(t_sr, 3),
-        "segmented_reduce_GBps": round(4 * tot / (t_sr * 1e-3) / 1e9, 1),
-        "segmented_scan_ms": round(t_ss, 3),
-        "segmented_scan_GBps": round(8 * tot / (t_ss * 1e-3) / 1e9, 1),
-        "scan_adjoint_ms": round(t_sb, 3),
-        "segment_sum_max_abs_err_vs_fp64": float(max(abs(float(hs[s]) - oracle.sum_f64(hx[s]))
-                                                     for s in chk)),
-        "gemms": len(shapes), "gemm_flop": flops, "gemm": bt,
-    }
-    progress("batched done")
+    if want("batched"):
+        import json as _json
+        with open(os.path.join(ROOT, "tests", "golden", "batched_shapes.json")) as f:
+            fx = _json.load(f)
+        lens = np.array(fx["segment_lengths"], np.int64)
+        off = np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)
+        tot = int(off[-1])
+        segx = skelcl.DeviceBuffer(4 * tot)
+        segy = skelcl.DeviceBuffer(4 * tot)
+        ck(L.skl_generate_f32_sym(segx.ptr, tot, 0x5002, st))
+        offb = skelcl.DeviceBuffer.from_array(off)
+        sums = skelcl.DeviceBuffer(4 * len(lens))
+        scan = skelcl.Scan("float(float)", "float func(float x, float y){ return x+y; }", "0.0f")
+        t_sr = _time(stream, lambda: ck(L.skl_reduce_segmented_run(add.handle, sums.ptr, segx.ptr,
+                                                                   tot, offb.ptr, len(lens), st)), reps=3)
+        t_ss = _time(stream, lambda: ck(L.skl_scan_segmented_run(scan.handle, segy.ptr, segx.ptr,
+                                                                 tot, offb.ptr, len(lens), 1, st)), reps=3)
+        t_sb = _time(stream, lambda: ck(L.skl_scan_backward(scan.handle, segy.ptr, segx.ptr, tot, 1,
+                                                            st)), reps=3)
+        shapes = fx["gemm_shapes"]
+        descs = (skelcl._lib.skl_gemm_desc * len(shapes))()
+        bufs = []
+        flops = 0.0
+        for i, (m, nn, k) in enumerate(shapes):
+            a_, b_, c_ = (skelcl.DeviceBuffer(4 * m * k), skelcl.DeviceBuffer(4 * k * nn),
+                          skelcl.DeviceBuffer(4 * m * nn))
+            ck(L.skl_generate_f32_sym(a_.ptr, m * k, 0x5004 + 2 * i, st))
+            ck(L.skl_generate_f32_sym(b_.ptr, k * nn, 0x5005 + 2 * i, st))
+            bufs += [a_, b_, c_]
+            descs[i].A, descs[i].B, descs[i].C = a_.value, b_.value, c_.value
+            descs[i].M, descs[i].N, descs[i].K = m, nn, k
+            flops += 2.0 * m * nn * k
+        dd = skelcl.DeviceBuffer(C.sizeof(descs))
+        ck(L.skl_memcpy_h2d(dd.ptr, 0, C.cast(descs, C.c_void_p), C.sizeof(descs), None, 1))
+        bt = {}
+        for name, prec in (("fp32", skelcl.PREC_FP32), ("bf16x3", skelcl.PREC_BF16X3)):
+            t = _time(stream, lambda: ck(L.skl_matmul_batched_f32(dd.ptr, len(shapes), prec, st)),
+                      reps=3)
+            bt[name] = {"ms": round(t, 3), "TFLOPs": round(flops / (t * 1e-3) / 1e12, 1)}
+        hs = sums.to_numpy(np.float32)
+        chk = [0, 1, 2, 100]
+        hx = {s: oracle.gen_f32_sym(0x5002, int(off[s + 1]))[off[s]:off[s + 1]] for s in chk}
+        ex["batched"] = {
+            "segments": len(lens), "elements": tot,
+            "segmented_reduce_ms": round(t_sr, 3),
+            "segmented_reduce_GBps": round(4 * tot / (t_sr * 1e-3) / 1e9, 1),
+            "segmented_scan_ms": round(t_ss, 3),
+            "segmented_scan_GBps": round(8 * tot / (t_ss * 1e-3) / 1e9, 1),
+            "scan_adjoint_ms": round(t_sb, 3),
+            "segment_sum_max_abs_err_vs_fp64": float(max(abs(float(hs[s]) - oracle.sum_f64(hx[s]))
+                                                         for s in chk)),
+            "gemms": len(shapes), "gemm_flop": flops, "gemm": bt,
+        }
+        progress("batched done")
+
     return ex
 
 
@@ -348,9 +374,17 @@ def main() -> int:
                          "fused dot product)")
     ap.add_argument("--autotune", type=int, default=1,
                     help="1: autotune the launch shapes of the timed skeletons first")
+    ap.add_argument("--only", default="",
+                    help="comma list of extra configurations (dot, saxpy_host, mapoverlap, "
+                         "allpairs, batched): run only those and print their JSON (for "
+                         "profiling); skips the headline legs and the CPU baseline")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also time an identity Map (copy) leg, for counter calibration")
     args = ap.parse_args()
+    known = {"dot", "saxpy_host", "mapoverlap", "allpairs", "batched"}
+    if args.only and not set(args.only.split(",")) <= known:
+        print(f"--only: choose from {', '.join(sorted(known))}", file=sys.stderr)
+        return 2
     if args.gpus != 1:
         print(f"bench.py drives one GPU; --gpus {args.gpus} is not supported: the library "
               "runs every skeleton on one MI355X", file=sys.stderr)
@@ -365,6 +399,10 @@ def main() -> int:
     info = skelcl.device_info()
     stream = skelcl.Stream()
     st = stream.handle
+    if args.only:
+        print(json.dumps({"extras": run_extras(skelcl, L, ck, stream, args),
+                          "device": {"name": info["name"], "arch": info["arch"]}}), flush=True)
+        return 0
 
     # inputs resident in HBM
     xs, ys = skelcl.DeviceBuffer(4 * N), skelcl.DeviceBuffer(4 * N)

@@ -214,6 +214,7 @@ extern "C" int skl_autotune(skl_skeleton s, size_t n, int enable)
     d.precision = s->precision;
     d.V = s->V;
     d.extras = s->extras;
+    d.stencil_radius = s->stencil_radius;
     d.apply_default_shape(cus);
     apply(s, Shape{d.unroll, d.max_blocks / cus, d.scan_j, d.mo_rb});
     return SKL_OK;

@@ -40,6 +40,24 @@
 
 namespace {
 
+// Global memory at a wave-uniform row address plus a 32-bit lane offset.
+// The row address passes through an empty asm, so that it stays a scalar
+// base: the compiler would otherwise fold the lane offset into the row
+// address once and add every row's offset in 64-bit vector arithmetic.
+__device__ __forceinline__ float load_row(const void* row, unsigned off)
+{
+  unsigned long long r = (unsigned long long)row;
+  asm volatile("" : "+s"(r));
+  return *(const __attribute__((address_space(1))) float*)(r + off);
+}
+
+__device__ __forceinline__ void store_row(void* row, unsigned off, float v)
+{
+  unsigned long long r = (unsigned long long)row;
+  asm volatile("" : "+s"(r));
+  *(__attribute__((address_space(1))) float*)(r + off) = v;
+}
+
 template <int R, int RB, bool NEUTRAL, bool FUSED>
 __global__ __launch_bounds__(256) void stencil_forward_kernel(const float* __restrict__ in,
                                                               float* __restrict__ out,
@@ -51,36 +69,54 @@ __global__ __launch_bounds__(256) void stencil_forward_kernel(const float* __res
   constexpr int NR = (THP + 3) / 4;  // tile rows per wave
   __shared__ float tile[THP * TWP];
   const int lane = threadIdx.x & 63;
-  // uniform per wave: row addresses are scalar and loads/stores take a
-  // scalar base with a 32-bit lane offset
+  // uniform per wave, so row addresses are scalar
   const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int c0 = blockIdx.x * TW, r0 = blockIdx.y * TH;
 
-  // ---- stage the tile
-  const int gc0 = c0 - R + lane, gc1 = gc0 + 64;
-  const int cc0 = gc0 < 0 ? 0 : (gc0 >= cols ? cols - 1 : gc0);
-  const int cc1 = gc1 < 0 ? 0 : (gc1 >= cols ? cols - 1 : gc1);
-  const bool halo = lane < 2 * R;
+  // ---- stage the tile.  Tiles whose window lies inside the matrix (all but the
+  // border tiles) load without clamping or neutral fill.
+  const unsigned l4 = (unsigned)lane * 4u;
+  const bool halo = lane < 2 * R;  // lanes that also load a column past 64
   float a[NR], b[NR];
+  const bool interior = c0 >= R && c0 + TW + R <= cols && r0 >= R && r0 + TH + R <= rows;
+  if (interior) {
+    const float* base = in + (size_t)(r0 - R) * (size_t)cols + (c0 - R);
 #pragma unroll
-  for (int k = 0; k < NR; ++k) {
-    const int tr = wv + 4 * k < THP ? wv + 4 * k : THP - 1;
-    const int gr = r0 - R + tr;
-    const int cr = gr < 0 ? 0 : (gr >= rows ? rows - 1 : gr);
-    const float* row = in + (size_t)cr * (size_t)cols;
-    a[k] = row[(unsigned)cc0];
-    b[k] = halo ? row[(unsigned)cc1] : 0.0f;
-    if (NEUTRAL) {
-      if (gr != cr || gc0 != cc0) a[k] = neutral;
-      if (gr != cr || gc1 != cc1) b[k] = neutral;
+    for (int k = 0; k < NR; ++k) {
+      const int tr = wv + 4 * k;
+      if (tr < THP) {
+        const float* row = base + (size_t)tr * (size_t)cols;
+        a[k] = load_row(row, l4);
+        if (halo) b[k] = load_row(row, l4 + 256u);
+      }
+    }
+  } else {
+    const int gc0 = c0 - R + lane, gc1 = gc0 + 64;
+    const int cc0 = gc0 < 0 ? 0 : (gc0 >= cols ? cols - 1 : gc0);
+    const int cc1 = gc1 < 0 ? 0 : (gc1 >= cols ? cols - 1 : gc1);
+    const unsigned o0 = (unsigned)cc0 * 4u, o1 = (unsigned)cc1 * 4u;
+#pragma unroll
+    for (int k = 0; k < NR; ++k) {
+      const int tr = wv + 4 * k;
+      if (tr < THP) {
+        const int gr = r0 - R + tr;
+        const int cr = gr < 0 ? 0 : (gr >= rows ? rows - 1 : gr);
+        const float* row = in + (size_t)cr * (size_t)cols;
+        a[k] = load_row(row, o0);
+        if (halo) b[k] = load_row(row, o1);
+        if (NEUTRAL) {
+          if (gr != cr || gc0 != cc0) a[k] = neutral;
+          if (gr != cr || gc1 != cc1) b[k] = neutral;
+        }
+      }
     }
   }
+  float* trow = tile + wv * TWP + lane;
 #pragma unroll
   for (int k = 0; k < NR; ++k) {
-    const int tr = wv + 4 * k;
-    if (tr < THP) {
-      tile[tr * TWP + lane] = a[k];
-      if (halo) tile[tr * TWP + 64 + lane] = b[k];
+    if (wv + 4 * k < THP) {
+      trow[4 * k * TWP] = a[k];
+      if (halo) trow[4 * k * TWP + 64] = b[k];
     }
   }
   __syncthreads();
@@ -135,12 +171,11 @@ __global__ __launch_bounds__(256) void stencil_forward_kernel(const float* __res
 #pragma unroll
     for (int i = 0; i < RB; ++i) asm volatile("" : "+s"(zt[n]), "+s"(zw[n]) : "v"(acc[i]));
   }
-  const int c = c0 + lane;
-  if (c < cols) {
+  if (c0 + lane < cols) {
 #pragma unroll
     for (int i = 0; i < RB; ++i) {
       const int r = r0 + wv * RB + i;
-      if (r < rows) (out + (size_t)r * (size_t)cols)[(unsigned)c] = acc[i];
+      if (r < rows) store_row(out + (size_t)r * (size_t)cols + c0, l4, acc[i]);
     }
   }
 }

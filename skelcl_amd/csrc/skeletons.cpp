@@ -89,7 +89,9 @@ void skl_skeleton_s::apply_default_shape(int cus)
     max_blocks = cus * 8;
   }
   if (kind == SKL_SCAN) scan_j = default_scan_j();
-  mo_rb = 4;
+  // MapOverlap rows per thread: the native stencil kernel is fastest at 8
+  // (8192^2, radius 5: 0.284 ms against 0.293 at 6 and 0.336 at 4)
+  mo_rb = stencil_radius >= 0 ? 8 : 4;
 }
 
 int skl_skeleton_s::default_scan_j() const
