@@ -47,9 +47,32 @@ def progress(msg: str) -> None:
     print(f"bench: {msg}", file=sys.stderr, flush=True)
 
 
-def cpu_baseline(nthreads: int):
-    """The oracle (reference-order CPU restatement) on the host cores: one
-    step of the same workload at full size, timed per skeleton."""
+def host_cpus():
+    """(threads the CPU baseline uses, CPUs this process may run on, CPU
+    model).  The threads are the process's CPU share: OMP_NUM_THREADS when
+    set (the share a one-GPU machine grants), else every CPU in its
+    affinity mask."""
+    avail = len(os.sched_getaffinity(0))
+    try:
+        share = int(os.environ.get("OMP_NUM_THREADS", "0"))
+    except ValueError:
+        share = 0
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return (min(share, avail) if share > 0 else avail), avail, model
+
+
+def cpu_baseline(nthreads: int, reps: int = 5):
+    """The oracle (reference-order CPU restatement) on the host cores: the
+    same workload at full size, each skeleton run once untimed and then
+    `reps` times; the median per skeleton."""
     import oracle
     t0 = time.perf_counter()
     xs = oracle.gen_i32_below(SEED_SCAN, N, 1000)
@@ -57,17 +80,21 @@ def cpu_baseline(nthreads: int):
     x = oracle.gen_f32_sym(SEED_X, N)
     y = oracle.gen_f32_sym(SEED_Y, N)
     tgen = time.perf_counter() - t0
+
+    def timed(fn):
+        out = fn()
+        ts = []
+        for _ in range(reps):
+            t1 = time.perf_counter()
+            out = fn()
+            ts.append(time.perf_counter() - t1)
+        return statistics.median(ts), out
+
     t = {}
-    t0 = time.perf_counter()
-    ex = oracle.scan_excl_add_u32_reforder(xs, 256, nthreads)
+    t["scan"], ex = timed(lambda: oracle.scan_excl_add_u32_reforder(xs, 256, nthreads))
     inc = ex + xs.view(np.uint32)  # inclusive from the reference's exclusive form
-    t["scan"] = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    red = oracle.reduce_add_f32_reforder(xr, 256, nthreads)
-    t["reduce"] = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    sax = oracle.saxpy_f32(SAXPY_A, x, y, nthreads)
-    t["saxpy"] = time.perf_counter() - t0
+    t["reduce"], red = timed(lambda: oracle.reduce_add_f32_reforder(xr, 256, nthreads))
+    t["saxpy"], sax = timed(lambda: oracle.saxpy_f32(SAXPY_A, x, y, nthreads))
     ref64 = oracle.sum_f64(xr)
     total = sum(t.values())
     return {
@@ -169,6 +196,35 @@ def run_extras(skelcl, L, ck, stream, args):
                                      == oracle.reduce_add_u32_reforder(k1)),
         }
         progress("dot_2^20 done")
+
+    # ---- store-only streams: Map<int(Index)> over 2^28 indices and
+    # Map<int(IndexPoint)> over 16384 x 16384 points (4 B written per element)
+    if want("index"):
+        ni = 1 << 28
+        oi = skelcl.DeviceBuffer(4 * ni)
+        mi = skelcl.Map("int(Index)", "int func(Index i){ return (int)(i * 2654435761u); }")
+        mp = skelcl.Map("int(IndexPoint)", "int func(IndexPoint p){ return p.x - 3 * p.y; }")
+        t_i = _time(stream, lambda: ck(L.skl_map_index_run(mi.handle, oi.ptr, ni, 0, None, 0, st)),
+                    reps=5)
+        hi = oi.to_numpy(np.int32)
+        idx = np.arange(0, ni, 4099, dtype=np.uint64)
+        ok_i = bool(np.array_equal(hi[::4099], ((idx * 2654435761) & 0xFFFFFFFF)
+                                   .astype(np.uint32).view(np.int32)))
+        side = 1 << 14
+        t_p = _time(stream, lambda: ck(L.skl_map_indexpoint_run(mp.handle, oi.ptr, side, side, 0,
+                                                                None, 0, st)), reps=5)
+        hp = oi.to_numpy(np.int32).reshape(side, side)
+        rr = np.arange(0, side, 97)
+        ok_p = bool(np.array_equal(hp[rr], (np.arange(side)[None, :] - 3 * rr[:, None])
+                                   .astype(np.int32)))
+        ex["index_map_2^28"] = {
+            "index_ms": round(t_i, 4), "index_GBps": round(4 * ni / (t_i * 1e-3) / 1e9, 1),
+            "indexpoint_ms": round(t_p, 4),
+            "indexpoint_GBps": round(4 * ni / (t_p * 1e-3) / 1e9, 1),
+            "sampled_exact": ok_i and ok_p,
+        }
+        progress("index_map_2^28 done")
+        del oi, hi, hp
 
     # ---- host buffers handed over: saxpy 2^26 through the Vector API
     if want("saxpy_host"):
@@ -315,13 +371,31 @@ def run_extras(skelcl, L, ck, stream, args):
         ck(L.skl_generate_f32_sym(segx.ptr, tot, 0x5002, st))
         offb = skelcl.DeviceBuffer.from_array(off)
         sums = skelcl.DeviceBuffer(4 * len(lens))
-        scan = skelcl.Scan("float(float)", "float func(float x, float y){ return x+y; }", "0.0f")
-        t_sr = _time(stream, lambda: ck(L.skl_reduce_segmented_run(add.handle, sums.ptr, segx.ptr,
-                                                                   tot, offb.ptr, len(lens), st)), reps=3)
-        t_ss = _time(stream, lambda: ck(L.skl_scan_segmented_run(scan.handle, segy.ptr, segx.ptr,
-                                                                 tot, offb.ptr, len(lens), 1, st)), reps=3)
-        t_sb = _time(stream, lambda: ck(L.skl_scan_backward(scan.handle, segy.ptr, segx.ptr, tot, 1,
-                                                            st)), reps=3)
+        # the precision forms of float (+): DEFAULT (compensated FP32 inside a
+        # block, FP64 across blocks) and FP64 (FP64 throughout)
+        ADD = "float func(float x, float y){ return x+y; }"
+        forms = {}
+        for pname, prec in (("default", skelcl.PREC_DEFAULT), ("fp64", skelcl.PREC_FP64)):
+            red_p = skelcl.Reduce("float(float)", ADD, "0", precision=prec)
+            scan_p = skelcl.Scan("float(float)", ADD, "0.0f", precision=prec)
+            f = {}
+            f["segmented_reduce_ms"] = _time(stream, lambda: ck(L.skl_reduce_segmented_run(
+                red_p.handle, sums.ptr, segx.ptr, tot, offb.ptr, len(lens), st)), reps=3)
+            f["segmented_scan_ms"] = _time(stream, lambda: ck(L.skl_scan_segmented_run(
+                scan_p.handle, segy.ptr, segx.ptr, tot, offb.ptr, len(lens), 1, st)), reps=3)
+            f["scan_ms"] = _time(stream, lambda: ck(L.skl_scan_run(
+                scan_p.handle, segy.ptr, segx.ptr, tot, 1, None, 0, st)), reps=3)
+            f["scan_adjoint_ms"] = _time(stream, lambda: ck(L.skl_scan_backward(
+                scan_p.handle, segy.ptr, segx.ptr, tot, 1, st)), reps=3)
+            f["reduce_ms"] = _time(stream, lambda: ck(L.skl_reduce_run(
+                red_p.handle, sums.ptr, segx.ptr, tot, None, 0, st)), reps=3)
+            forms[pname] = {k: round(v, 4) for k, v in f.items()}
+        # the DEFAULT form's segment sums are the ones checked below
+        ck(L.skl_reduce_segmented_run(add.handle, sums.ptr, segx.ptr, tot, offb.ptr, len(lens),
+                                      st))
+        t_sr = forms["default"]["segmented_reduce_ms"]
+        t_ss = forms["default"]["segmented_scan_ms"]
+        t_sb = forms["default"]["scan_adjoint_ms"]
         shapes = fx["gemm_shapes"]
         descs = (skelcl._lib.skl_gemm_desc * len(shapes))()
         bufs = []
@@ -355,6 +429,7 @@ def run_extras(skelcl, L, ck, stream, args):
             "segment_sum_max_abs_err_vs_fp64": float(max(abs(float(hs[s]) - oracle.sum_f64(hx[s]))
                                                          for s in chk)),
             "gemms": len(shapes), "gemm_flop": flops, "gemm": bt,
+        "float_add_precision_forms_ms": forms,
         }
         progress("batched done")
 
@@ -375,13 +450,13 @@ def main() -> int:
     ap.add_argument("--autotune", type=int, default=1,
                     help="1: autotune the launch shapes of the timed skeletons first")
     ap.add_argument("--only", default="",
-                    help="comma list of extra configurations (dot, saxpy_host, mapoverlap, "
+                    help="comma list of extra configurations (dot, index, saxpy_host, mapoverlap, "
                          "allpairs, batched): run only those and print their JSON (for "
                          "profiling); skips the headline legs and the CPU baseline")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also time an identity Map (copy) leg, for counter calibration")
     args = ap.parse_args()
-    known = {"dot", "saxpy_host", "mapoverlap", "allpairs", "batched"}
+    known = {"dot", "index", "saxpy_host", "mapoverlap", "allpairs", "batched"}
     if args.only and not set(args.only.split(",")) <= known:
         print(f"--only: choose from {', '.join(sorted(known))}", file=sys.stderr)
         return 2
@@ -531,7 +606,7 @@ def main() -> int:
         "scan_last": int(tail[0]),
     }
     if not args.no_cpu:
-        nthreads = min(os.cpu_count() or 1, 16)
+        nthreads, ncpu, model = host_cpus()
         cb = cpu_baseline(nthreads)
         progress("cpu baseline done")
         ref64 = cb["reduce_fp64"]
@@ -543,8 +618,10 @@ def main() -> int:
         cpu_gbs = total_bytes / cb["total_s"] / 1e9
         result["cpu_baseline"] = {
             "value": round(cpu_gbs, 2), "unit": "GB/s", "cores": nthreads, "kind": "port",
+            "host_cpus": ncpu, "cpu_model": model,
             "sample": "one full step (N=2^28 each): oracle reference-order multi-pass "
-                      "Blelloch Scan + 8192-partial Reduce + saxpy, pthreads; "
+                      "Blelloch Scan + 8192-partial Reduce + saxpy, pthreads; each skeleton "
+                      "run once untimed, then the median of 5; "
                       f"scan {cb['times_s']['scan']:.2f}s, reduce "
                       f"{cb['times_s']['reduce']:.2f}s, saxpy {cb['times_s']['saxpy']:.2f}s"}
         result["scan_last_matches_cpu"] = bool(result["scan_last_matches_cpu"])

@@ -57,13 +57,18 @@ enum skl_skeleton_kind {
 enum skl_padding { SKL_PADDING_NEUTRAL = 0, SKL_PADDING_NEAREST = 1 };
 
 /* Precision form of the float paths (beyond the reference):
- *  DEFAULT  Reduce(+) on float: compensated per-block partials with an
- *           FP64 cross-block combine; Scan(+) on float: FP64 prefixes and
- *           carries, one rounding per output; AllPairs (+,*) on float:
+ *  DEFAULT  Reduce(+) and Scan(+) on float: compensated FP32 (double-float
+ *           pairs) inside each work-group / tile, FP64 only in the
+ *           cross-block combine (Reduce partials, Scan look-back carries);
+ *           each Scan output is one rounding of carry + tile prefix.  The
+ *           batched segmented forms likewise.  AllPairs (+,*) on float:
  *           plain FP32.
  *  FP32     plain FP32 everywhere (user function, FP32 accumulation).
  *  BF16X3   AllPairs (+,*) on float: bf16x3 split on MFMA, FP32 accumulate.
  *  FP8      AllPairs (+,*) on float: FP8 e4m3 MFMA, per-128x128-tile scale.
+ *  FP64     Reduce(+) and Scan(+) on float, plain and segmented: FP64
+ *           accumulation throughout (every tile prefix and partial in FP64);
+ *           other skeletons run as DEFAULT.
  * The AllPairs forms apply to float inputs and output with a zero Reduce
  * identity; a non-zero identity starts each fold (AllPairsKernel.cl:63), so
  * that contraction runs the in-order FP32 FMA fold at every setting. */
@@ -71,7 +76,8 @@ enum skl_precision {
   SKL_PREC_DEFAULT = 0,
   SKL_PREC_FP32 = 1,
   SKL_PREC_BF16X3 = 2,
-  SKL_PREC_FP8 = 3
+  SKL_PREC_FP8 = 3,
+  SKL_PREC_FP64 = 4
 };
 
 typedef struct {

@@ -29,12 +29,12 @@ import numpy as np
 from . import _lib
 from ._lib import (SkelError, check, lib, take_string, SKL_PADDING_NEAREST,
                    SKL_PADDING_NEUTRAL, SKL_PREC_BF16X3, SKL_PREC_DEFAULT, SKL_PREC_FP32,
-                   SKL_PREC_FP8)
+                   SKL_PREC_FP8, SKL_PREC_FP64)
 
 NEUTRAL = SKL_PADDING_NEUTRAL
 NEAREST = SKL_PADDING_NEAREST
-PREC_DEFAULT, PREC_FP32, PREC_BF16X3, PREC_FP8 = (SKL_PREC_DEFAULT, SKL_PREC_FP32,
-                                                  SKL_PREC_BF16X3, SKL_PREC_FP8)
+PREC_DEFAULT, PREC_FP32, PREC_BF16X3, PREC_FP8, PREC_FP64 = (
+    SKL_PREC_DEFAULT, SKL_PREC_FP32, SKL_PREC_BF16X3, SKL_PREC_FP8, SKL_PREC_FP64)
 
 __all__ = [
     "init", "terminate", "device_info", "synchronize", "Vector", "Matrix", "IndexVector",

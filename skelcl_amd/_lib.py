@@ -18,7 +18,7 @@ SKL_ERR_PARSE, SKL_ERR_UNSUPPORTED, SKL_ERR_NOT_INIT = 5, 6, 7
 (SKL_MAP, SKL_MAP_INDEX, SKL_MAP_INDEXPOINT, SKL_ZIP, SKL_REDUCE, SKL_SCAN,
  SKL_MAPOVERLAP, SKL_ALLPAIRS, SKL_ALLPAIRS_GENERAL) = range(9)
 SKL_PADDING_NEUTRAL, SKL_PADDING_NEAREST = 0, 1
-SKL_PREC_DEFAULT, SKL_PREC_FP32, SKL_PREC_BF16X3, SKL_PREC_FP8 = 0, 1, 2, 3
+SKL_PREC_DEFAULT, SKL_PREC_FP32, SKL_PREC_BF16X3, SKL_PREC_FP8, SKL_PREC_FP64 = 0, 1, 2, 3, 4
 SKL_ARG_VALUE, SKL_ARG_BUFFER, SKL_ARG_MATRIX, SKL_ARG_LOCAL = 0, 1, 2, 3
 
 

@@ -22,11 +22,11 @@ extern "C" int skl_internal_stencil_transpose(float* din, const float* dout, con
                                               int rows, int cols, int R, int nearest,
                                               hipStream_t s);
 extern "C" int skl_internal_segmented_sum(float* out, const float* in, unsigned long long n,
-                                          const int64_t* offsets, int nseg, void* ws,
+                                          const int64_t* offsets, int nseg, int fp64, void* ws,
                                           hipStream_t s);
 extern "C" int skl_internal_segmented_scan(float* out, const float* in, unsigned long long n,
                                            const int64_t* offsets, int nseg, int inclusive,
-                                           void* ws, hipStream_t s);
+                                           int fp64, void* ws, hipStream_t s);
 extern "C" size_t skl_internal_segmented_workspace(unsigned long long n, int nseg);
 extern "C" int skl_internal_broadcast(void* dst, const void* src, size_t elem, size_t count,
                                       hipStream_t s);
@@ -289,8 +289,10 @@ extern "C" int skl_reduce_segmented_run(skl_skeleton s, void* out, const void* i
   if (nseg == 0) return SKL_OK;
   if (!out || !offsets || (n && !in)) return fail(SKL_ERR_INVALID, "NULL buffer");
   SKL_TRY(seg_scratch(s, skl_internal_segmented_workspace(n, nseg)));
+  // precision FP64: FP64 chunk sums; otherwise compensated FP32 in a chunk
   return skl_internal_segmented_sum((float*)out, (const float*)in, n, offsets, nseg,
-                                    s->seg_scratch, resolve_stream(stream));
+                                    s->precision == SKL_PREC_FP64 ? 1 : 0, s->seg_scratch,
+                                    resolve_stream(stream));
 }
 
 extern "C" int skl_scan_segmented_run(skl_skeleton s, void* out, const void* in, size_t n,
@@ -306,7 +308,8 @@ extern "C" int skl_scan_segmented_run(skl_skeleton s, void* out, const void* in,
   if (!out || !in || !offsets) return fail(SKL_ERR_INVALID, "NULL buffer");
   SKL_TRY(seg_scratch(s, skl_internal_segmented_workspace(n, nseg)));
   return skl_internal_segmented_scan((float*)out, (const float*)in, n, offsets, nseg,
-                                     inclusive, s->seg_scratch, resolve_stream(stream));
+                                     inclusive, s->precision == SKL_PREC_FP64 ? 1 : 0,
+                                     s->seg_scratch, resolve_stream(stream));
 }
 
 extern "C" int skl_matmul_batched_f32(const skl_gemm_desc* descs_device, int count,

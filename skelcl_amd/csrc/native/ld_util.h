@@ -37,3 +37,65 @@ __device__ __forceinline__ T ld_or_g(const T* p, size_t i, size_t safe, bool ok,
   const T v = as_global(p)[ok ? i : safe];
   return ok ? v : fill;
 }
+
+// Accumulators of the float (+) kernels: FP64, or a double-float pair (a
+// compensated FP32 sum: Knuth's two-sum on the high words, the rounding
+// errors and low words summed in the low word).  Both are 8 bytes; `acc_*`
+// are overloaded on the two.
+struct acc_df {
+  float hi, lo;
+};
+
+template <typename A>
+__device__ __forceinline__ A acc_zero();
+template <>
+__device__ __forceinline__ double acc_zero<double>() { return 0.0; }
+template <>
+__device__ __forceinline__ acc_df acc_zero<acc_df>() { return acc_df{0.0f, 0.0f}; }
+
+__device__ __forceinline__ double acc_add(double a, double b) { return a + b; }
+__device__ __forceinline__ double acc_add(double a, float x) { return a + (double)x; }
+__device__ __forceinline__ acc_df acc_add(const acc_df& a, const acc_df& b)
+{
+  const float s = a.hi + b.hi;
+  const float bb = s - a.hi;
+  const float e = (a.hi - (s - bb)) + (b.hi - bb);
+  return acc_df{s, (a.lo + b.lo) + e};
+}
+__device__ __forceinline__ acc_df acc_add(const acc_df& a, float x)
+{
+  const float s = a.hi + x;
+  const float bb = s - a.hi;
+  const float e = (a.hi - (s - bb)) + (x - bb);
+  return acc_df{s, a.lo + e};
+}
+__device__ __forceinline__ double acc_to_double(double a) { return a; }
+__device__ __forceinline__ double acc_to_double(const acc_df& a)
+{
+  return (double)a.hi + (double)a.lo;
+}
+
+// the two dwords of an accumulator, and back
+__device__ __forceinline__ void acc_bits(double v, int* lo, int* hi)
+{
+  const long long b = __double_as_longlong(v);
+  *lo = (int)b;
+  *hi = (int)(b >> 32);
+}
+__device__ __forceinline__ void acc_bits(const acc_df& v, int* lo, int* hi)
+{
+  *lo = __float_as_int(v.hi);
+  *hi = __float_as_int(v.lo);
+}
+template <typename A>
+__device__ __forceinline__ A acc_from_bits(int lo, int hi);
+template <>
+__device__ __forceinline__ double acc_from_bits<double>(int lo, int hi)
+{
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+template <>
+__device__ __forceinline__ acc_df acc_from_bits<acc_df>(int lo, int hi)
+{
+  return acc_df{__int_as_float(lo), __int_as_float(hi)};
+}

@@ -5,14 +5,16 @@
 // Reduce: a one-work-group prologue validates the offsets (non-decreasing,
 // inside [0, n]), cuts segment s (length L) into ceil(L / 16384) chunks,
 // records each segment's first chunk and each chunk's segment.  One
-// work-group per chunk stores the chunk's FP64 sum as a partial; a second
-// kernel, one wave per segment, adds the segment's partials in a fixed order
-// and writes out[s].
+// work-group per chunk stores the chunk's sum as an FP64 partial; a second
+// kernel, one wave per segment, adds the segment's partials in FP64 in a
+// fixed order and writes out[s].
 // Scan: the elements are cut into 32768-element tiles regardless of the
 // segments and scanned in one pass with the segmented operator (see the
 // flat Scan section below); a prologue, one thread per segment, validates
-// the offsets and records each tile's first segment.  All sums in FP64, every output is
-// float(carry + local prefix).
+// the offsets and records each tile's first segment.  Every output is
+// float(carry + local prefix), the look-back carries in FP64.
+// Inside a chunk / tile the sums are compensated FP32 (double-float pairs,
+// acc_df), or FP64 throughout when the caller asks for FP64.
 // Invalid offsets set an error flag: no chunk or tile runs, Reduce writes NaN.
 #include <hip/hip_runtime.h>
 
@@ -97,21 +99,32 @@ __global__ __launch_bounds__(256) void seg_sum_fill(const long long* __restrict_
   for (long long c = cstart[s] + (threadIdx.x & 63); c < c1; c += 64) cseg[c] = s;
 }
 
-__device__ __forceinline__ double block_sum_double(double v, double* red)
+template <typename A>
+__device__ __forceinline__ A acc_shfl_down(const A& v, int o)
 {
-  for (int o = 32; o >= 1; o >>= 1) v += __shfl_down(v, o, 64);
+  int lo, hi;
+  acc_bits(v, &lo, &hi);
+  return acc_from_bits<A>(__shfl_down(lo, o, 64), __shfl_down(hi, o, 64));
+}
+
+// the work-group's sum, as FP64 (valid in thread 0)
+template <typename A>
+__device__ __forceinline__ double block_sum(A v, A* red)
+{
+  for (int o = 32; o >= 1; o >>= 1) v = acc_add(v, acc_shfl_down(v, o));
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   if (lane == 0) red[w] = v;
   __syncthreads();
-  double t = 0.0;
+  A t = acc_zero<A>();
   if (threadIdx.x == 0)
-    for (int i = 0; i < WG / 64; ++i) t += red[i];
-  return t;  // valid in thread 0
+    for (int i = 0; i < WG / 64; ++i) t = acc_add(t, red[i]);
+  return acc_to_double(t);
 }
 
-// one work-group per chunk: FP64 sums (4 accumulators per thread, fixed
+// one work-group per chunk: sums in A (4 accumulators per thread, fixed
 // order), a full chunk read as 16 float4 packs per thread all in flight
 // (dword-aligned: a segment may start anywhere)
+template <typename A>
 __global__ __launch_bounds__(WG) void seg_sum_kernel(const float* __restrict__ in,
                                                      const int64_t* __restrict__ off,
                                                      const long long* __restrict__ cstart,
@@ -120,7 +133,7 @@ __global__ __launch_bounds__(WG) void seg_sum_kernel(const float* __restrict__ i
                                                      const SegCtl* __restrict__ ctl)
 {
   typedef float f4u __attribute__((ext_vector_type(4), aligned(4)));
-  __shared__ double red[WG / 64];
+  __shared__ A red[WG / 64];
   const long long c = blockIdx.x;
   if (c >= (long long)ctl->total) return;
   const int s = cseg[c];
@@ -128,7 +141,7 @@ __global__ __launch_bounds__(WG) void seg_sum_kernel(const float* __restrict__ i
   const long long lo = off[s] + k * SUM_CH;
   const long long end = off[s + 1];
   const long long hi = lo + SUM_CH < end ? lo + SUM_CH : end;
-  double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+  A a0 = acc_zero<A>(), a1 = a0, a2 = a0, a3 = a0;
   if (hi - lo == SUM_CH) {
     f4u v[SUM_J];
 #pragma unroll
@@ -137,24 +150,24 @@ __global__ __launch_bounds__(WG) void seg_sum_kernel(const float* __restrict__ i
           reinterpret_cast<const f4u*>(in + lo + ((long long)j * WG + threadIdx.x) * 4));
 #pragma unroll
     for (int j = 0; j < SUM_J; ++j) {
-      a0 += (double)v[j][0];
-      a1 += (double)v[j][1];
-      a2 += (double)v[j][2];
-      a3 += (double)v[j][3];
+      a0 = acc_add(a0, v[j][0]);
+      a1 = acc_add(a1, v[j][1]);
+      a2 = acc_add(a2, v[j][2]);
+      a3 = acc_add(a3, v[j][3]);
     }
   } else {
     // the last chunk of a segment
     long long i = lo + threadIdx.x;
     for (; i + 3 * WG < hi; i += 4 * WG) {
       const float x0 = in[i], x1 = in[i + WG], x2 = in[i + 2 * WG], x3 = in[i + 3 * WG];
-      a0 += (double)x0;
-      a1 += (double)x1;
-      a2 += (double)x2;
-      a3 += (double)x3;
+      a0 = acc_add(a0, x0);
+      a1 = acc_add(a1, x1);
+      a2 = acc_add(a2, x2);
+      a3 = acc_add(a3, x3);
     }
-    for (; i < hi; i += WG) a0 += (double)in[i];
+    for (; i < hi; i += WG) a0 = acc_add(a0, in[i]);
   }
-  const double tot = block_sum_double((a0 + a1) + (a2 + a3), red);
+  const double tot = block_sum<A>(acc_add(acc_add(a0, a1), acc_add(a2, a3)), red);
   if (threadIdx.x == 0) partials[c] = tot;
 }
 
@@ -220,15 +233,15 @@ constexpr int FJ = 16;                                     // float4 packs per l
 constexpr long long FTILE = (long long)FWG * FJ * 4;       // 32768 elements
 constexpr int FWAVES = FWG / 64;
 
-// v of the lane named by the DPP control (0.0 where the control names no
-// lane or the row is outside ROW_MASK), FP64 moved as two dwords
-template <int CTRL, int ROW_MASK>
-__device__ __forceinline__ double dpp_f64(double v)
+// v of the lane named by the DPP control (zero where the control names no
+// lane or the row is outside ROW_MASK), moved as two dwords
+template <int CTRL, int ROW_MASK, typename A>
+__device__ __forceinline__ A dpp_acc(const A& v)
 {
-  const long long b = __double_as_longlong(v);
-  const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, ROW_MASK, 0xf, false);
-  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, ROW_MASK, 0xf, false);
-  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+  int lo, hi;
+  acc_bits(v, &lo, &hi);
+  return acc_from_bits<A>(__builtin_amdgcn_update_dpp(0, lo, CTRL, ROW_MASK, 0xf, false),
+                          __builtin_amdgcn_update_dpp(0, hi, CTRL, ROW_MASK, 0xf, false));
 }
 
 // inclusive segmented wave scan: lane l ends with the sum of lanes
@@ -236,33 +249,35 @@ __device__ __forceinline__ double dpp_f64(double v)
 // row_shr 1, 2, 4, 8 inside each 16-lane row, then row_bcast 15 (lane 15 of
 // each row into rows 1 and 3) and row_bcast 31 (lane 31 into rows 2 and 3);
 // a step adds its source lane s only when s >= hl.  All VALU, no LDS.
-__device__ __forceinline__ double seg_wave_scan(double v, int lane, int hl)
+template <typename A>
+__device__ __forceinline__ A seg_wave_scan(A v, int lane, int hl)
 {
-  double t;
-  t = dpp_f64<0x111, 0xf>(v);
-  if (lane - 1 >= hl) v += t;
-  t = dpp_f64<0x112, 0xf>(v);
-  if (lane - 2 >= hl) v += t;
-  t = dpp_f64<0x114, 0xf>(v);
-  if (lane - 4 >= hl) v += t;
-  t = dpp_f64<0x118, 0xf>(v);
-  if (lane - 8 >= hl) v += t;
-  t = dpp_f64<0x142, 0xa>(v);
-  if ((lane & ~15) - 1 >= hl) v += t;
-  t = dpp_f64<0x143, 0xc>(v);
-  if (31 >= hl) v += t;
+  A t;
+  t = dpp_acc<0x111, 0xf>(v);
+  if (lane - 1 >= hl) v = acc_add(v, t);
+  t = dpp_acc<0x112, 0xf>(v);
+  if (lane - 2 >= hl) v = acc_add(v, t);
+  t = dpp_acc<0x114, 0xf>(v);
+  if (lane - 4 >= hl) v = acc_add(v, t);
+  t = dpp_acc<0x118, 0xf>(v);
+  if (lane - 8 >= hl) v = acc_add(v, t);
+  t = dpp_acc<0x142, 0xa>(v);
+  if ((lane & ~15) - 1 >= hl) v = acc_add(v, t);
+  t = dpp_acc<0x143, 0xc>(v);
+  if (31 >= hl) v = acc_add(v, t);
   return v;
 }
 
-// value of lane - 1 (0.0 in lane 0)
-__device__ __forceinline__ double wave_shr1_f64(double v) { return dpp_f64<0x138, 0xf>(v); }
+// value of lane - 1 (zero in lane 0)
+template <typename A>
+__device__ __forceinline__ A wave_shr1(const A& v) { return dpp_acc<0x138, 0xf>(v); }
 
-__device__ __forceinline__ double readlane63_f64(double v)
+template <typename A>
+__device__ __forceinline__ A readlane63(const A& v)
 {
-  const long long b = __double_as_longlong(v);
-  const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)b, 63);
-  const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(b >> 32), 63);
-  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+  int lo, hi;
+  acc_bits(v, &lo, &hi);
+  return acc_from_bits<A>(__builtin_amdgcn_readlane(lo, 63), __builtin_amdgcn_readlane(hi, 63));
 }
 
 // Prologue, one thread per run s = -1 .. nseg (run -1 is [0, offsets[0]),
@@ -304,6 +319,7 @@ __global__ __launch_bounds__(256) void seg_flat_prologue(const int64_t* __restri
   for (long long t = (a + FTILE - 1) / FTILE; t < t1; ++t) tfirst[t] = (int)s;
 }
 
+template <typename A>
 __global__ __launch_bounds__(FWG) void seg_flat_scan_kernel(
     const float* __restrict__ in, float* __restrict__ out, unsigned long long n,
     const int64_t* __restrict__ off, int nseg, const int* __restrict__ tfirst,
@@ -314,8 +330,8 @@ __global__ __launch_bounds__(FWG) void seg_flat_scan_kernel(
   __shared__ unsigned long long s_tile;
   __shared__ unsigned s_epoch, s_err;
   __shared__ unsigned s_bits[FTILE / 32];
-  __shared__ double s_lpv[FJ][FWG];  // per pack: wave-exclusive prefix (own thread only)
-  __shared__ double s_wval[FWAVES];
+  __shared__ A s_lpv[FJ][FWG];  // per pack: wave-exclusive prefix (own thread only)
+  __shared__ A s_wval[FWAVES];
   __shared__ int s_wopen[FWAVES];
   __shared__ double s_carry;
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -390,31 +406,31 @@ __global__ __launch_bounds__(FWG) void seg_flat_scan_kernel(
   // row of 64 packs
   unsigned lpo = 0;  // bit j: pack j's wave-exclusive prefix is open
   bool ropen = true;
-  double rval = 0.0;
+  A rval = acc_zero<A>();
 #pragma unroll
   for (int j = 0; j < FJ; ++j) {
     const int p = (wv * 64 * FJ + j * 64 + lane) * 4;
     const unsigned hb = (s_bits[p >> 5] >> (p & 31)) & 0xFu;
-    double ps = 0.0;
+    A ps = acc_zero<A>();
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
-      if (hb & (1u << k)) ps = 0.0;
-      ps += (double)x[j][k];
+      if (hb & (1u << k)) ps = acc_zero<A>();
+      ps = acc_add(ps, x[j][k]);
     }
     const unsigned long long M = __ballot(hb != 0);
     const unsigned long long le = M & ((2ull << lane) - 1ull);
     const int hl = le ? 63 - __builtin_clzll(le) : 0;
-    const double inc = seg_wave_scan(ps, lane, hl);
-    const double ex = wave_shr1_f64(inc);
+    const A inc = seg_wave_scan(ps, lane, hl);
+    const A ex = wave_shr1(inc);
     const bool eopen = (M & ((1ull << lane) - 1ull)) == 0;
-    const double rt = readlane63_f64(inc);
-    s_lpv[j][threadIdx.x] = eopen ? rval + ex : ex;
+    const A rt = readlane63(inc);
+    s_lpv[j][threadIdx.x] = eopen ? acc_add(rval, ex) : ex;
     if (eopen && ropen) lpo |= 1u << j;
     if (M) {
       ropen = false;
       rval = rt;
     } else {
-      rval += rt;
+      rval = acc_add(rval, rt);
     }
   }
   // keep the floats, not their FP64 conversions, live until the outputs
@@ -428,23 +444,24 @@ __global__ __launch_bounds__(FWG) void seg_flat_scan_kernel(
   }
   __syncthreads();
   bool wopen = true, topen = true;
-  double wval = 0.0, tval = 0.0;
+  A wval = acc_zero<A>(), tval = acc_zero<A>();
 #pragma unroll
   for (int w = 0; w < FWAVES; ++w) {
-    const double v = s_wval[w];
+    const A v = s_wval[w];
     const bool o = s_wopen[w] != 0;
     if (w < wv) {
-      wval = o ? wval + v : v;
+      wval = o ? acc_add(wval, v) : v;
       wopen = wopen && o;
     }
-    tval = o ? tval + v : v;
+    tval = o ? acc_add(tval, v) : v;
     topen = topen && o;
   }
+  const double tsum = acc_to_double(tval);
 
   // look-back (wave 0); a tile whose first element is a head needs no carry
   if (wv == 0) {
     const bool need = tile > 0 && (s_bits[0] & 1u) == 0;
-    if (lane == 0) publish(pub, (long long)tile, topen ? ST_AGG : ST_INCL, tval, epoch);
+    if (lane == 0) publish(pub, (long long)tile, topen ? ST_AGG : ST_INCL, tsum, epoch);
     double excl = 0.0;
     if (need) {
       long long wend = (long long)tile;  // window [wend - 64, wend)
@@ -468,26 +485,41 @@ __global__ __launch_bounds__(FWG) void seg_flat_scan_kernel(
       }
     }
     if (lane == 0) {
-      if (topen) publish(pub, (long long)tile, ST_INCL, excl + tval, epoch);
+      if (topen) publish(pub, (long long)tile, ST_INCL, excl + tsum, epoch);
       s_carry = excl;
     }
   }
   __syncthreads();
-  const double wcarry = wopen ? s_carry + wval : wval;
+  // an element's prefix = (the tile's carry, if no head precedes it in the
+  // tile) + (its sum in A since the tile's start or its last head); in the
+  // FP64 form the carry is folded into the sum at the pack's start
+  const double carry = s_carry;
 
 #pragma unroll
   for (int j = 0; j < FJ; ++j) {
     const int p = (wv * 64 * FJ + j * 64 + lane) * 4;
     const unsigned hb = (s_bits[p >> 5] >> (p & 31)) & 0xFu;
-    const double lp = s_lpv[j][threadIdx.x];
-    double r = (lpo >> j) & 1u ? wcarry + lp : lp;
+    const A lp = s_lpv[j][threadIdx.x];
+    const bool open = (lpo >> j) & 1u;
+    A r;
+    double c = 0.0;
+    if constexpr (__is_same(A, double)) {
+      const double wcarry = wopen ? carry + wval : wval;
+      r = open ? wcarry + lp : lp;
+    } else {
+      r = open ? acc_add(wval, lp) : lp;
+      c = open && wopen ? carry : 0.0;
+    }
     float o[4];
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
-      if (hb & (1u << k)) r = 0.0;
-      const double ex = r;
-      r += (double)x[j][k];
-      o[k] = (float)(inclusive ? r : ex);
+      if (hb & (1u << k)) {
+        r = acc_zero<A>();
+        c = 0.0;
+      }
+      const A ex = r;
+      r = acc_add(r, x[j][k]);
+      o[k] = (float)(c + acc_to_double(inclusive ? r : ex));
     }
     const unsigned long long e0 = wbase + ((unsigned long long)j * 64 + lane) * 4;
     if (full && (long long)e0 >= lo && (long long)e0 + 4 <= hi) {
@@ -552,7 +584,7 @@ extern "C" size_t skl_internal_segmented_workspace(unsigned long long n, int nse
 }
 
 extern "C" int skl_internal_segmented_sum(float* out, const float* in, unsigned long long n,
-                                          const int64_t* offsets, int nseg, void* ws,
+                                          const int64_t* offsets, int nseg, int fp64, void* ws,
                                           hipStream_t s)
 {
   const size_t cap = max_chunks(n, nseg, SUM_CH);
@@ -564,7 +596,10 @@ extern "C" int skl_internal_segmented_sum(float* out, const float* in, unsigned 
   double* partials = reinterpret_cast<double*>(reinterpret_cast<char*>(cseg) + pad256(cap * 4));
   seg_sum_prologue<<<1, 1024, 0, s>>>(offsets, nseg, n, (long long)cap, cstart, ctl);
   seg_sum_fill<<<(unsigned)((nseg + 3) / 4), 256, 0, s>>>(cstart, nseg, cseg, ctl);
-  seg_sum_kernel<<<(unsigned)cap, WG, 0, s>>>(in, offsets, cstart, cseg, partials, ctl);
+  if (fp64)
+    seg_sum_kernel<double><<<(unsigned)cap, WG, 0, s>>>(in, offsets, cstart, cseg, partials, ctl);
+  else
+    seg_sum_kernel<acc_df><<<(unsigned)cap, WG, 0, s>>>(in, offsets, cstart, cseg, partials, ctl);
   seg_combine_kernel<<<(unsigned)((nseg + WG / 64 - 1) / (WG / 64)), WG, 0, s>>>(
       cstart, nseg, partials, out, ctl);
   return hipGetLastError() == hipSuccess ? SKL_OK : SKL_ERR_HIP;
@@ -572,7 +607,7 @@ extern "C" int skl_internal_segmented_sum(float* out, const float* in, unsigned 
 
 extern "C" int skl_internal_segmented_scan(float* out, const float* in, unsigned long long n,
                                            const int64_t* offsets, int nseg, int inclusive,
-                                           void* ws, hipStream_t s)
+                                           int fp64, void* ws, hipStream_t s)
 {
   const long long ntiles = flat_tiles(n);
   if (ntiles > 0x7FFFFFFE) return SKL_ERR_INVALID;
@@ -584,7 +619,11 @@ extern "C" int skl_internal_segmented_scan(float* out, const float* in, unsigned
       reinterpret_cast<unsigned long long*>(p + 512 + flat_table_bytes(ntiles));
   seg_flat_prologue<<<(unsigned)((nseg + 2 + 255) / 256), 256, 0, s>>>(offsets, nseg, n, ntiles,
                                                                       tfirst, ticket, err);
-  seg_flat_scan_kernel<<<(unsigned)ntiles, FWG, 0, s>>>(in, out, n, offsets, nseg, tfirst,
-                                                        ticket, pub, inclusive ? 1 : 0, err);
+  if (fp64)
+    seg_flat_scan_kernel<double><<<(unsigned)ntiles, FWG, 0, s>>>(
+        in, out, n, offsets, nseg, tfirst, ticket, pub, inclusive ? 1 : 0, err);
+  else
+    seg_flat_scan_kernel<acc_df><<<(unsigned)ntiles, FWG, 0, s>>>(
+        in, out, n, offsets, nseg, tfirst, ticket, pub, inclusive ? 1 : 0, err);
   return hipGetLastError() == hipSuccess ? SKL_OK : SKL_ERR_HIP;
 }

@@ -57,8 +57,12 @@ struct skl_skeleton_s {
   // scratch of the batched segmented forms (zeroed when allocated)
   void* seg_scratch = nullptr;
   size_t seg_scratch_bytes = 0;
-  // the float (+) forms with compensated / FP64 combine
+  // float (+) without extra arguments
+  bool float_add() const;
+  // its DEFAULT form: compensated pairs in a block, FP64 across blocks
   bool precise_add() const;
+  // its FP64 form (precision FP64): FP64 throughout
+  bool fp64_add() const;
   // default Scan tile depth: about 256 bytes of accumulators per lane
   int default_scan_j() const;
   // launch shape defaults per kind (unroll, max_blocks, scan_j, mo_rb)

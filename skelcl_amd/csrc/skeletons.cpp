@@ -70,11 +70,14 @@ extern "C" int skl_common_definitions(char** text)
   return SKL_OK;
 }
 
-bool skl_skeleton_s::precise_add() const
+bool skl_skeleton_s::float_add() const
 {
-  return precision == SKL_PREC_DEFAULT && op == UserOp::Add && extras.empty() &&
-         !types.empty() && types[0].name == "float";
+  return op == UserOp::Add && extras.empty() && !types.empty() && types[0].name == "float";
 }
+
+bool skl_skeleton_s::precise_add() const { return precision == SKL_PREC_DEFAULT && float_add(); }
+
+bool skl_skeleton_s::fp64_add() const { return precision == SKL_PREC_FP64 && float_add(); }
 
 void skl_skeleton_s::apply_default_shape(int cus)
 {
@@ -98,9 +101,10 @@ int skl_skeleton_s::default_scan_j() const
 {
   // float (+) Scan, FP64 prefixes: 16 packs per lane measured fastest (2^28
   // and 577.5 M elements, forward and reversed) although the FP64 register
-  // tile is twice the float one
-  if (precise_add() && kind == SKL_SCAN) return 16;
-  const size_t acc = precise_add() ? 2 * sizeof(float) : (types.empty() ? 4 : types[0].size);
+  // tile is twice the float one.  Double-float pairs: 8 (256 bytes).
+  if (fp64_add() && kind == SKL_SCAN) return 16;
+  const size_t acc = precise_add() || fp64_add() ? 2 * sizeof(float)
+                                                 : (types.empty() ? 4 : types[0].size);
   const size_t per_pack = (size_t)(V > 0 ? V : 1) * (acc > 0 ? acc : 1);
   size_t j = 256 / per_pack;
   return (int)(j < 2 ? 2 : (j > 16 ? 16 : j));
@@ -296,10 +300,10 @@ std::vector<Param> extras_of(const UserFunction& f, int lead)
 // accumulator hooks for Reduce / Scan
 void emit_reduce_scan_hooks(std::ostringstream& o, const skl_skeleton_s* s)
 {
-  if (s->precise_add() && s->kind == SKL_SCAN) {
-    // float (+) Scan: FP64 tile prefixes and carry (one conversion and one
-    // FP64 add per element, against about six FP32 operations for a
-    // compensated pair), every output one rounding of the FP64 prefix
+  if (s->fp64_add()) {
+    // float (+), precision FP64: FP64 accumulators, tile prefixes and carry
+    // (one conversion and one FP64 add per element); every Scan output is
+    // one rounding of the FP64 prefix
     o << R"(
 typedef double SKL_ACC_T;
 typedef double SKL_CARRY_T;
@@ -316,6 +320,8 @@ __device__ __forceinline__ float skl_carry_out(const double& c) { return (float)
 __device__ __forceinline__ float skl_emit0(const double& P) { return (float)P; }
 )";
   } else if (s->precise_add()) {
+    // float (+), DEFAULT: double-float pairs inside the work-group / tile,
+    // FP64 across blocks (Reduce partials, Scan carries)
     o << R"(
 typedef skl_df SKL_ACC_T;
 typedef double SKL_CARRY_T;
@@ -643,6 +649,8 @@ static int setup_skeleton(const skl_skeleton_desc* d, skl_skeleton_s* s)
   // against 4.83 with 256 threads, float 13% faster); everything else 256
   s->wg = d->work_group_size > 0 ? d->work_group_size : (s->kind == SKL_SCAN ? 512 : 256);
   if (s->wg > 1024) return fail(SKL_ERR_INVALID, "work-group size %d exceeds 1024", s->wg);
+  if (d->precision < SKL_PREC_DEFAULT || d->precision > SKL_PREC_FP64)
+    return fail(SKL_ERR_INVALID, "unknown precision %d", d->precision);
   s->precision = d->precision;
   s->overlap = d->overlap_range;
   s->padding = d->padding;
@@ -837,7 +845,7 @@ extern "C" int skl_skeleton_set_work_group_size(skl_skeleton s, int wg)
 extern "C" int skl_skeleton_set_precision(skl_skeleton s, int precision)
 {
   if (!s) return fail(SKL_ERR_INVALID, "skeleton is NULL");
-  if (precision < SKL_PREC_DEFAULT || precision > SKL_PREC_FP8)
+  if (precision < SKL_PREC_DEFAULT || precision > SKL_PREC_FP64)
     return fail(SKL_ERR_INVALID, "unknown precision %d", precision);
   if (precision != s->precision) {
     const bool default_j = s->kind == SKL_SCAN && s->scan_j == s->default_scan_j();
@@ -1046,7 +1054,7 @@ extern "C" int skl_zip_run(skl_skeleton s, void* out, const void* left, const vo
 
 static size_t carry_size(const skl_skeleton_s* s)
 {
-  return s->precise_add() ? sizeof(double) : s->types[0].size;
+  return s->precise_add() || s->fp64_add() ? sizeof(double) : s->types[0].size;
 }
 
 // Reduce scratch: the top completion counter at 0, 64 sub-counters 256 bytes

@@ -164,20 +164,24 @@ def test_segmented_reduce_small(skl):
     assert np.all(np.abs(got - ref) <= np.spacing(np.abs(ref).astype(np.float32)) + 1e-12)
 
 
-def test_segmented_reduce_scan_cfg5_subset(skl):
+@pytest.mark.parametrize("prec", ["DEFAULT", "FP64"])
+def test_segmented_reduce_scan_cfg5_subset(skl, prec):
     """The batched configuration's segment lengths (log-uniform [1e3, 1e6),
-    committed fixture), first 256 segments."""
+    committed fixture), first 256 segments; compensated FP32 inside chunks
+    and tiles (DEFAULT) and FP64 throughout (FP64)."""
     with open(os.path.join(os.path.dirname(__file__), "golden", "batched_shapes.json")) as f:
         fx = json.load(f)
     lens = np.array(fx["segment_lengths"][:256], np.int64)
     off = np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)
     x = oracle.gen_f32_sym(0x5002, int(off[-1]))
     X = skl.Vector(x)
-    r = skl.Reduce("float(float)", "float func(float x, float y){ return x+y; }")
+    P = getattr(skl, "PREC_" + prec)
+    r = skl.Reduce("float(float)", "float func(float x, float y){ return x+y; }", precision=P)
     got = np.asarray(r.segmented(X, off)).astype(np.float64)
     ref = oracle.segmented_sum_f64(x, off)
     assert np.all(np.abs(got - ref) <= np.spacing(np.abs(ref).astype(np.float32)) + 1e-12)
-    s = skl.Scan("float(float)", "float func(float x, float y){ return x+y; }", "0.0f")
+    s = skl.Scan("float(float)", "float func(float x, float y){ return x+y; }", "0.0f",
+                 precision=P)
     for incl in (False, True):
         got = np.asarray(s.segmented(X, off, inclusive=incl)).astype(np.float64)
         ref = oracle.segmented_scan_f64(x, off, incl)

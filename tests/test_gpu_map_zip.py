@@ -170,6 +170,29 @@ def test_map_indexpoint_kat(skl):  # IndexMatrixTests.cpp:61-107
     assert np.array_equal(out, (c + 1000 * r).astype(np.int32))
 
 
+@pytest.mark.parametrize("n", [1, 255, 256, 100003])
+def test_map_void_index_writes_through_an_out_buffer(skl, n):
+    """Map<void(Index)> (src/Map.cpp:53-103): no output container; the user
+    function stores through its extra buffer, one call per index."""
+    o = skl.Vector(np.full(n, -1, np.int32))
+    m = skl.Map("void(Index)", "void func(Index i, __global int* o){ o[i] = 3 * (int)i + 1; }")
+    assert m(skl.IndexVector(n), skl.out(o)) is None
+    assert np.array_equal(np.asarray(o), 3 * np.arange(n, dtype=np.int32) + 1)
+
+
+@pytest.mark.parametrize("rows,cols", [(1, 1), (37, 129), (300, 65)])
+def test_map_void_indexpoint_writes_through_an_out_buffer(skl, rows, cols):
+    """Map<void(IndexPoint)> (src/Map.cpp:105-159): p.x is the column, p.y
+    the row; every point is visited once."""
+    o = skl.Vector(np.zeros(rows * cols, np.int32))
+    m = skl.Map("void(IndexPoint)",
+                "void func(IndexPoint p, __global int* o, int cols)"
+                "{ o[p.y * cols + p.x] += p.x - 7 * p.y + 1; }")
+    assert m(skl.IndexMatrix((rows, cols)), skl.out(o), cols) is None
+    r, c = np.mgrid[0:rows, 0:cols]
+    assert np.array_equal(np.asarray(o).reshape(rows, cols), (c - 7 * r + 1).astype(np.int32))
+
+
 def test_generators_bit_exact(skl):
     """The device generators reproduce the host SplitMix64 restatement."""
     n = 100003

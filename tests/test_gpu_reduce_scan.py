@@ -63,12 +63,14 @@ def test_reduce_int_wraps(skl):
 
 
 @pytest.mark.parametrize("n", [1, 100, 4099, 1 << 20, 3 * (1 << 21) + 1])
-def test_reduce_float_compensated_vs_fp64(skl, n):
-    """float Reduce(+) (compensated partials, FP64 cross-block combine):
-    |gpu - fp64| <= 2 ulp(|fp64|) + 1e-6*(sum|x|)*2^-24, and never worse
-    than the reference-order fp32 fold's own error."""
+@pytest.mark.parametrize("prec", ["DEFAULT", "FP64"])
+def test_reduce_float_compensated_vs_fp64(skl, n, prec):
+    """float Reduce(+), DEFAULT (compensated partials, FP64 cross-block
+    combine) and FP64 (FP64 accumulators): |gpu - fp64| <= 2 ulp(|fp64|) +
+    1e-6*(sum|x|)*2^-24, and never worse than the reference-order fp32
+    fold's own error."""
     x = oracle.gen_f32_sym(0x2002, n)
-    r = skl.Reduce("float(float)", ADD_F)
+    r = skl.Reduce("float(float)", ADD_F, precision=getattr(skl, "PREC_" + prec))
     got = float(r(skl.Vector(x))[0])
     ref = oracle.sum_f64(x)
     refo = float(oracle.reduce_add_f32_reforder(x))
@@ -175,12 +177,14 @@ def test_scan_repeated_launches(skl):
 
 @pytest.mark.parametrize("n", [1, 1000, 4097, (1 << 20) + 3])
 @pytest.mark.parametrize("inclusive", [False, True])
-def test_scan_float_vs_fp64(skl, n, inclusive):
-    """float Scan(+) with compensated tile prefixes and FP64 carries: every
-    prefix within 1 ulp of the correctly rounded FP64 prefix, plus a tiny
-    absolute term; the reference-order Blelloch result is reported."""
+@pytest.mark.parametrize("prec", ["DEFAULT", "FP64"])
+def test_scan_float_vs_fp64(skl, n, inclusive, prec):
+    """float Scan(+), DEFAULT (compensated FP32 tile prefixes, FP64 carries)
+    and FP64 (FP64 prefixes): every prefix within 1 ulp of the correctly
+    rounded FP64 prefix, plus a tiny absolute term; the reference-order
+    Blelloch result is reported."""
     x = oracle.gen_f32_sym(0x5002, n)
-    s = skl.Scan("float(float)", ADD_F, "0.0f")
+    s = skl.Scan("float(float)", ADD_F, "0.0f", precision=getattr(skl, "PREC_" + prec))
     out = np.asarray(s(skl.Vector(x), inclusive=inclusive)).astype(np.float64)
     ref = oracle.scan_add_f64(x, inclusive)
     err = np.abs(out - ref)
