@@ -376,37 +376,68 @@ __global__ __launch_bounds__(256 * MB, 1) void gemm_bf16x3_kernel(
 // of a block sees the same pattern.
 __device__ __forceinline__ int swz8(int r) { return (r ^ ((r >> 3) & 1)) & 7; }
 
-// One 128x128 tile of X (R x Cc row-major, or its transpose when `trans`)
-// per work-group: read once into registers, 4 consecutive elements of X's
-// contiguous dimension per load, absmax, a power-of-two scale s >=
-// absmax/448 (stored as a float whose exponent field is the E8M0 scale the
-// MFMA takes), e4m3 bytes by the hardware conversion (v_cvt_pk_fp8_f32,
-// round to nearest even; |x / s| <= 448 stays in range) staged in LDS and
-// written row-wise in 16-byte chunks.  `vec`: X is 16-byte aligned and its
-// row length is a multiple of 4, so in-bounds quads load as one float4.
-__global__ __launch_bounds__(256) void quant_fp8_kernel(const float* __restrict__ X, int R,
-                                                        int Cc, int trans, int vec,
-                                                        uint8_t* __restrict__ out,
-                                                        float* __restrict__ scales, int Rp,
-                                                        int Cp)
+// One FP8 operand to quantise: X is R x Cc row-major (trans = 0), or the
+// transpose of the stored matrix (trans = 1: X^T is read from a Cc x R
+// array); out / scales are the tiled bytes and the tile scales of an
+// Rp x Cp (multiples of 128) operand.
+struct QuantJob {
+  const float* X;
+  int R, Cc, trans, vec;
+  uint8_t* out;
+  float* scales;
+  int Rp, Cp;
+};
+
+// One 128x128 tile per work-group; blockIdx.z picks the operand (both in one
+// launch).  The tile is read once into registers, 16 bytes of X's
+// contiguous dimension per load; absmax (wave shuffles, one barrier); a
+// power-of-two scale s >= absmax/448 (stored as a float whose exponent
+// field is the E8M0 scale the MFMA takes); e4m3 bytes by the hardware
+// conversion (v_cvt_pk_fp8_f32, round to nearest even; |x / s| <= 448 stays
+// in range), four to a dword, staged in LDS rows of the output and written
+// row-wise in 16-byte chunks.  For trans each thread holds 4 x 4 blocks (4
+// lines of 4 elements) and transposes them in registers, so its LDS stores
+// are whole dwords; the 132-byte LDS row pitch spreads a wave's stores over
+// the banks.  `vec`: X is 16-byte aligned and its row length a multiple of
+// 4, so in-bounds quads load as one float4.
+constexpr int QP = 132;  // LDS bytes per staged row (dword aligned)
+__global__ __launch_bounds__(256) void quant_fp8_kernel(QuantJob ja, QuantJob jb)
 {
+  const QuantJob j = blockIdx.z ? jb : ja;
   const int tr = blockIdx.y, tc = blockIdx.x;
+  if (tr * 128 >= j.Rp || tc * 128 >= j.Cp) return;  // the grid covers the larger operand
   const int r0 = tr * 128, c0 = tc * 128;
-  __shared__ float red[256];
-  __shared__ __attribute__((aligned(16))) uint8_t qt[128][128 + 16];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  __shared__ float red[4];
+  __shared__ __attribute__((aligned(16))) uint8_t qt[128 * QP];
   f32x4 v[16];
   float amax = 0.0f;
-  // quad u: 32 quads per line of the contiguous dimension, 128 lines
-  const size_t ld = trans ? (size_t)R : (size_t)Cc;
-  const int nl = trans ? Cc : R, nc = trans ? R : Cc;   // lines, length
-  const int l0 = trans ? c0 : r0, e0 = trans ? r0 : c0;  // tile origin
+  const float* X = j.X;
+  const size_t ld = j.trans ? (size_t)j.R : (size_t)j.Cc;
+  const int nl = j.trans ? j.Cc : j.R, nc = j.trans ? j.R : j.Cc;  // lines, length
+  const int l0 = j.trans ? c0 : r0, e0 = j.trans ? r0 : c0;        // tile origin
+  // element quad u: line l, elements e .. e + 3.  Not trans: 32 quads per
+  // line, 128 lines.  Trans: u = 4 g + r is line 4 kq + r of block g, whose
+  // quad index along the line is nq (lanes: kq = lane % 8, nq = lane / 8).
+  auto quad = [&](int u, int& l, int& e) {
+    if (!j.trans) {
+      const int q = threadIdx.x + 256 * u;
+      l = l0 + (q >> 5);
+      e = e0 + (q & 31) * 4;
+    } else {
+      const int g = u >> 2, r = u & 3;
+      const int kq = (lane & 7) + 8 * g, nq = (lane >> 3) + 8 * wv;
+      l = l0 + 4 * kq + r;
+      e = e0 + 4 * nq;
+    }
+  };
 #pragma unroll
   for (int u = 0; u < 16; ++u) {
-    const int q = threadIdx.x + 256 * u;
-    const int l = l0 + (q >> 5), e = e0 + (q & 31) * 4;
+    int l, e;
+    quad(u, l, e);
     const bool lok = l < nl;
     const size_t base = (size_t)(lok ? l : 0) * ld;
-    if (vec && lok && e + 3 < nc) {
+    if (j.vec && lok && e + 3 < nc) {
       v[u] = *as_global(reinterpret_cast<const f32x4*>(X + base + e));
     } else {
 #pragma unroll
@@ -415,44 +446,53 @@ __global__ __launch_bounds__(256) void quant_fp8_kernel(const float* __restrict_
 #pragma unroll
     for (int i = 0; i < 4; ++i) amax = fmaxf(amax, fabsf(v[u][i]));
   }
-  red[threadIdx.x] = amax;
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) amax = fmaxf(amax, __shfl_xor(amax, o, 64));
+  if (lane == 0) red[wv] = amax;
   __syncthreads();
-  for (int k = 128; k > 0; k >>= 1) {
-    if ((int)threadIdx.x < k) red[threadIdx.x] = fmaxf(red[threadIdx.x], red[threadIdx.x + k]);
-    __syncthreads();
-  }
+  amax = fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]));
   int ex = 0;
-  if (red[0] > 0.0f) frexpf(red[0] / 448.0f, &ex);  // absmax/448 = m * 2^ex, m in [0.5, 1)
+  if (amax > 0.0f) frexpf(amax / 448.0f, &ex);  // absmax/448 = m * 2^ex, m in [0.5, 1)
   // 2^-127 is the smallest E8M0 scale (exponent field 0 of the float
   // below); tinier tiles flush towards zero in e4m3 instead of making
   // 1/scale overflow
   ex = ex < -127 ? -127 : ex;
   const float scale = ldexpf(1.0f, ex);
   const float inv = ldexpf(1.0f, -ex);
-  if (threadIdx.x == 0) scales[tr * (Cp / 128) + tc] = scale;
+  if (threadIdx.x == 0) j.scales[tr * (j.Cp / 128) + tc] = scale;
+  if (!j.trans) {
 #pragma unroll
-  for (int u = 0; u < 16; ++u) {
-    const int q = threadIdx.x + 256 * u;
-    const int ll = q >> 5, le = (q & 31) * 4;
-    int w = __builtin_amdgcn_cvt_pk_fp8_f32(v[u][0] * inv, v[u][1] * inv, 0, false);
-    w = __builtin_amdgcn_cvt_pk_fp8_f32(v[u][2] * inv, v[u][3] * inv, w, true);
-    if (!trans) {
-      *reinterpret_cast<int*>(&qt[ll][le]) = w;
-    } else {
+    for (int u = 0; u < 16; ++u) {
+      const int q = threadIdx.x + 256 * u;
+      int w = __builtin_amdgcn_cvt_pk_fp8_f32(v[u][0] * inv, v[u][1] * inv, 0, false);
+      w = __builtin_amdgcn_cvt_pk_fp8_f32(v[u][2] * inv, v[u][3] * inv, w, true);
+      *reinterpret_cast<int*>(&qt[(q >> 5) * QP + (q & 31) * 4]) = w;
+    }
+  } else {
 #pragma unroll
-      for (int i = 0; i < 4; ++i) qt[le + i][ll] = (uint8_t)(w >> (8 * i));
+    for (int g = 0; g < 4; ++g) {
+      const int kq = (lane & 7) + 8 * g, nq = (lane >> 3) + 8 * wv;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {  // output row 4 nq + i: lines 4 kq .. 4 kq + 3
+        int w = __builtin_amdgcn_cvt_pk_fp8_f32(v[4 * g][i] * inv, v[4 * g + 1][i] * inv, 0,
+                                                false);
+        w = __builtin_amdgcn_cvt_pk_fp8_f32(v[4 * g + 2][i] * inv, v[4 * g + 3][i] * inv, w,
+                                            true);
+        *reinterpret_cast<int*>(&qt[(4 * nq + i) * QP + 4 * kq]) = w;
+      }
     }
   }
   __syncthreads();
   // tiled output: this tile is block tr * (Cp / 128) + tc, rows of 128
   // bytes, chunk kc of row r at position kc ^ swz8(r)
-  uint8_t* blk = out + ((size_t)tr * (Cp / 128) + tc) * (128 * 128);
+  uint8_t* blk = j.out + ((size_t)tr * (j.Cp / 128) + tc) * (128 * 128);
 #pragma unroll
-  for (int j = 0; j < 4; ++j) {
-    const int q = threadIdx.x + 256 * j;  // 128 rows x 8 chunks of 16 bytes
+  for (int u = 0; u < 4; ++u) {
+    const int q = threadIdx.x + 256 * u;  // 128 rows x 8 chunks of 16 bytes
     const int lr = q >> 3, kc = q & 7;
+    const int* src = reinterpret_cast<const int*>(&qt[lr * QP + kc * 16]);
     *reinterpret_cast<i32x4*>(blk + lr * 128 + ((kc ^ swz8(lr)) << 4)) =
-        *reinterpret_cast<const i32x4*>(&qt[lr][kc * 16]);
+        i32x4{src[0], src[1], src[2], src[3]};
   }
 }
 
@@ -460,9 +500,9 @@ __global__ __launch_bounds__(256) void quant_fp8_kernel(const float* __restrict_
 // Np/128 row blocks); sa: (Mp/128) x (Kp/128), sb: (Np/128) x (Kp/128).
 // Work-group tile MB x 128 rows by NB x 128 columns; waves of WR x 16 rows by
 // 64 columns (WR 16-row MFMA tiles, 4 16-column ones).  One 128-k block per
-// slice, two LDS buffers filled by global_load_lds (the fill of slice k+2 is
-// issued after the barrier that ends slice k, so it has one slice of MFMA
-// work to arrive).
+// slice, two LDS buffers filled by global_load_lds: after the barrier that
+// starts slice k the fill of slice k+1 is issued into the buffer slice k-1
+// read, so it has one slice of MFMA work to arrive.
 template <int MB, int NB, int WR>
 __global__ __launch_bounds__(MB * NB * 128 * 128 / (WR * 16 * 64) * 64, 1) void gemm_fp8_kernel(
     const uint8_t* __restrict__ Aq, const uint8_t* __restrict__ Bq,
@@ -477,6 +517,9 @@ __global__ __launch_bounds__(MB * NB * 128 * 128 / (WR * 16 * 64) * 64, 1) void 
   constexpr int PB = NB * (BLK / 1024) / WAVES;  // and of B
   static_assert(PA * WAVES == MB * (BLK / 1024) && PB * WAVES == NB * (BLK / 1024),
                 "fill pieces must split evenly over the waves");
+  // one named array per buffer: the compiler then knows that a fill of one
+  // buffer does not write what the LDS reads of the other read, and waits
+  // for the fills only where the code says so
   __shared__ __attribute__((aligned(1024))) uint8_t As0[MB * BLK], Bs0[NB * BLK];
   __shared__ __attribute__((aligned(1024))) uint8_t As1[MB * BLK], Bs1[NB * BLK];
   const int tid = threadIdx.x, lane = tid & 63;
@@ -537,7 +580,7 @@ __global__ __launch_bounds__(MB * NB * 128 * 128 / (WR * 16 * 64) * 64, 1) void 
   // the tile scales are powers of two: their exponent fields are the E8M0
   // block scales of the MFMA, which then accumulates directly
   const int ablock = tm * MB + (arow0 >> 7), bblock = tn * NB + (wn >> 1);
-  auto slice = [&](uint8_t* Ac, uint8_t* Bc, int k) {
+  auto slice = [&](const uint8_t* Ac, const uint8_t* Bc, int k) {
     const int ea = (int)((__float_as_uint(sa[(size_t)ablock * nkb + k]) >> 23) & 0xFF);
     const int eb = (int)((__float_as_uint(sb[(size_t)bblock * nkb + k]) >> 23) & 0xFF);
     // B fragments for the whole slice; A fragments one ahead of their four
@@ -558,22 +601,28 @@ __global__ __launch_bounds__(MB * NB * 128 * 128 / (WR * 16 * 64) * 64, 1) void 
       a = an;
     }
     __builtin_amdgcn_sched_barrier(0);
-    __builtin_amdgcn_s_waitcnt(0);
-    __syncthreads();
-    fill(Ac, Bc, k + 2);
+  };
+  // slice k from (Ac, Bc); the other buffer (An, Bn), which slice k - 1
+  // read, is refilled with slice k + 1
+  auto step = [&](const uint8_t* Ac, const uint8_t* Bc, uint8_t* An, uint8_t* Bn, int k) {
+    __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0): this wave's part of fill k has landed
+    // every wave's part, and every wave is done with slice k - 1.  A plain
+    // barrier (a release fence, as in __syncthreads, would add nothing after
+    // the wait above); the memory clobber keeps the slice's LDS reads below
+    // it.
+    asm volatile("s_barrier" ::: "memory");
+    fill(An, Bn, k + 1);
     __builtin_amdgcn_sched_barrier(0);
+    slice(Ac, Bc, k);
   };
 
   fill(As0, Bs0, 0);
-  fill(As1, Bs1, 1);
-  __builtin_amdgcn_s_waitcnt(0);
-  __syncthreads();
   int k = 0;
   for (; k + 2 <= nkb; k += 2) {
-    slice(As0, Bs0, k);
-    slice(As1, Bs1, k + 1);
+    step(As0, Bs0, As1, Bs1, k);
+    step(As1, Bs1, As0, Bs0, k + 1);
   }
-  if (k < nkb) slice(As0, Bs0, k);
+  if (k < nkb) step(As0, Bs0, As1, Bs1, k);
   __builtin_amdgcn_s_waitcnt(0);  // no fill may still be writing LDS at the end
 
   const int cc = lane & 15, cr = (lane >> 4) * 4;
@@ -785,12 +834,18 @@ extern "C" int skl_internal_matmul(float* C, const float* A, const float* B, int
     float* sb = sa + (size_t)(Ma / 128) * (Kp / 128);
     const int va = ((uintptr_t)A % 16 == 0) && ((ta ? M : K) % 4 == 0);
     const int vb = ((uintptr_t)B % 16 == 0) && ((tb ? K : N) % 4 == 0);
-    quant_fp8_kernel<<<dim3(Kp / 128, Ma / 128), 256, 0, s>>>(A, M, K, ta, va, Aq, sa, Ma, Kp);
-    // B^T: rows n, columns k; element (n, k) = B[k][n] (or B[n][k] when tb)
-    quant_fp8_kernel<<<dim3(Kp / 128, Na / 128), 256, 0, s>>>(B, N, K, tb ? 0 : 1, vb, Bq, sb, Na,
-                                                               Kp);
-    // 256 x 128 tiles (8 waves of 64 x 64) when every CU gets one
-    if ((long long)(Ma / 256) * (Na / 128) >= device_cus()) {
+    // A, and B^T: rows n, columns k; element (n, k) = B[k][n] (or B[n][k]
+    // when tb); one launch for both
+    const QuantJob qa{A, M, K, ta, va, Aq, sa, Ma, Kp};
+    const QuantJob qb{B, N, K, tb ? 0 : 1, vb, Bq, sb, Na, Kp};
+    quant_fp8_kernel<<<dim3(Kp / 128, std::max(Ma, Na) / 128, 2), 256, 0, s>>>(qa, qb);
+    // 256 x 256 tiles (8 waves of 128 x 64: per MFMA 3/4 of the LDS reads
+    // and 2/3 of the fill bytes of 256 x 128) when every CU gets one; at
+    // 8192^3 465 us against 590.  Then 256 x 128, then 128 x 128.
+    if ((long long)(Ma / 256) * (Na / 256) >= device_cus()) {
+      gemm_fp8_kernel<2, 2, 8><<<dim3(Na / 256, Ma / 256), 512, 0, s>>>(Aq, Bq, sa, sb, C, M, N,
+                                                                         Kp, N);
+    } else if ((long long)(Ma / 256) * (Na / 128) >= device_cus()) {
       gemm_fp8_kernel<2, 1, 4><<<dim3(Na / 128, Ma / 256), 512, 0, s>>>(Aq, Bq, sa, sb, C, M, N,
                                                                          Kp, N);
     } else {
