@@ -507,7 +507,10 @@ def main() -> int:
         for name, sk in (("scan", scan), ("reduce", red), ("saxpy", sax)):
             sh = (C.c_int * 4)()
             ck(L.skl_skeleton_launch_shape(sk.handle, sh))
+            wg = C.c_int()
+            ck(L.skl_skeleton_work_group_size(sk.handle, C.byref(wg)))
             shapes[name] = dict(zip(("unroll", "blocks_per_cu", "scan_j", "mo_rb"), list(sh)))
+            shapes[name]["work_group_size"] = wg.value
         progress("launch shapes tuned")
 
     def run_scan():

@@ -140,6 +140,9 @@ int skl_skeleton_create(const skl_skeleton_desc* desc, skl_skeleton* out);
 int skl_skeleton_release(skl_skeleton s);
 /* Skeleton::setWorkGroupSize (src/Skeleton.cpp:75-78) */
 int skl_skeleton_set_work_group_size(skl_skeleton s, int wg);
+/* the work-group size in use (Skeleton::workGroupSize,
+ * include/SkelCL/detail/Skeleton.h:64) */
+int skl_skeleton_work_group_size(skl_skeleton s, int* wg);
 int skl_skeleton_set_precision(skl_skeleton s, int precision);
 /* the generated kernel source (for inspection); free with skl_free_string */
 int skl_skeleton_source(skl_skeleton s, char** source);
@@ -302,14 +305,19 @@ int skl_matmul_batched_f32(const skl_gemm_desc* descs_device, int count,
                            int precision, skl_stream stream);
 
 /* Autotuner: time candidate launch shapes for this skeleton at size n
- * (elements; for MapOverlap an n-pixel square, for AllPairs n-sized square
- * products) on scratch buffers, keep the fastest and cache it per
- * (kind, user source, types, log2 size) in memory and in
- * $SKELCL_CACHE_DIR/autotune.txt.  Tuned: packs in flight per thread and
- * grid cap (Map/Zip/Reduce), packs per lane (Scan tile), rows per thread
- * (MapOverlap).  enable = 0 restores the defaults.  skl_autotune needs a user
- * function without extra parameters; skl_autotune_args times with the
- * caller's extra arguments. */
+ * (elements; for MapOverlap an n-pixel square image, for AllPairs a product
+ * with n output elements and square operands, sqrt(n) on each side) on
+ * scratch buffers, keep the fastest and cache it per (kind, user source,
+ * types, log2 size) in memory and in $SKELCL_CACHE_DIR/autotune_v2.txt.
+ * Tuned: work-group size (256 / 512 for Map/Zip/Reduce, 256 / 512 / 1024 for
+ * Scan) with packs in flight per thread and grid cap (Map/Zip/Reduce) or
+ * packs per lane (Scan tile); rows per thread (MapOverlap); the MFMA
+ * work-group tile of the AllPairs (+,*) products at BF16X3 (128 x 128,
+ * 256 x 128) and FP8 (128 x 128, 256 x 128, 256 x 256).  Other AllPairs forms
+ * and the Index maps have fixed shapes.  A tuned work-group size replaces
+ * one set by skl_skeleton_set_work_group_size.  enable = 0 restores the
+ * defaults.  skl_autotune needs a user function without extra parameters;
+ * skl_autotune_args times with the caller's extra arguments. */
 int skl_autotune(skl_skeleton s, size_t n, int enable);
 int skl_autotune_args(skl_skeleton s, size_t n, const skl_arg* args, int nargs);
 /* the tuned shape in use: {unroll, grid blocks per CU, scan packs per lane,
@@ -319,6 +327,13 @@ int skl_skeleton_launch_shape(skl_skeleton s, int shape[4]);
  * scan packs 1-32, rows 1-16).  No counterpart in the reference (its launch
  * shapes are fixed): for callers that tune offline or pin a measured shape. */
 int skl_skeleton_set_launch_shape(skl_skeleton s, const int shape[4]);
+/* The MFMA work-group tile of an AllPairs (+,*) product at BF16X3 / FP8
+ * (what skl_autotune tunes for it): 0 = chosen by the product's size (the
+ * default), 1 = 128 x 128, 2 = 256 x 128, 3 = 256 x 256 (FP8; BF16X3 takes
+ * 256 x 128).  Results do not depend on it.  No counterpart in the
+ * reference: for callers that tune offline or pin a measured tile. */
+int skl_skeleton_set_mma_tile(skl_skeleton s, int tile);
+int skl_skeleton_mma_tile(skl_skeleton s, int* tile);
 
 #ifdef __cplusplus
 }

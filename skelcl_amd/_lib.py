@@ -115,6 +115,7 @@ SIGNATURES = {
     "skl_skeleton_create": (_i, [C.POINTER(skl_skeleton_desc), C.POINTER(_vp)]),
     "skl_skeleton_release": (_i, [_vp]),
     "skl_skeleton_set_work_group_size": (_i, [_vp, _i]),
+    "skl_skeleton_work_group_size": (_i, [_vp, C.POINTER(_i)]),
     "skl_skeleton_set_precision": (_i, [_vp, _i]),
     "skl_skeleton_source": (_i, [_vp, C.POINTER(_vp)]),
     "skl_skeleton_check": (_i, [C.POINTER(skl_skeleton_desc), C.POINTER(_vp),
@@ -147,6 +148,8 @@ SIGNATURES = {
     "skl_autotune_args": (_i, [_vp, _sz, _argp, _i]),
     "skl_skeleton_launch_shape": (_i, [_vp, C.POINTER(_i)]),
     "skl_skeleton_set_launch_shape": (_i, [_vp, C.POINTER(_i)]),
+    "skl_skeleton_set_mma_tile": (_i, [_vp, _i]),
+    "skl_skeleton_mma_tile": (_i, [_vp, C.POINTER(_i)]),
 }
 
 _lib = None

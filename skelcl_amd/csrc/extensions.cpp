@@ -16,7 +16,7 @@ using namespace skl;
 
 extern "C" size_t skl_internal_matmul_workspace(size_t M, size_t N, size_t K, int precision);
 extern "C" int skl_internal_matmul(float* C, const float* A, const float* B, int M, int N,
-                                   int K, int ta, int tb, int precision, void* ws,
+                                   int K, int ta, int tb, int precision, int tile, void* ws,
                                    hipStream_t s);
 extern "C" int skl_internal_stencil_transpose(float* din, const float* dout, const float* w,
                                               int rows, int cols, int R, int nearest,
@@ -93,9 +93,8 @@ bool is_float_addmul(const skl_skeleton_s* s)
 }
 }  // namespace
 
-extern "C" int skl_matmul_f32(float* C, const float* A, const float* B, size_t M, size_t N,
-                              size_t K, int trans_a, int trans_b, int precision,
-                              skl_stream stream)
+int skl::matmul_run(float* C, const float* A, const float* B, size_t M, size_t N, size_t K,
+                    int trans_a, int trans_b, int precision, int tile, skl_stream stream)
 {
   SKL_TRY(ensure_initialized());
   if (M == 0 || N == 0) return SKL_OK;
@@ -109,9 +108,16 @@ extern "C" int skl_matmul_f32(float* C, const float* A, const float* B, size_t M
   size_t wsb = skl_internal_matmul_workspace(M, N, K, precision);
   if (wsb) SKL_TRY(workspace(wsb, &ws));
   int st = skl_internal_matmul(C, A, B, (int)M, (int)N, (int)K, trans_a ? 1 : 0,
-                               trans_b ? 1 : 0, precision, ws, resolve_stream(stream));
+                               trans_b ? 1 : 0, precision, tile, ws, resolve_stream(stream));
   if (st != SKL_OK) return fail(st, "matmul launch failed: %s", hipGetErrorString(hipGetLastError()));
   return SKL_OK;
+}
+
+extern "C" int skl_matmul_f32(float* C, const float* A, const float* B, size_t M, size_t N,
+                              size_t K, int trans_a, int trans_b, int precision,
+                              skl_stream stream)
+{
+  return skl::matmul_run(C, A, B, M, N, K, trans_a, trans_b, precision, 0, stream);
 }
 
 extern "C" int skl_allpairs_backward(skl_skeleton s, void* d_left, void* d_right,
@@ -124,8 +130,11 @@ extern "C" int skl_allpairs_backward(skl_skeleton s, void* d_left, void* d_right
     return fail(SKL_ERR_UNSUPPORTED,
                 "AllPairs backward needs the (+,*) contraction on float (a matmul)");
   if (!d_out) return fail(SKL_ERR_INVALID, "d_out is NULL");
-  int prec = s->precision == SKL_PREC_DEFAULT ? SKL_PREC_FP32 : s->precision;
-  // C = A.B with A rows x dim, B dim x cols
+  // FP64 (a Reduce/Scan form) runs the products as DEFAULT: plain FP32
+  int prec = s->precision == SKL_PREC_DEFAULT || s->precision == SKL_PREC_FP64 ? SKL_PREC_FP32
+                                                                                : s->precision;
+  // C = A.B with A rows x dim, B dim x cols; the tuned tile is the forward
+  // product's, so the adjoint products take the size rule
   if (d_left)  // dA = dC . B^T : (rows x cols) . (cols x dim)
     SKL_TRY(skl_matmul_f32((float*)d_left, (const float*)d_out, (const float*)right, rows, dim,
                            cols, 0, 1, prec, stream));

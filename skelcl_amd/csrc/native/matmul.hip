@@ -788,8 +788,11 @@ extern "C" size_t skl_internal_matmul_workspace(size_t M, size_t N, size_t K, in
   return 0;
 }
 
+// tile: the MFMA work-group tile, 0 = by the size rule below.  BF16X3: 1 =
+// 128 x 128, 2 or 3 = 256 x 128.  FP8: 1 = 128 x 128, 2 = 256 x 128, 3 =
+// 256 x 256.
 extern "C" int skl_internal_matmul(float* C, const float* A, const float* B, int M, int N,
-                                   int K, int ta, int tb, int precision, void* ws,
+                                   int K, int ta, int tb, int precision, int tile, void* ws,
                                    hipStream_t s)
 {
   const int Np = (int)round_up(N, 128);
@@ -818,7 +821,8 @@ extern "C" int skl_internal_matmul(float* C, const float* A, const float* B, int
     }
     // 256 x 128 tiles (8 waves, twice the MFMA work per fill) when there are
     // enough of them for every CU; 128 x 128 otherwise
-    if ((long long)(Ma / 256) * (Np / BN) >= device_cus()) {
+    if (tile == 0) tile = (long long)(Ma / 256) * (Np / BN) >= device_cus() ? 2 : 1;
+    if (tile >= 2) {
       gemm_bf16x3_kernel<2><<<dim3(Np / BN, Ma / 256), 512, 0, s>>>(Ap, Bp, C, M, N, Kp, N);
     } else {
       gemm_bf16x3_kernel<1><<<dim3(Np / BN, Ma / 128), 256, 0, s>>>(Ap, Bp, C, M, N, Kp, N);
@@ -842,10 +846,14 @@ extern "C" int skl_internal_matmul(float* C, const float* A, const float* B, int
     // 256 x 256 tiles (8 waves of 128 x 64: per MFMA 3/4 of the LDS reads
     // and 2/3 of the fill bytes of 256 x 128) when every CU gets one; at
     // 8192^3 465 us against 590.  Then 256 x 128, then 128 x 128.
-    if ((long long)(Ma / 256) * (Na / 256) >= device_cus()) {
+    if (tile == 0)
+      tile = (long long)(Ma / 256) * (Na / 256) >= device_cus()   ? 3
+             : (long long)(Ma / 256) * (Na / 128) >= device_cus() ? 2
+                                                                   : 1;
+    if (tile == 3) {
       gemm_fp8_kernel<2, 2, 8><<<dim3(Na / 256, Ma / 256), 512, 0, s>>>(Aq, Bq, sa, sb, C, M, N,
                                                                          Kp, N);
-    } else if ((long long)(Ma / 256) * (Na / 128) >= device_cus()) {
+    } else if (tile == 2) {
       gemm_fp8_kernel<2, 1, 4><<<dim3(Na / 128, Ma / 256), 512, 0, s>>>(Aq, Bq, sa, sb, C, M, N,
                                                                          Kp, N);
     } else {

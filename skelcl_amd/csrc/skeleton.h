@@ -49,6 +49,7 @@ struct skl_skeleton_s {
   int unroll = 4;   // packs in flight per thread (Map/Zip/Reduce)
   int scan_j = 4;   // packs per lane (Scan tile = 256*J*V)
   int mo_rb = 4;    // MapOverlap rows per thread
+  int mm_tile = 0;  // AllPairs (+,*) MFMA tile (skl_internal_matmul), 0 = size rule
   int max_blocks = 0;
 
   // scratch: [0,8) ticket, [8,12) epoch, partials / look-back words at 256
@@ -76,6 +77,10 @@ int rebuild_programs(skl_skeleton_s* s);
 // Reduce with caller-provided scratch: no allocation and no build
 // (skeletons.cpp); the layout needs `counter_bytes` of zeroed counters at
 // the start of `total_bytes`.
+// AllPairs (+,*) / matmul on float with the MFMA tile `tile` (0: by size)
+// (extensions.cpp)
+int matmul_run(float* C, const float* A, const float* B, size_t M, size_t N, size_t K,
+               int trans_a, int trans_b, int precision, int tile, skl_stream stream);
 int reduce_scratch_need(skl_skeleton_s* s, size_t n, size_t* counter_bytes, size_t* total_bytes);
 int reduce_run_fixed_scratch(skl_skeleton_s* s, void* out, const void* in, size_t n,
                              const skl_arg* args, int nargs, skl_stream stream, void* scratch,

@@ -833,6 +833,13 @@ extern "C" int skl_skeleton_release(skl_skeleton s)
   return SKL_OK;
 }
 
+extern "C" int skl_skeleton_work_group_size(skl_skeleton s, int* wg)
+{
+  if (!s || !wg) return fail(SKL_ERR_INVALID, "NULL argument");
+  *wg = s->wg;
+  return SKL_OK;
+}
+
 extern "C" int skl_skeleton_set_work_group_size(skl_skeleton s, int wg)
 {
   if (!s) return fail(SKL_ERR_INVALID, "skeleton is NULL");
@@ -1331,8 +1338,9 @@ extern "C" int skl_allpairs_run(skl_skeleton s, void* out, const void* left, con
     // takes the in-order FMA fold at every precision.
     const bool product_form = s->precision == SKL_PREC_BF16X3 || s->precision == SKL_PREC_FP8;
     if ((key == "fma" || (float_addmul(s) && product_form)) && identity_is_zero(s->identity))
-      return skl_matmul_f32((float*)out, (const float*)left, (const float*)right, rows, cols,
-                            dim, 0, 0, s->precision, stream);
+      return matmul_run((float*)out, (const float*)left, (const float*)right, rows, cols, dim,
+                        0, 0, s->precision == SKL_PREC_FP64 ? SKL_PREC_DEFAULT : s->precision,
+                        s->mm_tile, stream);
     if (pk.dyn_lds) return fail(SKL_ERR_UNSUPPORTED, "AllPairs user functions cannot take Local");
     SKL_TRY(get_kernel(s, key, "skl_allpairs", &k));
     unsigned gy = (unsigned)((rows + 63) / 64);

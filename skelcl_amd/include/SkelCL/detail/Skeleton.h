@@ -197,6 +197,12 @@ public:
     if (_h) skl_skeleton_release(_h);
   }
   // Skeleton::setWorkGroupSize (src/Skeleton.cpp:75-78)
+  size_t workGroupSize() const
+  {
+    int wg = 0;
+    check(skl_skeleton_work_group_size(_h, &wg), "workGroupSize");
+    return (size_t)wg;
+  }
   void setWorkGroupSize(unsigned int wg)
   {
     check(skl_skeleton_set_work_group_size(_h, (int)wg), "setWorkGroupSize");

@@ -474,6 +474,40 @@ def test_autotune_keeps_results_and_caches(skl, tmp_path, monkeypatch):
     _lib.check(L.skl_autotune(m.handle, 0, 0))  # back to defaults
 
 
+@pytest.mark.parametrize("prec", ["BF16X3", "FP8"])
+def test_mma_tiles_agree_and_autotune_picks_one(skl, prec, tmp_path, monkeypatch):
+    """Every MFMA work-group tile of the AllPairs (+,*) product gives the
+    same bits (each output tile runs the same MFMAs in the same k order), on
+    ragged shapes where the size rule would not pick the larger tiles; the
+    autotuner keeps the size rule or picks one of them, and the result stays
+    the same."""
+    import ctypes as C
+    from skelcl_amd import _lib
+    monkeypatch.setenv("SKELCL_CACHE_DIR", str(tmp_path))
+    L = _lib.lib()
+    P = getattr(skl, "PREC_" + prec)
+    add = skl.Reduce("float(float)", "float func(float x, float y){ return x+y; }", "0")
+    mul = skl.Zip("float(float, float)", "float func(float x, float y){ return x*y; }")
+    ap = skl.AllPairs("float(float, float)", reduce=add, zip=mul, precision=P)
+    for (M, N, K) in [(300, 517, 260), (512, 512, 512)]:
+        A = skl.Matrix(oracle.gen_f32_sym(0x7101, M * K).reshape(M, K))
+        B = skl.Matrix(oracle.gen_f32_sym(0x7102, K * N).reshape(K, N))
+        outs = []
+        for tile in (0, 1, 2, 3):
+            _lib.check(L.skl_skeleton_set_mma_tile(ap.handle, tile))
+            print(f"{prec} {M}x{N}x{K} tile {tile}", flush=True)
+            outs.append(np.asarray(ap(A, B)).view(np.uint32).copy())
+        for o in outs[1:]:
+            assert np.array_equal(o, outs[0])
+    _lib.check(L.skl_skeleton_set_mma_tile(ap.handle, 0))
+    _lib.check(L.skl_autotune(ap.handle, 512 * 512, 1))
+    t = C.c_int(-1)
+    _lib.check(L.skl_skeleton_mma_tile(ap.handle, C.byref(t)))
+    # 0: the size rule stayed (no tile was clearly faster)
+    assert t.value in ((0, 1, 2) if prec == "BF16X3" else (0, 1, 2, 3))
+    assert np.array_equal(np.asarray(ap(A, B)).view(np.uint32), outs[0])
+
+
 def test_fused_reduce_follows_a_new_zip(skl):
     """One Reduce fused with a Zip, the Zip released, then a Zip with another
     function (possibly at the released one's address): the fused program
