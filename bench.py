@@ -417,6 +417,42 @@ def run_extras(skelcl, L, ck, stream, args):
                       reps=3)
             bt[name] = {"ms": round(t, 3), "TFLOPs": round(flops / (t * 1e-3) / 1e12, 1)}
         hs = sums.to_numpy(np.float32)
+        # the whole batched pipeline (segmented Reduce: 4 launches, segmented
+        # Scan: 2, batched BF16X3 products: 3) eager and as one captured
+        # hipGraph, at full size and on the first 64 / 8 segments and products
+        scan_d = skelcl.Scan("float(float)", ADD, "0.0f")
+        subsets = {}
+        for nsm in (64, 8):
+            dd_sm = skelcl.DeviceBuffer(C.sizeof(descs))
+            ck(L.skl_memcpy_h2d(dd_sm.ptr, 0, C.cast(descs, C.c_void_p),
+                                C.sizeof(skelcl._lib.skl_gemm_desc) * nsm, None, 1))
+            subsets[f"first{nsm}"] = (nsm, skelcl.DeviceBuffer.from_array(off[:nsm + 1]),
+                                      int(off[nsm]), dd_sm, nsm)
+
+        def pipeline(nseg, ob, ne, descs_dev, ng):
+            ck(L.skl_reduce_segmented_run(add.handle, sums.ptr, segx.ptr, ne, ob.ptr, nseg, st))
+            ck(L.skl_scan_segmented_run(scan_d.handle, segy.ptr, segx.ptr, ne, ob.ptr, nseg, 1,
+                                        st))
+            ck(L.skl_matmul_batched_f32(descs_dev.ptr, ng, skelcl.PREC_BF16X3, st))
+
+        pipe = {}
+        for pname, argsp in [("full", (len(lens), offb, tot, dd, len(shapes)))] + \
+                list(subsets.items()):
+            t_e = _time(stream, lambda: pipeline(*argsp), reps=5)
+            ref_sums = sums.to_numpy(np.float32)
+            g = C.c_void_p()
+            ck(L.skl_graph_begin_capture(st))
+            pipeline(*argsp)
+            ck(L.skl_graph_end_capture(st, C.byref(g)))
+            t_g = _time(stream, lambda: ck(L.skl_graph_launch(g, st)), reps=5)
+            same = bool(np.array_equal(sums.to_numpy(np.float32).view(np.uint32),
+                                       ref_sums.view(np.uint32)))
+            ck(L.skl_graph_destroy(g))
+            pipe[pname] = {"eager_ms": round(t_e, 4), "graph_ms": round(t_g, 4),
+                           "graph_sums_bit_identical": same}
+        ck(L.skl_reduce_segmented_run(add.handle, sums.ptr, segx.ptr, tot, offb.ptr, len(lens),
+                                      st))
+        hs = sums.to_numpy(np.float32)
         chk = [0, 1, 2, 100]
         hx = {s: oracle.gen_f32_sym(0x5002, int(off[s + 1]))[off[s]:off[s + 1]] for s in chk}
         ex["batched"] = {
@@ -430,6 +466,7 @@ def run_extras(skelcl, L, ck, stream, args):
                                                          for s in chk)),
             "gemms": len(shapes), "gemm_flop": flops, "gemm": bt,
         "float_add_precision_forms_ms": forms,
+        "pipeline_eager_vs_graph": pipe,
         }
         progress("batched done")
 

@@ -45,18 +45,19 @@ int skl_run_backward_zip(skl_skeleton s, void* d_left, void* d_right, const void
 
 namespace {
 // Workspace of the MFMA forms: grows, never shrinks; calls on different
-// streams at the same time race on it.
+// streams at the same time race on it.  An outgrown buffer is kept, not
+// freed: a captured graph may still use it.
 std::mutex g_ws_mutex;
 void* g_ws = nullptr;
 size_t g_ws_bytes = 0;
+std::vector<void*> g_ws_retired;
 
 int workspace(size_t bytes, void** out)
 {
   std::lock_guard<std::mutex> lk(g_ws_mutex);
   if (g_ws_bytes < bytes) {
     if (g_ws) {
-      SKL_HIP_TRY(hipDeviceSynchronize());
-      SKL_HIP_TRY(hipFree(g_ws));
+      g_ws_retired.push_back(g_ws);
       g_ws = nullptr;
       g_ws_bytes = 0;
     }
@@ -72,9 +73,8 @@ int workspace(size_t bytes, void** out)
 int seg_scratch(skl_skeleton_s* s, size_t bytes)
 {
   if (s->seg_scratch_bytes >= bytes) return SKL_OK;
-  if (s->seg_scratch) {
-    SKL_HIP_TRY(hipDeviceSynchronize());
-    SKL_HIP_TRY(hipFree(s->seg_scratch));
+  if (s->seg_scratch) {  // kept until the skeleton is released (graphs may hold it)
+    s->retired.push_back(s->seg_scratch);
     s->seg_scratch = nullptr;
     s->seg_scratch_bytes = 0;
   }

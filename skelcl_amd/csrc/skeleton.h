@@ -58,6 +58,9 @@ struct skl_skeleton_s {
   // scratch of the batched segmented forms (zeroed when allocated)
   void* seg_scratch = nullptr;
   size_t seg_scratch_bytes = 0;
+  // scratch buffers outgrown by later calls, freed with the skeleton: a
+  // captured graph may still use them
+  std::vector<void*> retired;
   // float (+) without extra arguments
   bool float_add() const;
   // its DEFAULT form: compensated pairs in a block, FP64 across blocks

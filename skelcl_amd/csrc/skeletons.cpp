@@ -552,9 +552,8 @@ static int get_kernel(skl_skeleton_s* s, const std::string& key, const char* nam
 static int ensure_scratch(skl_skeleton_s* s, size_t bytes)
 {
   if (s->scratch_bytes >= bytes) return SKL_OK;
-  if (s->scratch) {
-    SKL_HIP_TRY(hipDeviceSynchronize());
-    SKL_HIP_TRY(hipFree(s->scratch));
+  if (s->scratch) {  // kept until the skeleton is released (graphs may hold it)
+    s->retired.push_back(s->scratch);
     s->scratch = nullptr;
     s->scratch_bytes = 0;
   }
@@ -826,9 +825,10 @@ extern "C" int skl_skeleton_release(skl_skeleton s)
 {
   if (!s) return SKL_OK;
   for (auto& kv : s->progs) skl_program_release(kv.second.program);
-  if (s->scratch || s->seg_scratch) hipDeviceSynchronize();
+  if (s->scratch || s->seg_scratch || !s->retired.empty()) hipDeviceSynchronize();
   if (s->scratch) hipFree(s->scratch);
   if (s->seg_scratch) hipFree(s->seg_scratch);
+  for (void* p : s->retired) hipFree(p);
   delete s;
   return SKL_OK;
 }
