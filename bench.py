@@ -183,37 +183,63 @@ extern "C" __global__ __launch_bounds__(256) void probe_copy(const u4* __restric
   }
   for (; i < n; i += G) __builtin_nontemporal_store(__builtin_nontemporal_load(in + i), out + i);
 }
+extern "C" __global__ __launch_bounds__(256) void probe_zip(const u4* __restrict__ a,
+    const u4* __restrict__ b, u4* __restrict__ out, unsigned long long n) {
+  const unsigned long long G = (unsigned long long)gridDim.x * 256;
+  unsigned long long i = (unsigned long long)blockIdx.x * 256 + threadIdx.x;
+  for (; i + 3 * G < n; i += 4 * G) {
+    const u4 a0 = __builtin_nontemporal_load(a + i), a1 = __builtin_nontemporal_load(a + i + G);
+    const u4 a2 = __builtin_nontemporal_load(a + i + 2 * G);
+    const u4 a3 = __builtin_nontemporal_load(a + i + 3 * G);
+    const u4 b0 = __builtin_nontemporal_load(b + i), b1 = __builtin_nontemporal_load(b + i + G);
+    const u4 b2 = __builtin_nontemporal_load(b + i + 2 * G);
+    const u4 b3 = __builtin_nontemporal_load(b + i + 3 * G);
+    __builtin_nontemporal_store(a0 ^ b0, out + i);
+    __builtin_nontemporal_store(a1 ^ b1, out + i + G);
+    __builtin_nontemporal_store(a2 ^ b2, out + i + 2 * G);
+    __builtin_nontemporal_store(a3 ^ b3, out + i + 3 * G);
+  }
+  for (; i < n; i += G)
+    __builtin_nontemporal_store(__builtin_nontemporal_load(a + i) ^ __builtin_nontemporal_load(b + i),
+                                out + i);
+}
 """
 
 
 def run_ceiling(skelcl, L, ck, stream):
-    """Read-only, write-only and copy streams over 1 GiB at a sweep of grid
-    sizes (work-groups of 256 threads per CU): the practical HBM ceiling the
-    skeleton legs are compared with."""
+    """Read-only, write-only, copy and two-read-one-write (the saxpy pattern)
+    streams over 1 GiB per array at a sweep of grid sizes (work-groups of 256
+    threads per CU): the practical HBM ceiling the skeleton legs are
+    compared with."""
     import ctypes as C
     st = stream.handle
     prog, log = C.c_void_p(), C.c_void_p()
     ck(L.skl_program_build(CEILING_SRC.encode(), b"", C.byref(prog), C.byref(log)))
     skelcl._lib.take_string(log)
     kern = {}
-    for name in ("probe_read", "probe_write", "probe_copy"):
+    for name in ("probe_read", "probe_write", "probe_copy", "probe_zip"):
         k = C.c_void_p()
         ck(L.skl_kernel_get(prog, name.encode(), C.byref(k)))
         kern[name] = k
     nbytes = 1 << 30
     n4 = nbytes // 16
-    src, dst, sink = (skelcl.DeviceBuffer(nbytes), skelcl.DeviceBuffer(nbytes),
-                      skelcl.DeviceBuffer(256))
+    src, src2, dst, sink = (skelcl.DeviceBuffer(nbytes), skelcl.DeviceBuffer(nbytes),
+                            skelcl.DeviceBuffer(nbytes), skelcl.DeviceBuffer(256))
     ck(L.skl_generate_f32_sym(src.ptr, nbytes // 4, 0x6001, st))
+    ck(L.skl_generate_f32_sym(src2.ptr, nbytes // 4, 0x6002, st))
     cus = skelcl.device_info()["compute_units"]
     n_arg = C.c_ulonglong(n4)
     sp, dp, kp = C.c_void_p(src.ptr.value), C.c_void_p(dst.ptr.value), C.c_void_p(sink.ptr.value)
+    sp2 = C.c_void_p(src2.ptr.value)
     argv = {"probe_read": ([C.cast(C.pointer(sp), C.c_void_p), C.cast(C.pointer(n_arg), C.c_void_p),
                             C.cast(C.pointer(kp), C.c_void_p)], nbytes),
             "probe_write": ([C.cast(C.pointer(dp), C.c_void_p),
                              C.cast(C.pointer(n_arg), C.c_void_p)], nbytes),
             "probe_copy": ([C.cast(C.pointer(sp), C.c_void_p), C.cast(C.pointer(dp), C.c_void_p),
-                            C.cast(C.pointer(n_arg), C.c_void_p)], 2 * nbytes)}
+                            C.cast(C.pointer(n_arg), C.c_void_p)], 2 * nbytes),
+            "probe_zip": ([C.cast(C.pointer(sp), C.c_void_p), C.cast(C.pointer(sp2), C.c_void_p),
+                           C.cast(C.pointer(dp), C.c_void_p), C.cast(C.pointer(n_arg), C.c_void_p)],
+                          3 * nbytes)}
     out = {}
     for name, (a, moved) in argv.items():
         args = (C.c_void_p * len(a))(*[x.value for x in a])
@@ -586,7 +612,7 @@ def main() -> int:
                          "allpairs, batched): run only those and print their JSON (for "
                          "profiling); skips the headline legs and the CPU baseline")
     ap.add_argument("--ceiling", action="store_true",
-                    help="only time plain read / write / copy streaming probes over 1 GiB "
+                    help="only time plain read / write / copy / zip streaming probes over 1 GiB "
                          "at a sweep of grid sizes (the practical HBM ceiling) and print them")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also time an identity Map (copy) leg, for counter calibration")
