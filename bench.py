@@ -120,7 +120,7 @@ def measured_traffic(kernel: str):
     return None
 
 
-def _time(stream, fn, reps=3, warm=1):
+def _time(stream, fn, reps=20, warm=3):
     """Median of `reps` HIP-event timings (ms) of fn() on `stream`."""
     import skelcl_amd as skelcl
     for _ in range(warm):
@@ -247,8 +247,7 @@ def run_ceiling(skelcl, L, ck, stream):
         for per_cu in (1, 1.5, 2, 3, 4, 8, 16, 64):
             grid = (C.c_uint * 3)(int(cus * per_cu), 1, 1)
             block = (C.c_uint * 3)(256, 1, 1)
-            t = _time(stream, lambda: ck(L.skl_kernel_launch(kern[name], grid, block, 0, args, st)),
-                      reps=5, warm=2)
+            t = _time(stream, lambda: ck(L.skl_kernel_launch(kern[name], grid, block, 0, args, st)))
             rows.append({"work_groups_per_cu": per_cu, "ms": round(t, 4),
                          "GBps": round(moved / (t * 1e-3) / 1e9, 1)})
         best = max(rows, key=lambda r: r["GBps"])
@@ -327,15 +326,14 @@ def run_extras(skelcl, L, ck, stream, args):
         oi = skelcl.DeviceBuffer(4 * ni)
         mi = skelcl.Map("int(Index)", "int func(Index i){ return (int)(i * 2654435761u); }")
         mp = skelcl.Map("int(IndexPoint)", "int func(IndexPoint p){ return p.x - 3 * p.y; }")
-        t_i = _time(stream, lambda: ck(L.skl_map_index_run(mi.handle, oi.ptr, ni, 0, None, 0, st)),
-                    reps=5)
+        t_i = _time(stream, lambda: ck(L.skl_map_index_run(mi.handle, oi.ptr, ni, 0, None, 0, st)))
         hi = oi.to_numpy(np.int32)
         idx = np.arange(0, ni, 4099, dtype=np.uint64)
         ok_i = bool(np.array_equal(hi[::4099], ((idx * 2654435761) & 0xFFFFFFFF)
                                    .astype(np.uint32).view(np.int32)))
         side = 1 << 14
         t_p = _time(stream, lambda: ck(L.skl_map_indexpoint_run(mp.handle, oi.ptr, side, side, 0,
-                                                                None, 0, st)), reps=5)
+                                                                None, 0, st)))
         hp = oi.to_numpy(np.int32).reshape(side, side)
         rr = np.arange(0, side, 97)
         ok_p = bool(np.array_equal(hp[rr], (np.arange(side)[None, :] - 3 * rr[:, None])
@@ -389,12 +387,12 @@ def run_extras(skelcl, L, ck, stream, args):
         if args.autotune:
             ck(L.skl_autotune_args(gauss.handle, S * S, warg, nw))
         t_gen = _time(stream, lambda: ck(L.skl_mapoverlap_run(gauss_gen.handle, outb.ptr, img.ptr,
-                                                              S, S, warg, nw, st)), reps=5)
+                                                              S, S, warg, nw, st)))
         o_gen = outb.to_numpy(np.float32)
         t_mo = _time(stream, lambda: ck(L.skl_mapoverlap_run(gauss.handle, outb.ptr, img.ptr, S, S,
-                                                             warg, nw, st)), reps=5)
+                                                             warg, nw, st)))
         t_adj = _time(stream, lambda: ck(L.skl_mapoverlap_stencil_backward(
-            gauss.handle, dinb.ptr, Wb.ptr, wv.ptr, S, S, st)), reps=5)
+            gauss.handle, dinb.ptr, Wb.ptr, wv.ptr, S, S, st)))
         himg = oracle.gen_f32_unit(0x3001, S * S, 255.0).reshape(S, S)
         rng = np.random.default_rng(0x3003)
         pix = rng.integers(0, S * S, 64).astype(np.int64)
@@ -435,8 +433,7 @@ def run_extras(skelcl, L, ck, stream, args):
         ap = {}
         for name, prec, peak in (("fp32", skelcl.PREC_FP32, 157.3), ("bf16x3", skelcl.PREC_BF16X3, 2500.0),
                                  ("fp8", skelcl.PREC_FP8, 5000.0)):
-            t = _time(stream, lambda: ck(L.skl_matmul_f32(Cb.ptr, A.ptr, B.ptr, M, M, M, 0, 0, prec, st)),
-                      reps=3)
+            t = _time(stream, lambda: ck(L.skl_matmul_f32(Cb.ptr, A.ptr, B.ptr, M, M, M, 0, 0, prec, st)))
             c = Cb.to_numpy(np.float32).reshape(M, M)[rows].astype(np.float64)
             err = float(np.linalg.norm(c - ref) / np.linalg.norm(ref))
             tf = flop / (t * 1e-3) / 1e12
@@ -451,8 +448,7 @@ def run_extras(skelcl, L, ck, stream, args):
         apk = skelcl.AllPairs("float(float, float)", reduce=add, zip=zip_,
                               precision=skelcl.PREC_BF16X3)
         t_bw = _time(stream, lambda: ck(L.skl_allpairs_backward(apk.handle, A.ptr, B.ptr, Cb.ptr,
-                                                                Cb.ptr, dC.ptr, M, M, M, st)),
-                     reps=2)
+                                                                Cb.ptr, dC.ptr, M, M, M, st)))
         ap["backward_bf16x3_ms"] = round(t_bw, 3)
         del A, B, Cb, dC, hA, hB
         n, d = 16384, 256
@@ -467,7 +463,7 @@ def run_extras(skelcl, L, ck, stream, args):
                              " float t = getElementFromRow(row, k) - getElementFromColumn(col, k);"
                              " s += t*t; } return s; }")
         t_sq = _time(stream, lambda: ck(L.skl_allpairs_run(sq.handle, D.ptr, P.ptr, Pt.ptr, n, n, d,
-                                                           None, 0, st)), reps=3)
+                                                           None, 0, st)))
         rows = np.array([0, 1, 4097, 16383], np.int32)
         dref = oracle.allpairs_sqdist_f64(hP, np.ascontiguousarray(hP.T), rows=rows)
         dg = D.to_numpy(np.float32).reshape(n, n)[rows]
@@ -501,15 +497,15 @@ def run_extras(skelcl, L, ck, stream, args):
             scan_p = skelcl.Scan("float(float)", ADD, "0.0f", precision=prec)
             f = {}
             f["segmented_reduce_ms"] = _time(stream, lambda: ck(L.skl_reduce_segmented_run(
-                red_p.handle, sums.ptr, segx.ptr, tot, offb.ptr, len(lens), st)), reps=3)
+                red_p.handle, sums.ptr, segx.ptr, tot, offb.ptr, len(lens), st)))
             f["segmented_scan_ms"] = _time(stream, lambda: ck(L.skl_scan_segmented_run(
-                scan_p.handle, segy.ptr, segx.ptr, tot, offb.ptr, len(lens), 1, st)), reps=3)
+                scan_p.handle, segy.ptr, segx.ptr, tot, offb.ptr, len(lens), 1, st)))
             f["scan_ms"] = _time(stream, lambda: ck(L.skl_scan_run(
-                scan_p.handle, segy.ptr, segx.ptr, tot, 1, None, 0, st)), reps=3)
+                scan_p.handle, segy.ptr, segx.ptr, tot, 1, None, 0, st)))
             f["scan_adjoint_ms"] = _time(stream, lambda: ck(L.skl_scan_backward(
-                scan_p.handle, segy.ptr, segx.ptr, tot, 1, st)), reps=3)
+                scan_p.handle, segy.ptr, segx.ptr, tot, 1, st)))
             f["reduce_ms"] = _time(stream, lambda: ck(L.skl_reduce_run(
-                red_p.handle, sums.ptr, segx.ptr, tot, None, 0, st)), reps=3)
+                red_p.handle, sums.ptr, segx.ptr, tot, None, 0, st)))
             forms[pname] = {k: round(v, 4) for k, v in f.items()}
         # the DEFAULT form's segment sums are the ones checked below
         ck(L.skl_reduce_segmented_run(add.handle, sums.ptr, segx.ptr, tot, offb.ptr, len(lens),
@@ -534,8 +530,7 @@ def run_extras(skelcl, L, ck, stream, args):
         ck(L.skl_memcpy_h2d(dd.ptr, 0, C.cast(descs, C.c_void_p), C.sizeof(descs), None, 1))
         bt = {}
         for name, prec in (("fp32", skelcl.PREC_FP32), ("bf16x3", skelcl.PREC_BF16X3)):
-            t = _time(stream, lambda: ck(L.skl_matmul_batched_f32(dd.ptr, len(shapes), prec, st)),
-                      reps=3)
+            t = _time(stream, lambda: ck(L.skl_matmul_batched_f32(dd.ptr, len(shapes), prec, st)))
             bt[name] = {"ms": round(t, 3), "TFLOPs": round(flops / (t * 1e-3) / 1e12, 1)}
         hs = sums.to_numpy(np.float32)
         # the whole batched pipeline (segmented Reduce: 4 launches, segmented
@@ -559,13 +554,13 @@ def run_extras(skelcl, L, ck, stream, args):
         pipe = {}
         for pname, argsp in [("full", (len(lens), offb, tot, dd, len(shapes)))] + \
                 list(subsets.items()):
-            t_e = _time(stream, lambda: pipeline(*argsp), reps=5)
+            t_e = _time(stream, lambda: pipeline(*argsp))
             ref_sums = sums.to_numpy(np.float32)
             g = C.c_void_p()
             ck(L.skl_graph_begin_capture(st))
             pipeline(*argsp)
             ck(L.skl_graph_end_capture(st, C.byref(g)))
-            t_g = _time(stream, lambda: ck(L.skl_graph_launch(g, st)), reps=5)
+            t_g = _time(stream, lambda: ck(L.skl_graph_launch(g, st)))
             same = bool(np.array_equal(sums.to_numpy(np.float32).view(np.uint32),
                                        ref_sums.view(np.uint32)))
             ck(L.skl_graph_destroy(g))
