@@ -180,6 +180,182 @@ __global__ __launch_bounds__(256) void stencil_forward_kernel(const float* __res
   }
 }
 
+// The same schedule with two adjacent columns per thread: a 128 x 4RB tile,
+// outputs (r, 2l) and (r, 2l + 1) for lane l.  Each tile row is read as
+// R + 1 aligned 8-byte pairs (2R + 2 values) that feed both columns'
+// windows, so every LDS byte feeds twice the fused multiply-adds of the
+// one-column kernel and every weight row loaded feeds two columns.  Every
+// output still takes its terms in the loop's order.
+template <int R, int RB, bool NEUTRAL, bool FUSED>
+__global__ __launch_bounds__(256) void stencil_forward2_kernel(const float* __restrict__ in,
+                                                               float* __restrict__ out,
+                                                               const float* __restrict__ w,
+                                                               int rows, int cols, float neutral)
+{
+  constexpr int W = 2 * R + 1;
+  constexpr int TW = 128, TH = 4 * RB, TWP = TW + 2 * R, THP = TH + 2 * R;
+  constexpr int NR = (THP + 3) / 4;  // tile rows per wave
+  static_assert(TWP % 2 == 0, "tile rows must stay 8-byte aligned");
+  __shared__ __align__(16) float tile[THP * TWP];
+  const int lane = threadIdx.x & 63;
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int c0 = blockIdx.x * TW, r0 = blockIdx.y * TH;
+
+  // ---- stage the tile: lane l loads columns l, l + 64 and, for l < 2R, the
+  // halo column l + 128 of each of its wave's tile rows.
+  const unsigned l4 = (unsigned)lane * 4u;
+  const bool halo = lane < 2 * R;
+  float a[NR], b[NR], h[NR];
+  const bool interior = c0 >= R && c0 + TW + R <= cols && r0 >= R && r0 + TH + R <= rows;
+  if (interior) {
+    const float* base = in + (size_t)(r0 - R) * (size_t)cols + (c0 - R);
+#pragma unroll
+    for (int k = 0; k < NR; ++k) {
+      const int tr = wv + 4 * k;
+      if (tr < THP) {
+        const float* row = base + (size_t)tr * (size_t)cols;
+        a[k] = load_row(row, l4);
+        b[k] = load_row(row, l4 + 256u);
+        if (halo) h[k] = load_row(row, l4 + 512u);
+      }
+    }
+  } else {
+    const int gc0 = c0 - R + lane, gc1 = gc0 + 64, gc2 = gc0 + 128;
+    const int cc0 = gc0 < 0 ? 0 : (gc0 >= cols ? cols - 1 : gc0);
+    const int cc1 = gc1 < 0 ? 0 : (gc1 >= cols ? cols - 1 : gc1);
+    const int cc2 = gc2 < 0 ? 0 : (gc2 >= cols ? cols - 1 : gc2);
+    const unsigned o0 = (unsigned)cc0 * 4u, o1 = (unsigned)cc1 * 4u, o2 = (unsigned)cc2 * 4u;
+#pragma unroll
+    for (int k = 0; k < NR; ++k) {
+      const int tr = wv + 4 * k;
+      if (tr < THP) {
+        const int gr = r0 - R + tr;
+        const int cr = gr < 0 ? 0 : (gr >= rows ? rows - 1 : gr);
+        const float* row = in + (size_t)cr * (size_t)cols;
+        a[k] = load_row(row, o0);
+        b[k] = load_row(row, o1);
+        if (halo) h[k] = load_row(row, o2);
+        if (NEUTRAL) {
+          if (gr != cr || gc0 != cc0) a[k] = neutral;
+          if (gr != cr || gc1 != cc1) b[k] = neutral;
+          if (gr != cr || gc2 != cc2) h[k] = neutral;
+        }
+      }
+    }
+  }
+  float* trow = tile + wv * TWP + lane;
+#pragma unroll
+  for (int k = 0; k < NR; ++k) {
+    if (wv + 4 * k < THP) {
+      trow[4 * k * TWP] = a[k];
+      trow[4 * k * TWP + 64] = b[k];
+      if (halo) trow[4 * k * TWP + 128] = h[k];
+    }
+  }
+  __syncthreads();
+
+  // ---- RB rows x 2 columns per thread, tile rows streamed (the loads of
+  // step t wait, through opaque offsets, for every accumulator of step
+  // t - LAG, as in the one-column kernel)
+  float acc[RB][2];
+#pragma unroll
+  for (int i = 0; i < RB; ++i) acc[i][0] = acc[i][1] = 0.0f;
+  float wr[W][W];
+  const char* win = (const char*)(tile + (wv * RB) * TWP + 2 * lane);
+  constexpr int LAG = SKL_STENCIL_LAG;
+  unsigned zt[LAG + 1], zw[LAG + 1];
+#pragma unroll
+  for (int q = 0; q < LAG; ++q) {
+    zt[q] = q * TWP * 4;
+    zw[q] = q * W * 4;
+    asm volatile("" : "+s"(zt[q]), "+s"(zw[q]));
+  }
+#pragma unroll
+  for (int t = 0; t < RB + 2 * R; ++t) {
+    const int q = t % (LAG + 1);
+    if (t < W) {
+      const __attribute__((address_space(4))) float* wrow =
+          (const __attribute__((address_space(4))) float*)((unsigned long long)w + zw[q]);
+#pragma unroll
+      for (int j = 0; j < W; ++j) wr[t][j] = wrow[j];
+    }
+    float v[W + 1];
+    const float2* wt = (const float2*)(win + zt[q]);
+#pragma unroll
+    for (int j = 0; j < (W + 1) / 2; ++j) {
+      const float2 p = wt[j];
+      v[2 * j] = p.x;
+      v[2 * j + 1] = p.y;
+    }
+#pragma unroll
+    for (int i = 0; i < RB; ++i) {
+      const int k = t - i;  // weight row (dy + R) of row i at this step
+      if (k < 0 || k >= W) continue;
+#pragma unroll
+      for (int j = 0; j < W; ++j) {
+        acc[i][0] = FUSED ? __builtin_fmaf(v[j], wr[k][j], acc[i][0]) : acc[i][0] + v[j] * wr[k][j];
+        acc[i][1] = FUSED ? __builtin_fmaf(v[j + 1], wr[k][j], acc[i][1])
+                          : acc[i][1] + v[j + 1] * wr[k][j];
+      }
+    }
+    const int n = (t + LAG) % (LAG + 1);
+    zt[n] = (t + LAG) * TWP * 4;
+    zw[n] = (t + LAG) * W * 4;
+#pragma unroll
+    for (int i = 0; i < RB; ++i)
+      asm volatile("" : "+s"(zt[n]), "+s"(zw[n]) : "v"(acc[i][0]), "v"(acc[i][1]));
+  }
+  const int cl = c0 + 2 * lane;
+  const unsigned o8 = (unsigned)lane * 8u;
+  if ((cols & 1) == 0 && ((unsigned long long)out & 7u) == 0 && cl + 1 < cols) {
+    // even column count and an 8-byte aligned output: every row starts
+    // 8-byte aligned
+#pragma unroll
+    for (int i = 0; i < RB; ++i) {
+      const int r = r0 + wv * RB + i;
+      if (r < rows) {
+        unsigned long long rb = (unsigned long long)(out + (size_t)r * (size_t)cols + c0);
+        asm volatile("" : "+s"(rb));
+        typedef float f2 __attribute__((ext_vector_type(2)));
+        *(__attribute__((address_space(1))) f2*)(rb + o8) = f2{acc[i][0], acc[i][1]};
+      }
+    }
+  } else if (cl < cols) {
+#pragma unroll
+    for (int i = 0; i < RB; ++i) {
+      const int r = r0 + wv * RB + i;
+      if (r < rows) {
+        float* row = out + (size_t)r * (size_t)cols + c0;
+        store_row(row, o8, acc[i][0]);
+        if (cl + 1 < cols) store_row(row, o8 + 4u, acc[i][1]);
+      }
+    }
+  }
+}
+
+template <int R, int RB>
+hipError_t launch_rb2(float* out, const float* in, const float* w, int rows, int cols,
+                      bool neutral_pad, float neutral, bool fused, hipStream_t s)
+{
+  dim3 grid((cols + 127) / 128, (rows + 4 * RB - 1) / (4 * RB)), block(256);
+  if (neutral_pad) {
+    if (fused)
+      hipLaunchKernelGGL((stencil_forward2_kernel<R, RB, true, true>), grid, block, 0, s, in, out,
+                         w, rows, cols, neutral);
+    else
+      hipLaunchKernelGGL((stencil_forward2_kernel<R, RB, true, false>), grid, block, 0, s, in,
+                         out, w, rows, cols, neutral);
+  } else {
+    if (fused)
+      hipLaunchKernelGGL((stencil_forward2_kernel<R, RB, false, true>), grid, block, 0, s, in,
+                         out, w, rows, cols, neutral);
+    else
+      hipLaunchKernelGGL((stencil_forward2_kernel<R, RB, false, false>), grid, block, 0, s, in,
+                         out, w, rows, cols, neutral);
+  }
+  return hipGetLastError();
+}
+
 template <int R, int RB>
 hipError_t launch_rb(float* out, const float* in, const float* w, int rows, int cols,
                      bool neutral_pad, float neutral, bool fused, hipStream_t s)
@@ -205,8 +381,17 @@ hipError_t launch_rb(float* out, const float* in, const float* w, int rows, int 
 
 template <int R>
 hipError_t launch_r(float* out, const float* in, const float* w, int rows, int cols,
-                    bool neutral_pad, float neutral, bool fused, int rb, hipStream_t s)
+                    bool neutral_pad, float neutral, bool fused, int rb, int cb, hipStream_t s)
 {
+  if (cb == 2) {
+    switch (rb) {
+      case 4: return launch_rb2<R, 4>(out, in, w, rows, cols, neutral_pad, neutral, fused, s);
+      case 6: return launch_rb2<R, 6>(out, in, w, rows, cols, neutral_pad, neutral, fused, s);
+      case 8: return launch_rb2<R, 8>(out, in, w, rows, cols, neutral_pad, neutral, fused, s);
+    }
+    return hipErrorInvalidValue;
+  }
+  if (cb != 1) return hipErrorInvalidValue;
   switch (rb) {
     case 4: return launch_rb<R, 4>(out, in, w, rows, cols, neutral_pad, neutral, fused, s);
     case 6: return launch_rb<R, 6>(out, in, w, rows, cols, neutral_pad, neutral, fused, s);
@@ -220,21 +405,23 @@ hipError_t launch_r(float* out, const float* in, const float* w, int rows, int c
 // Largest radius the stencil kernel is instantiated for.
 extern "C" int skl_internal_stencil_max_radius() { return 5; }
 
-// rb: rows per thread (4, 6 or 8).  Returns SKL_OK, or SKL_ERR_UNSUPPORTED
-// for a radius or rb without an instantiation.
+// rb: rows per thread (4, 6 or 8); cb: columns per thread (1 or 2).
+// Returns SKL_OK, or SKL_ERR_UNSUPPORTED for a radius, rb or cb without an
+// instantiation.
 extern "C" int skl_internal_stencil_forward(float* out, const float* in, const float* w,
                                             int rows, int cols, int R, int neutral_pad,
-                                            float neutral, int fused, int rb, hipStream_t s)
+                                            float neutral, int fused, int rb, int cb,
+                                            hipStream_t s)
 {
   if (rows <= 0 || cols <= 0) return rows < 0 || cols < 0 ? SKL_ERR_INVALID : SKL_OK;
   hipError_t e = hipErrorInvalidValue;
   switch (R) {
-    case 0: e = launch_r<0>(out, in, w, rows, cols, neutral_pad, neutral, fused, rb, s); break;
-    case 1: e = launch_r<1>(out, in, w, rows, cols, neutral_pad, neutral, fused, rb, s); break;
-    case 2: e = launch_r<2>(out, in, w, rows, cols, neutral_pad, neutral, fused, rb, s); break;
-    case 3: e = launch_r<3>(out, in, w, rows, cols, neutral_pad, neutral, fused, rb, s); break;
-    case 4: e = launch_r<4>(out, in, w, rows, cols, neutral_pad, neutral, fused, rb, s); break;
-    case 5: e = launch_r<5>(out, in, w, rows, cols, neutral_pad, neutral, fused, rb, s); break;
+    case 0: e = launch_r<0>(out, in, w, rows, cols, neutral_pad, neutral, fused, rb, cb, s); break;
+    case 1: e = launch_r<1>(out, in, w, rows, cols, neutral_pad, neutral, fused, rb, cb, s); break;
+    case 2: e = launch_r<2>(out, in, w, rows, cols, neutral_pad, neutral, fused, rb, cb, s); break;
+    case 3: e = launch_r<3>(out, in, w, rows, cols, neutral_pad, neutral, fused, rb, cb, s); break;
+    case 4: e = launch_r<4>(out, in, w, rows, cols, neutral_pad, neutral, fused, rb, cb, s); break;
+    case 5: e = launch_r<5>(out, in, w, rows, cols, neutral_pad, neutral, fused, rb, cb, s); break;
     default: return SKL_ERR_UNSUPPORTED;
   }
   if (e == hipErrorInvalidValue) return SKL_ERR_UNSUPPORTED;

@@ -24,7 +24,8 @@ extern "C" const char* skl_tmpl_mapoverlap;
 extern "C" int skl_internal_stencil_max_radius();
 extern "C" int skl_internal_stencil_forward(float* out, const float* in, const float* w,
                                             int rows, int cols, int R, int neutral_pad,
-                                            float neutral, int fused, int rb, hipStream_t s);
+                                            float neutral, int fused, int rb, int cb,
+                                            hipStream_t s);
 extern "C" const char* skl_tmpl_allpairs;
 
 using namespace skl;
@@ -1299,7 +1300,7 @@ extern "C" int skl_mapoverlap_run(skl_skeleton s, void* out, const void* in, siz
     const int st = skl_internal_stencil_forward(
         (float*)out, (const float*)in, (const float*)args[0].value, r, c, s->stencil_radius,
         s->padding == SKL_PADDING_NEUTRAL ? 1 : 0, s->stencil_neutral,
-        s->contract == "off" ? 0 : 1, rb, resolve_stream(stream));
+        s->contract == "off" ? 0 : 1, rb, 2, resolve_stream(stream));
     if (st != SKL_OK) return fail(st, "MapOverlap: stencil kernel launch failed");
     return SKL_OK;
   }
