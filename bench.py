@@ -566,10 +566,47 @@ def run_extras(skelcl, L, ck, stream, args):
             ck(L.skl_graph_destroy(g))
             pipe[pname] = {"eager_ms": round(t_e, 4), "graph_ms": round(t_g, 4),
                            "graph_sums_bit_identical": same}
+        # The same work as ONE launch (skl_batch_run): the segment sums and the
+        # inclusive segmented scan from one pass over the array, and the
+        # BF16X3 products walked by 1 or 2 work-groups per CU meanwhile
+        red_b = skelcl.Reduce("float(float)", ADD, "0")
+        scan_b = skelcl.Scan("float(float)", ADD, "0.0f")
+        sums_b = skelcl.DeviceBuffer(4 * len(lens))
+
+        def batch_one(w, with_products=True):
+            ck(L.skl_batch_run(red_b.handle, sums_b.ptr, scan_b.handle, segy.ptr, 1, segx.ptr,
+                               tot, offb.ptr, len(lens), dd.ptr if with_products else None,
+                               len(shapes) if with_products else 0, skelcl.PREC_BF16X3, w, st))
+
+        one = {"segments_only_ms": round(_time(stream, lambda: batch_one(0, False)), 4)}
+        for w in (1, 2):
+            one[f"workers_per_cu_{w}_ms"] = round(_time(stream, lambda: batch_one(w)), 4)
+        # the subsets of the eager pipeline above, as one launch each
+        for pname, (nsm, ob, ne, dsm, ng) in subsets.items():
+            one[f"{pname}_ms"] = round(_time(stream, lambda: ck(L.skl_batch_run(
+                red_b.handle, sums_b.ptr, scan_b.handle, segy.ptr, 1, segx.ptr, ne, ob.ptr, nsm,
+                dsm.ptr, ng, skelcl.PREC_BF16X3, 2, st))), 4)
+        wbest = 1 if one["workers_per_cu_1_ms"] <= one["workers_per_cu_2_ms"] else 2
+        batch_one(wbest)
+        eager_sums = sums_b.to_numpy(np.float32)
+        g = C.c_void_p()
+        ck(L.skl_graph_begin_capture(st))
+        batch_one(wbest)
+        ck(L.skl_graph_end_capture(st, C.byref(g)))
+        one["graph_ms"] = round(_time(stream, lambda: ck(L.skl_graph_launch(g, st))), 4)
+        one["graph_sums_bit_identical"] = bool(np.array_equal(
+            sums_b.to_numpy(np.float32).view(np.uint32), eager_sums.view(np.uint32)))
+        ck(L.skl_graph_destroy(g))
+        bad = C.c_int(-1)
+        ck(L.skl_batch_check(scan_b.handle, C.byref(bad)))
+        one["offsets_flagged"] = bool(bad.value)
         ck(L.skl_reduce_segmented_run(add.handle, sums.ptr, segx.ptr, tot, offb.ptr, len(lens),
                                       st))
         hs = sums.to_numpy(np.float32)
         chk = [0, 1, 2, 100]
+        one["segment_sum_max_abs_err_vs_fp64"] = float(max(
+            abs(float(eager_sums[s]) - oracle.sum_f64(
+                oracle.gen_f32_sym(0x5002, int(off[s + 1]))[off[s]:off[s + 1]])) for s in chk))
         hx = {s: oracle.gen_f32_sym(0x5002, int(off[s + 1]))[off[s]:off[s + 1]] for s in chk}
         ex["batched"] = {
             "segments": len(lens), "elements": tot,
@@ -581,8 +618,9 @@ def run_extras(skelcl, L, ck, stream, args):
             "segment_sum_max_abs_err_vs_fp64": float(max(abs(float(hs[s]) - oracle.sum_f64(hx[s]))
                                                          for s in chk)),
             "gemms": len(shapes), "gemm_flop": flops, "gemm": bt,
-        "float_add_precision_forms_ms": forms,
-        "pipeline_eager_vs_graph": pipe,
+            "float_add_precision_forms_ms": forms,
+            "pipeline_eager_vs_graph": pipe,
+            "one_launch": one,
         }
         progress("batched done")
 

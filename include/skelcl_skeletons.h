@@ -304,6 +304,33 @@ typedef struct {
 int skl_matmul_batched_f32(const skl_gemm_desc* descs_device, int count,
                            int precision, skl_stream stream);
 
+/* One launch for a batch of uneven work: the segmented Reduce(+) sums
+ * (`reduce`, into sums[nseg]) and the segmented Scan(+) (`scan`, into
+ * scan_out[n], exclusive or inclusive) of the same float array over the
+ * same nseg segments (device offsets as above), and `count` small float
+ * products (device descriptors, as for skl_matmul_batched_f32; at most
+ * 2048; precision FP32 or BF16X3).  Any part may be left out (NULL output,
+ * nseg or count 0).  One kernel launch, no prologue and no host
+ * synchronisation: the segment part is one pass over tiles of 16384
+ * elements (a decoupled look-back; the sums are the inclusive values at
+ * the segments' ends, so the array is read once for both), and
+ * workers_per_cu (0 = 1, or 2) work-groups per CU walk the product tiles at
+ * the same time.  The Reduce and the Scan must share a precision form
+ * (DEFAULT: compensated FP32 in a tile, FP64 across tiles; FP64).  The
+ * offsets are checked by the tiles themselves: after invalid offsets the
+ * outputs of the call are unspecified (in-bounds writes only) and
+ * skl_batch_check reports it.  No counterpart in the reference: for
+ * workloads of many uneven segments and small products. */
+int skl_batch_run(skl_skeleton reduce, float* sums, skl_skeleton scan,
+                  float* scan_out, int inclusive, const float* in, size_t n,
+                  const int64_t* offsets, int nseg, const skl_gemm_desc* descs,
+                  int count, int precision, int workers_per_cu,
+                  skl_stream stream);
+/* Synchronises the device, then *invalid_offsets = 1 when the last
+ * skl_batch_run on this skeleton (the scan if one was given, else the
+ * reduce) found invalid offsets, else 0. */
+int skl_batch_check(skl_skeleton s, int* invalid_offsets);
+
 /* Autotuner: time candidate launch shapes for this skeleton at size n
  * (elements; for MapOverlap an n-pixel square image, for AllPairs a product
  * with n output elements and square operands, sqrt(n) on each side) on

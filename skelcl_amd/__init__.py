@@ -695,7 +695,8 @@ class Reduce(_Skeleton):
 
     def segmented(self, inp, offsets, stream=None):
         """Batched segmented Reduce(+) (extension): out[s] = sum of
-        inp[offsets[s]:offsets[s+1]], all segments in one persistent launch."""
+        inp[offsets[s]:offsets[s+1]], all segments in one main launch after
+        small prologue launches (batch_run does it in one launch)."""
         off = _segment_offsets(offsets, len(inp))
         nseg = len(off) - 1
         o = Vector(size=nseg, etype=self.types[0])
@@ -740,7 +741,8 @@ class Scan(_Skeleton):
 
     def segmented(self, inp, offsets, inclusive=False, stream=None):
         """Batched segmented Scan(+) (extension): every segment scanned on its
-        own, all segments in one persistent launch."""
+        own, all segments in one main launch after a small prologue launch
+        (batch_run does it in one launch)."""
         off = _segment_offsets(offsets, len(inp))
         nseg = len(off) - 1
         o = Vector(size=len(inp), etype=self.types[0])
@@ -749,6 +751,35 @@ class Scan(_Skeleton):
                                            int(bool(inclusive)), _stream(stream)))
         o.data_on_device_modified()
         return o
+
+
+def batch_run(inp, offsets, reduce: "Reduce" = None, scan: "Scan" = None, inclusive=False,
+              descs=None, count: int = 0, precision: int = PREC_FP32, workers_per_cu: int = 0,
+              stream=None):
+    """One launch for a batch (extension, skl_batch_run): the segment sums
+    (with `reduce`) and the segmented scan (with `scan`) of the Vector `inp`
+    over `offsets`, and `count` small products described by the device
+    buffer `descs` (skl_gemm_desc records).  Returns (sums, scan) Vectors,
+    None for a part left out.  Host offsets are checked here; device
+    offsets are checked by the launch (skl_batch_check)."""
+    n = len(inp) if inp is not None else 0
+    off = _segment_offsets(offsets, n) if offsets is not None else None
+    nseg = len(off) - 1 if off is not None else 0
+    sums = Vector(size=nseg, etype="float") if reduce is not None and nseg > 0 else None
+    sc = Vector(size=n, etype="float") if scan is not None and nseg > 0 else None
+    check(lib().skl_batch_run(
+        reduce.handle if sums is not None else None,
+        sums.device_buffer_for_output() if sums is not None else None,
+        scan.handle if sc is not None else None,
+        sc.device_buffer_for_output() if sc is not None else None,
+        int(bool(inclusive)), inp.device_ptr() if n else None, n,
+        off.device_ptr() if off is not None else None, nseg,
+        descs.ptr if descs is not None else None, int(count), int(precision),
+        int(workers_per_cu), _stream(stream)))
+    for v in (sums, sc):
+        if v is not None:
+            v.data_on_device_modified()
+    return sums, sc
 
 
 class MapOverlap(_Skeleton):

@@ -144,6 +144,8 @@ SIGNATURES = {
     "skl_allpairs_backward": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _sz, _sz, _sz, _vp]),
     "skl_matmul_f32": (_i, [_vp, _vp, _vp, _sz, _sz, _sz, _i, _i, _i, _vp]),
     "skl_matmul_batched_f32": (_i, [_vp, _i, _i, _vp]),
+    "skl_batch_run": (_i, [_vp, _vp, _vp, _vp, _i, _vp, _sz, _vp, _i, _vp, _i, _i, _i, _vp]),
+    "skl_batch_check": (_i, [_vp, C.POINTER(_i)]),
     "skl_autotune": (_i, [_vp, _sz, _i]),
     "skl_autotune_args": (_i, [_vp, _sz, _argp, _i]),
     "skl_skeleton_launch_shape": (_i, [_vp, C.POINTER(_i)]),

@@ -36,6 +36,7 @@ TEMPLATES = ["prelude", "map", "reduce", "scan", "mapoverlap", "allpairs"]
 # register other than its accumulator input and copies the accumulators
 # round the k-loop (hundreds of v_accvgpr moves per slice).
 EXTRA = {"matmul.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form=1"],
+         "batch.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form=1"],
          # the weighted-stencil forward fuses only where it calls fmaf
          "stencil.hip": ["-ffp-contract=off"]}
 

@@ -31,6 +31,14 @@ extern "C" size_t skl_internal_segmented_workspace(unsigned long long n, int nse
 extern "C" int skl_internal_broadcast(void* dst, const void* src, size_t elem, size_t count,
                                       hipStream_t s);
 extern "C" size_t skl_internal_batched_workspace(int count);
+extern "C" size_t skl_internal_batch_workspace(unsigned long long n, int nseg);
+extern "C" int skl_internal_batch_max_products();
+extern "C" int skl_internal_batch_status(const void* ws, int* bad);
+extern "C" int skl_internal_batch_run(const float* in, float* scan_out, float* sums,
+                                      unsigned long long n, const int64_t* offsets, int nseg,
+                                      int inclusive, int fp64, const skl_gemm_desc* descs,
+                                      int count, int precision, int workers, void* ws,
+                                      hipStream_t s);
 extern "C" int skl_internal_batched_gemm_ws(const skl_gemm_desc* descs, int count, int precision,
                                             void* ws, hipStream_t s);
 
@@ -83,6 +91,30 @@ int seg_scratch(skl_skeleton_s* s, size_t bytes)
   SKL_HIP_TRY(hipDeviceSynchronize());
   s->seg_scratch_bytes = bytes;
   return SKL_OK;
+}
+
+// The same for one-launch batches (their own scratch: the ticket and
+// error words of the batch kernel must not share words with the
+// prologue-based segmented forms).
+int batch_scratch(skl_skeleton_s* s, size_t bytes)
+{
+  if (s->batch_scratch_bytes >= bytes) return SKL_OK;
+  if (s->batch_scratch) {
+    s->retired.push_back(s->batch_scratch);
+    s->batch_scratch = nullptr;
+    s->batch_scratch_bytes = 0;
+  }
+  SKL_HIP_TRY(hipMalloc(&s->batch_scratch, bytes));
+  SKL_HIP_TRY(hipMemset(s->batch_scratch, 0, bytes));
+  SKL_HIP_TRY(hipDeviceSynchronize());
+  s->batch_scratch_bytes = bytes;
+  return SKL_OK;
+}
+
+bool is_float_add_skeleton(const skl_skeleton_s* s, int kind)
+{
+  return s && s->kind == kind && s->op == UserOp::Add && s->extras.empty() &&
+         s->types[0].name == "float";
 }
 
 bool is_float_addmul(const skl_skeleton_s* s)
@@ -335,3 +367,66 @@ extern "C" int skl_matmul_batched_f32(const skl_gemm_desc* descs_device, int cou
   return skl_internal_batched_gemm_ws(descs_device, count, precision, ws, resolve_stream(stream));
 }
 
+extern "C" int skl_batch_run(skl_skeleton reduce, float* sums, skl_skeleton scan, float* scan_out,
+                             int inclusive, const float* in, size_t n, const int64_t* offsets,
+                             int nseg, const skl_gemm_desc* descs, int count, int precision,
+                             int workers_per_cu, skl_stream stream)
+{
+  SKL_TRY(ensure_initialized());
+  if (nseg < 0 || count < 0) return fail(SKL_ERR_INVALID, "negative segment or product count");
+  if (!sums) reduce = nullptr;
+  if (!scan_out) scan = nullptr;
+  if (sums && !is_float_add_skeleton(reduce, SKL_REDUCE))
+    return fail(SKL_ERR_UNSUPPORTED, "batch: segment sums need a float Reduce(+) skeleton");
+  if (scan_out && !is_float_add_skeleton(scan, SKL_SCAN))
+    return fail(SKL_ERR_UNSUPPORTED, "batch: the segmented scan needs a float Scan(+) skeleton");
+  const bool segs = nseg > 0 && (reduce || scan);
+  const int fp64r = reduce && reduce->precision == SKL_PREC_FP64;
+  const int fp64s = scan && scan->precision == SKL_PREC_FP64;
+  if (reduce && scan && fp64r != fp64s)
+    return fail(SKL_ERR_INVALID, "batch: the Reduce and the Scan must share a precision form");
+  if (segs && (!offsets || (n && !in)))
+    return fail(SKL_ERR_INVALID, "batch: NULL input or offsets");
+  if (count > skl_internal_batch_max_products())
+    return fail(SKL_ERR_INVALID, "batch: at most %d products per launch",
+                skl_internal_batch_max_products());
+  if (count > 0 && !descs) return fail(SKL_ERR_INVALID, "batch: descs is NULL");
+  if (precision == SKL_PREC_DEFAULT) precision = SKL_PREC_FP32;
+  if (count > 0 && precision != SKL_PREC_FP32 && precision != SKL_PREC_BF16X3)
+    return fail(SKL_ERR_UNSUPPORTED, "batch: products run in FP32 or BF16X3");
+  if (workers_per_cu < 0 || workers_per_cu > 2)
+    return fail(SKL_ERR_INVALID, "batch: product work-groups per CU must be 0, 1 or 2");
+  if (!segs && count == 0) return SKL_OK;
+  void* ws = nullptr;
+  if (segs) {
+    skl_skeleton_s* owner = scan ? scan : reduce;
+    SKL_TRY(batch_scratch(owner, skl_internal_batch_workspace(n, nseg)));
+    ws = owner->batch_scratch;
+  }
+  int dev = 0, cus = 256;
+  if (hipGetDevice(&dev) == hipSuccess) {
+    int v = 0;
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
+        v > 0)
+      cus = v;
+  }
+  const int workers = count > 0 ? cus * (workers_per_cu ? workers_per_cu : 1) : 0;
+  // no elements: the tile reads nothing, but its loads need a valid address
+  const float* src = n ? in : static_cast<const float*>(ws);
+  const int st = skl_internal_batch_run(src, segs ? scan_out : nullptr, segs ? sums : nullptr, n,
+                                        offsets, segs ? nseg : 0, inclusive,
+                                        fp64r || fp64s, descs, count, precision, workers, ws,
+                                        resolve_stream(stream));
+  if (st != SKL_OK) return fail(st, "batch launch failed");
+  return SKL_OK;
+}
+
+extern "C" int skl_batch_check(skl_skeleton s, int* invalid_offsets)
+{
+  SKL_TRY(ensure_initialized());
+  if (!s || !invalid_offsets) return fail(SKL_ERR_INVALID, "NULL argument");
+  *invalid_offsets = 0;
+  if (!s->batch_scratch) return SKL_OK;
+  SKL_HIP_TRY(hipDeviceSynchronize());
+  return skl_internal_batch_status(s->batch_scratch, invalid_offsets);
+}

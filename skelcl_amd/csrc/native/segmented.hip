@@ -22,13 +22,13 @@
 
 #include "skelcl_rt.h"
 #include "ld_util.h"
+#include "seg_tile.h"
 
 namespace {
 
 constexpr int WG = 256;
 constexpr int SUM_J = 16;                                 // float4 packs per thread
 constexpr long long SUM_CH = (long long)WG * SUM_J * 4;   // 16384 elements per chunk
-constexpr unsigned ST_AGG = 1u, ST_INCL = 2u;
 
 struct SegCtl {
   unsigned long long total;  // number of chunks (0 after an error)
@@ -193,92 +193,13 @@ __global__ __launch_bounds__(WG) void seg_combine_kernel(const long long* __rest
   if (lane == 0) out[s] = (float)v;
 }
 
-__device__ __forceinline__ void publish(unsigned long long* pub, long long c, unsigned state,
-                                        double v, unsigned epoch)
-{
-  const unsigned long long bits = (unsigned long long)__double_as_longlong(v);
-  const unsigned long long tag = (unsigned long long)((epoch << 2) | state) << 32;
-  __hip_atomic_store(&pub[2 * c], tag | (bits & 0xFFFFFFFFull), __ATOMIC_RELAXED,
-                     __HIP_MEMORY_SCOPE_AGENT);
-  __hip_atomic_store(&pub[2 * c + 1], tag | (bits >> 32), __ATOMIC_RELAXED,
-                     __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// state (AGG / INCL) once both words carry the same tag of this launch, else 0
-__device__ __forceinline__ unsigned read_status(const unsigned long long* pub, long long c,
-                                                unsigned epoch, double* v)
-{
-  const unsigned long long w0 =
-      __hip_atomic_load(&pub[2 * c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  const unsigned long long w1 =
-      __hip_atomic_load(&pub[2 * c + 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  const unsigned t0 = (unsigned)(w0 >> 32), t1 = (unsigned)(w1 >> 32);
-  if (t0 != t1 || (t0 >> 2) != epoch) return 0;
-  *v = __longlong_as_double((long long)((w1 << 32) | (w0 & 0xFFFFFFFFull)));
-  return t0 & 3u;
-}
-
-// ---- flat segmented Scan -------------------------------------------------
-// The n elements are cut into tiles of FTILE, independent of the segments.
-// Element e is a head when e == offsets[s] for some s (offsets[nseg] marks
-// the end of the last segment); a scan value restarts at every head.  The
-// pair (open, v) -- "no head seen yet", partial sum -- combines as
-//   (o1, v1) (+) (o2, v2) = (o1 && o2, o2 ? v1 + v2 : v2),
-// which is associative, so the tile runs the same single-pass decoupled
-// look-back as the plain Scan.  A tile that holds a head knows its inclusive
-// value at once (the sum after its last head) and publishes it as INCL
-// straight away, so a look-back never passes a tile with a head.
+// ---- flat segmented Scan ------------------------------------------------
+// One pass over tiles of FTILE elements cut regardless of the segments, the
+// segmented operator and a decoupled look-back (seg_tile.h); a prologue
+// checks the offsets and writes each tile's first segment.
 constexpr int FWG = 512;                                   // threads per tile
 constexpr int FJ = 16;                                     // float4 packs per lane
 constexpr long long FTILE = (long long)FWG * FJ * 4;       // 32768 elements
-constexpr int FWAVES = FWG / 64;
-
-// v of the lane named by the DPP control (zero where the control names no
-// lane or the row is outside ROW_MASK), moved as two dwords
-template <int CTRL, int ROW_MASK, typename A>
-__device__ __forceinline__ A dpp_acc(const A& v)
-{
-  int lo, hi;
-  acc_bits(v, &lo, &hi);
-  return acc_from_bits<A>(__builtin_amdgcn_update_dpp(0, lo, CTRL, ROW_MASK, 0xf, false),
-                          __builtin_amdgcn_update_dpp(0, hi, CTRL, ROW_MASK, 0xf, false));
-}
-
-// inclusive segmented wave scan: lane l ends with the sum of lanes
-// [hl, l], hl = the last lane <= l holding a head (0 when none).  Steps:
-// row_shr 1, 2, 4, 8 inside each 16-lane row, then row_bcast 15 (lane 15 of
-// each row into rows 1 and 3) and row_bcast 31 (lane 31 into rows 2 and 3);
-// a step adds its source lane s only when s >= hl.  All VALU, no LDS.
-template <typename A>
-__device__ __forceinline__ A seg_wave_scan(A v, int lane, int hl)
-{
-  A t;
-  t = dpp_acc<0x111, 0xf>(v);
-  if (lane - 1 >= hl) v = acc_add(v, t);
-  t = dpp_acc<0x112, 0xf>(v);
-  if (lane - 2 >= hl) v = acc_add(v, t);
-  t = dpp_acc<0x114, 0xf>(v);
-  if (lane - 4 >= hl) v = acc_add(v, t);
-  t = dpp_acc<0x118, 0xf>(v);
-  if (lane - 8 >= hl) v = acc_add(v, t);
-  t = dpp_acc<0x142, 0xa>(v);
-  if ((lane & ~15) - 1 >= hl) v = acc_add(v, t);
-  t = dpp_acc<0x143, 0xc>(v);
-  if (31 >= hl) v = acc_add(v, t);
-  return v;
-}
-
-// value of lane - 1 (zero in lane 0)
-template <typename A>
-__device__ __forceinline__ A wave_shr1(const A& v) { return dpp_acc<0x138, 0xf>(v); }
-
-template <typename A>
-__device__ __forceinline__ A readlane63(const A& v)
-{
-  int lo, hi;
-  acc_bits(v, &lo, &hi);
-  return acc_from_bits<A>(__builtin_amdgcn_readlane(lo, 63), __builtin_amdgcn_readlane(hi, 63));
-}
 
 // Prologue, one thread per run s = -1 .. nseg (run -1 is [0, offsets[0]),
 // run s < nseg is segment s, run nseg is [offsets[nseg], inf)):
@@ -326,15 +247,9 @@ __global__ __launch_bounds__(FWG) void seg_flat_scan_kernel(
     unsigned long long* __restrict__ ticket, unsigned long long* __restrict__ pub,
     int inclusive, const unsigned* __restrict__ err)
 {
-  typedef float f4u __attribute__((ext_vector_type(4), aligned(4)));
+  __shared__ segtile::Smem<A, FWG, FJ, 0> sm;
   __shared__ unsigned long long s_tile;
   __shared__ unsigned s_epoch, s_err;
-  __shared__ unsigned s_bits[FTILE / 32];
-  __shared__ A s_lpv[FJ][FWG];  // per pack: wave-exclusive prefix (own thread only)
-  __shared__ A s_wval[FWAVES];
-  __shared__ int s_wopen[FWAVES];
-  __shared__ double s_carry;
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
 
   // ticket word = epoch << 32 | tile; the work-group drawing the last tile
   // re-arms it for the next launch (every other draw has happened by then)
@@ -351,190 +266,12 @@ __global__ __launch_bounds__(FWG) void seg_flat_scan_kernel(
     const unsigned e = __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     s_err = (e >> 2) == ep && (e & 3u) != 0;
   }
-  for (int i = threadIdx.x; i < (int)(FTILE / 32); i += FWG) s_bits[i] = 0;
+  segtile::tile_clear(sm);
   __syncthreads();
-  if (s_err) return;  // invalid offsets: write nothing
-  const unsigned long long tile = s_tile;
-  const unsigned epoch = s_epoch;
-  const unsigned long long tbase = tile * (unsigned long long)FTILE;
-  const unsigned long long wbase = tbase + (unsigned long long)wv * (64 * FJ * 4);
-  const bool full = tbase + FTILE <= n;
-
-  float x[FJ][4];
-  if (full) {
-#pragma unroll
-    for (int j = 0; j < FJ; ++j) {
-      const f4u v = __builtin_nontemporal_load(
-          reinterpret_cast<const f4u*>(in + wbase + ((unsigned long long)j * 64 + lane) * 4));
-#pragma unroll
-      for (int k = 0; k < 4; ++k) x[j][k] = v[k];
-    }
-  } else {
-#pragma unroll
-    for (int j = 0; j < FJ; ++j)
-#pragma unroll
-      for (int k = 0; k < 4; ++k) {
-        const unsigned long long e = wbase + ((unsigned long long)j * 64 + lane) * 4 + k;
-        x[j][k] = ld_or(in, (size_t)e, 0, e < n, 0.0f);
-      }
-  }
-
-  // head bits of the tile: offsets[s0 .. s1].  A few are read by wave 0
-  // with scalar loads, which leave the data loads above in flight (a vector
-  // load issued after them could only be waited for together with them).
-  {
-    const int s1 = tfirst[tile + 1];
-    int s0 = tfirst[tile];
-    if (s0 < 0) s0 = 0;
-    if (s1 - s0 < 16) {
-      if (wv == 0)
-        for (int s = s0; s <= s1; ++s) {
-          const long long p = off[s] - (long long)tbase;
-          if (lane == 0 && p >= 0 && p < FTILE) atomicOr(&s_bits[p >> 5], 1u << (p & 31));
-        }
-    } else {
-      for (int s = s0 + (int)threadIdx.x; s <= s1; s += FWG) {
-        const long long p = off[s] - (long long)tbase;
-        if (p >= 0 && p < FTILE) atomicOr(&s_bits[p >> 5], 1u << (p & 31));
-      }
-    }
-  }
-  __syncthreads();
-  const long long lo = off[0], hi = off[nseg];
-
-  // per pack: (open, sum after its last head); one segmented wave scan per
-  // row of 64 packs
-  unsigned lpo = 0;  // bit j: pack j's wave-exclusive prefix is open
-  bool ropen = true;
-  A rval = acc_zero<A>();
-#pragma unroll
-  for (int j = 0; j < FJ; ++j) {
-    const int p = (wv * 64 * FJ + j * 64 + lane) * 4;
-    const unsigned hb = (s_bits[p >> 5] >> (p & 31)) & 0xFu;
-    A ps = acc_zero<A>();
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      if (hb & (1u << k)) ps = acc_zero<A>();
-      ps = acc_add(ps, x[j][k]);
-    }
-    const unsigned long long M = __ballot(hb != 0);
-    const unsigned long long le = M & ((2ull << lane) - 1ull);
-    const int hl = le ? 63 - __builtin_clzll(le) : 0;
-    const A inc = seg_wave_scan(ps, lane, hl);
-    const A ex = wave_shr1(inc);
-    const bool eopen = (M & ((1ull << lane) - 1ull)) == 0;
-    const A rt = readlane63(inc);
-    s_lpv[j][threadIdx.x] = eopen ? acc_add(rval, ex) : ex;
-    if (eopen && ropen) lpo |= 1u << j;
-    if (M) {
-      ropen = false;
-      rval = rt;
-    } else {
-      rval = acc_add(rval, rt);
-    }
-  }
-  // keep the floats, not their FP64 conversions, live until the outputs
-#pragma unroll
-  for (int j = 0; j < FJ; ++j)
-#pragma unroll
-    for (int k = 0; k < 4; ++k) asm volatile("" : "+v"(x[j][k]));
-  if (lane == 0) {
-    s_wval[wv] = rval;
-    s_wopen[wv] = ropen ? 1 : 0;
-  }
-  __syncthreads();
-  bool wopen = true, topen = true;
-  A wval = acc_zero<A>(), tval = acc_zero<A>();
-#pragma unroll
-  for (int w = 0; w < FWAVES; ++w) {
-    const A v = s_wval[w];
-    const bool o = s_wopen[w] != 0;
-    if (w < wv) {
-      wval = o ? acc_add(wval, v) : v;
-      wopen = wopen && o;
-    }
-    tval = o ? acc_add(tval, v) : v;
-    topen = topen && o;
-  }
-  const double tsum = acc_to_double(tval);
-
-  // look-back (wave 0); a tile whose first element is a head needs no carry
-  if (wv == 0) {
-    const bool need = tile > 0 && (s_bits[0] & 1u) == 0;
-    if (lane == 0) publish(pub, (long long)tile, topen ? ST_AGG : ST_INCL, tsum, epoch);
-    double excl = 0.0;
-    if (need) {
-      long long wend = (long long)tile;  // window [wend - 64, wend)
-      for (;;) {
-        const long long idx = wend - 64 + lane;
-        unsigned st = ST_INCL;
-        double v = 0.0;
-        if (idx >= 0) {
-          do {
-            st = read_status(pub, idx, epoch, &v);
-          } while (st == 0);
-        }
-        const unsigned long long im = __ballot(st == ST_INCL);
-        const int h = im ? 63 - __builtin_clzll(im) : -1;
-        double r = lane >= h ? v : 0.0;  // lanes [max(h, 0), 63]
-#pragma unroll
-        for (int o = 32; o >= 1; o >>= 1) r += __shfl_xor(r, o, 64);
-        excl += r;
-        if (h >= 0) break;
-        wend -= 64;
-      }
-    }
-    if (lane == 0) {
-      if (topen) publish(pub, (long long)tile, ST_INCL, excl + tsum, epoch);
-      s_carry = excl;
-    }
-  }
-  __syncthreads();
-  // an element's prefix = (the tile's carry, if no head precedes it in the
-  // tile) + (its sum in A since the tile's start or its last head); in the
-  // FP64 form the carry is folded into the sum at the pack's start
-  const double carry = s_carry;
-
-#pragma unroll
-  for (int j = 0; j < FJ; ++j) {
-    const int p = (wv * 64 * FJ + j * 64 + lane) * 4;
-    const unsigned hb = (s_bits[p >> 5] >> (p & 31)) & 0xFu;
-    const A lp = s_lpv[j][threadIdx.x];
-    const bool open = (lpo >> j) & 1u;
-    A r;
-    double c = 0.0;
-    if constexpr (__is_same(A, double)) {
-      const double wcarry = wopen ? carry + wval : wval;
-      r = open ? wcarry + lp : lp;
-    } else {
-      r = open ? acc_add(wval, lp) : lp;
-      c = open && wopen ? carry : 0.0;
-    }
-    float o[4];
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-      if (hb & (1u << k)) {
-        r = acc_zero<A>();
-        c = 0.0;
-      }
-      const A ex = r;
-      r = acc_add(r, x[j][k]);
-      o[k] = (float)(c + acc_to_double(inclusive ? r : ex));
-    }
-    const unsigned long long e0 = wbase + ((unsigned long long)j * 64 + lane) * 4;
-    if (full && (long long)e0 >= lo && (long long)e0 + 4 <= hi) {
-      f4u w;
-#pragma unroll
-      for (int k = 0; k < 4; ++k) w[k] = o[k];
-      __builtin_nontemporal_store(w, reinterpret_cast<f4u*>(out + e0));
-    } else {
-#pragma unroll
-      for (int k = 0; k < 4; ++k) {
-        const long long e = (long long)e0 + k;
-        if (e >= lo && e < hi) out[e] = o[k];
-      }
-    }
-  }
+  if (s_err) return;  // invalid offsets (every tile sees it): write nothing
+  segtile::Job jb{in, out, nullptr, n, off, nseg, tfirst, pub, nullptr,
+                  (long long)gridDim.x, inclusive};
+  segtile::tile_run<A, FWG, FJ, 0, false>(jb, s_tile, s_epoch, sm);
 }
 
 int device_cus()

@@ -58,6 +58,9 @@ struct skl_skeleton_s {
   // scratch of the batched segmented forms (zeroed when allocated)
   void* seg_scratch = nullptr;
   size_t seg_scratch_bytes = 0;
+  // scratch of one-launch batches (skl_batch_run; zeroed when allocated)
+  void* batch_scratch = nullptr;
+  size_t batch_scratch_bytes = 0;
   // scratch buffers outgrown by later calls, freed with the skeleton: a
   // captured graph may still use them
   std::vector<void*> retired;
