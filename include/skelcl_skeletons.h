@@ -306,8 +306,9 @@ int skl_matmul_batched_f32(const skl_gemm_desc* descs_device, int count,
 
 /* One launch for a batch of uneven work: the segmented Reduce(+) sums
  * (`reduce`, into sums[nseg]) and the segmented Scan(+) (`scan`, into
- * scan_out[n], exclusive or inclusive) of the same float array over the
- * same nseg segments (device offsets as above), and `count` small float
+ * scan_out[n], exclusive or inclusive) of the same array over the same
+ * nseg segments (device offsets as above; float, or int32 summed exactly
+ * modulo 2^32 -- both skeletons of one element type), and `count` small float
  * products (device descriptors, as for skl_matmul_batched_f32; at most
  * 2048; precision FP32 or BF16X3).  Any part may be left out (NULL output,
  * nseg or count 0).  One kernel launch, no prologue and no host
@@ -315,14 +316,14 @@ int skl_matmul_batched_f32(const skl_gemm_desc* descs_device, int count,
  * elements (a decoupled look-back; the sums are the inclusive values at
  * the segments' ends, so the array is read once for both), and
  * workers_per_cu (0 = 1, or 2) work-groups per CU walk the product tiles at
- * the same time.  The Reduce and the Scan must share a precision form
+ * the same time.  Float Reduce and Scan must share a precision form
  * (DEFAULT: compensated FP32 in a tile, FP64 across tiles; FP64).  The
  * offsets are checked by the tiles themselves: after invalid offsets the
  * outputs of the call are unspecified (in-bounds writes only) and
  * skl_batch_check reports it.  No counterpart in the reference: for
  * workloads of many uneven segments and small products. */
-int skl_batch_run(skl_skeleton reduce, float* sums, skl_skeleton scan,
-                  float* scan_out, int inclusive, const float* in, size_t n,
+int skl_batch_run(skl_skeleton reduce, void* sums, skl_skeleton scan,
+                  void* scan_out, int inclusive, const void* in, size_t n,
                   const int64_t* offsets, int nseg, const skl_gemm_desc* descs,
                   int count, int precision, int workers_per_cu,
                   skl_stream stream);

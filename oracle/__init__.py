@@ -83,6 +83,8 @@ def lib():
                                                ctypes.c_void_p, _int]),
             "orc_segmented_sum_f64": (None, [_f32p, _i64p, _int, _f64p]),
             "orc_segmented_scan_f64": (None, [_f32p, _i64p, _int, _int, _f64p]),
+            "orc_segmented_sum_u32": (None, [_u32p, _i64p, _int, _u32p]),
+            "orc_segmented_scan_u32": (None, [_u32p, _i64p, _int, _int, _u32p]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -361,6 +363,33 @@ def segmented_scan_f64(x, offsets, inclusive):
     _check_offsets(offsets, x.size)
     out = np.empty(x.size, np.float64)
     lib().orc_segmented_scan_f64(x, offsets, offsets.size - 1, int(bool(inclusive)), out)
+    return out
+
+
+def segmented_sum_u32(x, offsets):
+    """Per-segment sums of 32-bit integers modulo 2^32 (uint32 out)."""
+    x = np.ascontiguousarray(x)
+    if x.dtype == np.int32:
+        x = x.view(np.uint32)
+    x = _c(x, np.uint32)
+    offsets = _c(offsets, np.int64)
+    _check_offsets(offsets, x.size)
+    out = np.empty(offsets.size - 1, np.uint32)
+    lib().orc_segmented_sum_u32(x, offsets, offsets.size - 1, out)
+    return out
+
+
+def segmented_scan_u32(x, offsets, inclusive, fill=0):
+    """Per-segment scans of 32-bit integers modulo 2^32 (uint32 out);
+    elements outside [offsets[0], offsets[-1]) hold `fill`."""
+    x = np.ascontiguousarray(x)
+    if x.dtype == np.int32:
+        x = x.view(np.uint32)
+    x = _c(x, np.uint32)
+    offsets = _c(offsets, np.int64)
+    _check_offsets(offsets, x.size)
+    out = np.full(x.size, fill, np.uint32)
+    lib().orc_segmented_scan_u32(x, offsets, offsets.size - 1, int(bool(inclusive)), out)
     return out
 
 

@@ -47,7 +47,9 @@
  *                                  KATs (64^3, 32x128x32, 100x60x11) vs
  *                                  numpy matmul; sqdist vs
  *                                  |a|^2+|b|^2-2ab in FP64.
- *   orc_segmented_*                numpy per-segment sum/cumsum.
+ *   orc_segmented_*                numpy per-segment sum/cumsum; the u32
+ *                                  forms also against orc_scan_add_u32 per
+ *                                  segment and a Python loop mod 2^32.
  */
 #include "oracle.h"
 
@@ -708,3 +710,26 @@ void orc_segmented_scan_f64(const float* in, const int64_t* offsets, int nseg,
     orc_scan_add_f64(in + offsets[s], out + offsets[s],
                      (size_t)(offsets[s + 1] - offsets[s]), inclusive);
 }
+
+/* the same over 32-bit integers, modulo 2^32 (an int Reduce(+) / Scan(+)
+ * folds that wrap, ReduceKernel.cl:51-128 / ScanKernel.cl:74-250 on ints):
+ * each segment folded in element order; scan outputs outside
+ * [offsets[0], offsets[nseg]) are left as they are */
+void orc_segmented_sum_u32(const uint32_t* in, const int64_t* offsets, int nseg,
+                           uint32_t* out)
+{
+  for (int s = 0; s < nseg; ++s) {
+    uint32_t acc = 0;
+    for (int64_t i = offsets[s]; i < offsets[s + 1]; ++i) acc += in[i];
+    out[s] = acc;
+  }
+}
+
+void orc_segmented_scan_u32(const uint32_t* in, const int64_t* offsets, int nseg,
+                            int inclusive, uint32_t* out)
+{
+  for (int s = 0; s < nseg; ++s)
+    orc_scan_add_u32(in + offsets[s], out + offsets[s],
+                     (size_t)(offsets[s + 1] - offsets[s]), inclusive);
+}
+

@@ -102,6 +102,10 @@ void orc_segmented_sum_f64(const float* in, const int64_t* offsets,
                            int nseg, double* out);
 void orc_segmented_scan_f64(const float* in, const int64_t* offsets,
                             int nseg, int inclusive, double* out);
+void orc_segmented_sum_u32(const uint32_t* in, const int64_t* offsets, int nseg,
+                           uint32_t* out);
+void orc_segmented_scan_u32(const uint32_t* in, const int64_t* offsets, int nseg,
+                            int inclusive, uint32_t* out);
 
 #ifdef __cplusplus
 }

@@ -757,16 +757,16 @@ def batch_run(inp, offsets, reduce: "Reduce" = None, scan: "Scan" = None, inclus
               descs=None, count: int = 0, precision: int = PREC_FP32, workers_per_cu: int = 0,
               stream=None):
     """One launch for a batch (extension, skl_batch_run): the segment sums
-    (with `reduce`) and the segmented scan (with `scan`) of the Vector `inp`
-    over `offsets`, and `count` small products described by the device
+    (with `reduce`) and the segmented scan (with `scan`) of the float or int
+    Vector `inp` over `offsets`, and `count` small products described by the device
     buffer `descs` (skl_gemm_desc records).  Returns (sums, scan) Vectors,
     None for a part left out.  Host offsets are checked here; device
     offsets are checked by the launch (skl_batch_check)."""
     n = len(inp) if inp is not None else 0
     off = _segment_offsets(offsets, n) if offsets is not None else None
     nseg = len(off) - 1 if off is not None else 0
-    sums = Vector(size=nseg, etype="float") if reduce is not None and nseg > 0 else None
-    sc = Vector(size=n, etype="float") if scan is not None and nseg > 0 else None
+    sums = Vector(size=nseg, etype=reduce.types[0]) if reduce is not None and nseg > 0 else None
+    sc = Vector(size=n, etype=scan.types[0]) if scan is not None and nseg > 0 else None
     check(lib().skl_batch_run(
         reduce.handle if sums is not None else None,
         sums.device_buffer_for_output() if sums is not None else None,

@@ -36,7 +36,7 @@ extern "C" int skl_internal_batch_max_products();
 extern "C" int skl_internal_batch_status(const void* ws, int* bad);
 extern "C" int skl_internal_batch_run(const float* in, float* scan_out, float* sums,
                                       unsigned long long n, const int64_t* offsets, int nseg,
-                                      int inclusive, int fp64, const skl_gemm_desc* descs,
+                                      int inclusive, int form, const skl_gemm_desc* descs,
                                       int count, int precision, int workers, void* ws,
                                       hipStream_t s);
 extern "C" int skl_internal_batched_gemm_ws(const skl_gemm_desc* descs, int count, int precision,
@@ -111,10 +111,15 @@ int batch_scratch(skl_skeleton_s* s, size_t bytes)
   return SKL_OK;
 }
 
-bool is_float_add_skeleton(const skl_skeleton_s* s, int kind)
+// Reduce(+) / Scan(+) without extra arguments on float (1) or on 32-bit
+// integers (2, exact modulo 2^32); 0 otherwise
+int segment_add_type(const skl_skeleton_s* s, int kind)
 {
-  return s && s->kind == kind && s->op == UserOp::Add && s->extras.empty() &&
-         s->types[0].name == "float";
+  if (!s || s->kind != kind || s->op != UserOp::Add || !s->extras.empty()) return 0;
+  const std::string& t = s->types[0].name;
+  if (t == "float") return 1;
+  if (t == "int" || t == "unsigned int" || t == "unsigned" || t == "uint") return 2;
+  return 0;
 }
 
 bool is_float_addmul(const skl_skeleton_s* s)
@@ -367,8 +372,8 @@ extern "C" int skl_matmul_batched_f32(const skl_gemm_desc* descs_device, int cou
   return skl_internal_batched_gemm_ws(descs_device, count, precision, ws, resolve_stream(stream));
 }
 
-extern "C" int skl_batch_run(skl_skeleton reduce, float* sums, skl_skeleton scan, float* scan_out,
-                             int inclusive, const float* in, size_t n, const int64_t* offsets,
+extern "C" int skl_batch_run(skl_skeleton reduce, void* sums, skl_skeleton scan, void* scan_out,
+                             int inclusive, const void* in, size_t n, const int64_t* offsets,
                              int nseg, const skl_gemm_desc* descs, int count, int precision,
                              int workers_per_cu, skl_stream stream)
 {
@@ -376,14 +381,19 @@ extern "C" int skl_batch_run(skl_skeleton reduce, float* sums, skl_skeleton scan
   if (nseg < 0 || count < 0) return fail(SKL_ERR_INVALID, "negative segment or product count");
   if (!sums) reduce = nullptr;
   if (!scan_out) scan = nullptr;
-  if (sums && !is_float_add_skeleton(reduce, SKL_REDUCE))
-    return fail(SKL_ERR_UNSUPPORTED, "batch: segment sums need a float Reduce(+) skeleton");
-  if (scan_out && !is_float_add_skeleton(scan, SKL_SCAN))
-    return fail(SKL_ERR_UNSUPPORTED, "batch: the segmented scan needs a float Scan(+) skeleton");
+  const int tr = reduce ? segment_add_type(reduce, SKL_REDUCE) : 0;
+  const int ts = scan ? segment_add_type(scan, SKL_SCAN) : 0;
+  if (reduce && !tr)
+    return fail(SKL_ERR_UNSUPPORTED, "batch: segment sums need a float or int Reduce(+)");
+  if (scan && !ts)
+    return fail(SKL_ERR_UNSUPPORTED, "batch: the segmented scan needs a float or int Scan(+)");
+  if (reduce && scan && tr != ts)
+    return fail(SKL_ERR_INVALID, "batch: the Reduce and the Scan must share an element type");
   const bool segs = nseg > 0 && (reduce || scan);
+  const bool ints = (tr ? tr : ts) == 2;
   const int fp64r = reduce && reduce->precision == SKL_PREC_FP64;
   const int fp64s = scan && scan->precision == SKL_PREC_FP64;
-  if (reduce && scan && fp64r != fp64s)
+  if (!ints && reduce && scan && fp64r != fp64s)
     return fail(SKL_ERR_INVALID, "batch: the Reduce and the Scan must share a precision form");
   if (segs && (!offsets || (n && !in)))
     return fail(SKL_ERR_INVALID, "batch: NULL input or offsets");
@@ -412,11 +422,12 @@ extern "C" int skl_batch_run(skl_skeleton reduce, float* sums, skl_skeleton scan
   }
   const int workers = count > 0 ? cus * (workers_per_cu ? workers_per_cu : 1) : 0;
   // no elements: the tile reads nothing, but its loads need a valid address
-  const float* src = n ? in : static_cast<const float*>(ws);
-  const int st = skl_internal_batch_run(src, segs ? scan_out : nullptr, segs ? sums : nullptr, n,
+  const float* src = static_cast<const float*>(n ? in : ws);
+  const int st = skl_internal_batch_run(src, segs ? static_cast<float*>(scan_out) : nullptr,
+                                        segs ? static_cast<float*>(sums) : nullptr, n,
                                         offsets, segs ? nseg : 0, inclusive,
-                                        fp64r || fp64s, descs, count, precision, workers, ws,
-                                        resolve_stream(stream));
+                                        ints ? 2 : (fp64r || fp64s ? 1 : 0), descs, count,
+                                        precision, workers, ws, resolve_stream(stream));
   if (st != SKL_OK) return fail(st, "batch launch failed");
   return SKL_OK;
 }

@@ -1,6 +1,7 @@
 // One launch for a batch of uneven work (beyond the reference): the
-// segmented Reduce(+) sums and the segmented Scan(+) of one float array over
-// the same segments, and a list of small float products, in one grid.
+// segmented Reduce(+) sums and the segmented Scan(+) of one float or int32
+// array over the same segments, and a list of small float products, in one
+// grid.
 //
 // Work-groups [0, W) are product workers: each numbers the tiles of all
 // products itself (a block scan of the per-product tile counts into LDS) and
@@ -216,11 +217,12 @@ extern "C" int skl_internal_batch_status(const void* ws, int* bad)
   return SKL_OK;
 }
 
-// workers: product work-groups (0: none).  fp64: FP64 segment sums
-// throughout, else compensated pairs in a tile and FP64 across tiles.
+// workers: product work-groups (0: none).  form: the segments' element
+// type and accumulator: 0 float, compensated pairs in a tile and FP64
+// across tiles; 1 float, FP64 throughout; 2 int32, exact modulo 2^32.
 extern "C" int skl_internal_batch_run(const float* in, float* scan_out, float* sums,
                                       unsigned long long n, const int64_t* offsets, int nseg,
-                                      int inclusive, int fp64, const skl_gemm_desc* descs,
+                                      int inclusive, int form, const skl_gemm_desc* descs,
                                       int count, int precision, int workers, void* ws,
                                       hipStream_t s)
 {
@@ -251,14 +253,18 @@ extern "C" int skl_internal_batch_run(const float* in, float* scan_out, float* s
               count,
               workers};
   const dim3 g((unsigned)grid);
+  if (form < 0 || form > 2) return SKL_ERR_INVALID;
   if (count == 0) {
-    if (fp64) batch_segments_kernel<double><<<g, NT_SEG, 0, s>>>(jb);
+    if (form == 2) batch_segments_kernel<acc_u32><<<g, NT_SEG, 0, s>>>(jb);
+    else if (form == 1) batch_segments_kernel<double><<<g, NT_SEG, 0, s>>>(jb);
     else batch_segments_kernel<acc_df><<<g, NT_SEG, 0, s>>>(jb);
   } else if (precision == SKL_PREC_BF16X3) {
-    if (fp64) batch_kernel<double, SKL_PREC_BF16X3><<<g, NT, 0, s>>>(jb);
+    if (form == 2) batch_kernel<acc_u32, SKL_PREC_BF16X3><<<g, NT, 0, s>>>(jb);
+    else if (form == 1) batch_kernel<double, SKL_PREC_BF16X3><<<g, NT, 0, s>>>(jb);
     else batch_kernel<acc_df, SKL_PREC_BF16X3><<<g, NT, 0, s>>>(jb);
   } else if (precision == SKL_PREC_FP32) {
-    if (fp64) batch_kernel<double, SKL_PREC_FP32><<<g, NT, 0, s>>>(jb);
+    if (form == 2) batch_kernel<acc_u32, SKL_PREC_FP32><<<g, NT, 0, s>>>(jb);
+    else if (form == 1) batch_kernel<double, SKL_PREC_FP32><<<g, NT, 0, s>>>(jb);
     else batch_kernel<acc_df, SKL_PREC_FP32><<<g, NT, 0, s>>>(jb);
   } else {
     return SKL_ERR_UNSUPPORTED;

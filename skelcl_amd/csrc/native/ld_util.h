@@ -99,3 +99,31 @@ __device__ __forceinline__ acc_df acc_from_bits<acc_df>(int lo, int hi)
 {
   return acc_df{__int_as_float(lo), __int_as_float(hi)};
 }
+
+// Exact int32 (+) accumulator of the segmented forms: the element bits read
+// as an integer, sums modulo 2^32 (as the reference's int Scan and Reduce
+// wrap).
+struct acc_u32 {
+  unsigned v;
+};
+template <>
+__device__ __forceinline__ acc_u32 acc_zero<acc_u32>() { return acc_u32{0u}; }
+__device__ __forceinline__ acc_u32 acc_add(const acc_u32& a, const acc_u32& b)
+{
+  return acc_u32{a.v + b.v};
+}
+__device__ __forceinline__ acc_u32 acc_add(const acc_u32& a, float x)
+{
+  return acc_u32{a.v + __float_as_uint(x)};
+}
+__device__ __forceinline__ void acc_bits(const acc_u32& v, int* lo, int* hi)
+{
+  *lo = (int)v.v;
+  *hi = 0;
+}
+template <>
+__device__ __forceinline__ acc_u32 acc_from_bits<acc_u32>(int lo, int)
+{
+  return acc_u32{(unsigned)lo};
+}
+

@@ -15,7 +15,9 @@
 // Optionally the tile also writes each segment's sum (the Reduce(+) of the
 // segment: the inclusive prefix at its last element, carry included, in
 // FP64 before the final rounding; 0 for an empty segment) -- one pass
-// serves the segmented Reduce and Scan of the same array.
+// serves the segmented Reduce and Scan of the same array.  Float tiles sum
+// in A (compensated pairs or FP64); int32 tiles (A = acc_u32) sum exactly
+// modulo 2^32.
 //
 // The tile finds the segments it meets either in a table a prologue wrote
 // (tfirst) or by searching the offsets itself; in the second form it also
@@ -33,10 +35,58 @@ namespace segtile {
 
 constexpr unsigned ST_AGG = 1u, ST_INCL = 2u;
 
-__device__ __forceinline__ void publish(unsigned long long* pub, long long c, unsigned state,
-                                        double v, unsigned epoch)
+// Carries (tile aggregates, look-back sums, segment totals): FP64 for float,
+// the accumulator itself for int32.  A float tile adds its carry to each
+// output's compensated sum at the end; where the carry and the accumulator
+// are one type (FP64, int32) the carry is folded in at each pack's start.
+template <typename A>
+struct CarryOf {
+  typedef double type;
+};
+template <>
+struct CarryOf<acc_u32> {
+  typedef acc_u32 type;
+};
+
+__device__ __forceinline__ double to_carry(double a) { return a; }
+__device__ __forceinline__ double to_carry(const acc_df& a) { return acc_to_double(a); }
+__device__ __forceinline__ acc_u32 to_carry(const acc_u32& a) { return a; }
+__device__ __forceinline__ unsigned long long carry_bits(double v)
 {
-  const unsigned long long bits = (unsigned long long)__double_as_longlong(v);
+  return (unsigned long long)__double_as_longlong(v);
+}
+__device__ __forceinline__ unsigned long long carry_bits(const acc_u32& v) { return v.v; }
+template <typename C>
+__device__ __forceinline__ C carry_from_bits(unsigned long long b);
+template <>
+__device__ __forceinline__ double carry_from_bits<double>(unsigned long long b)
+{
+  return __longlong_as_double((long long)b);
+}
+template <>
+__device__ __forceinline__ acc_u32 carry_from_bits<acc_u32>(unsigned long long b)
+{
+  return acc_u32{(unsigned)b};
+}
+__device__ __forceinline__ double carry_add(double a, double b) { return a + b; }
+__device__ __forceinline__ acc_u32 carry_add(const acc_u32& a, const acc_u32& b)
+{
+  return acc_u32{a.v + b.v};
+}
+__device__ __forceinline__ double carry_shfl_xor(double v, int o) { return __shfl_xor(v, o, 64); }
+__device__ __forceinline__ acc_u32 carry_shfl_xor(const acc_u32& v, int o)
+{
+  return acc_u32{(unsigned)__shfl_xor((int)v.v, o, 64)};
+}
+// the stored bits of an output or a segment total
+__device__ __forceinline__ float out_value(double v) { return (float)v; }
+__device__ __forceinline__ float out_value(const acc_u32& v) { return __uint_as_float(v.v); }
+
+template <typename C>
+__device__ __forceinline__ void publish(unsigned long long* pub, long long c, unsigned state,
+                                        const C& v, unsigned epoch)
+{
+  const unsigned long long bits = carry_bits(v);
   const unsigned long long tag = (unsigned long long)((epoch << 2) | state) << 32;
   __hip_atomic_store(&pub[2 * c], tag | (bits & 0xFFFFFFFFull), __ATOMIC_RELAXED,
                      __HIP_MEMORY_SCOPE_AGENT);
@@ -45,8 +95,9 @@ __device__ __forceinline__ void publish(unsigned long long* pub, long long c, un
 }
 
 // state (AGG / INCL) once both words carry the same tag of this launch, else 0
+template <typename C>
 __device__ __forceinline__ unsigned read_status(const unsigned long long* pub, long long c,
-                                                unsigned epoch, double* v)
+                                                unsigned epoch, C* v)
 {
   const unsigned long long w0 =
       __hip_atomic_load(&pub[2 * c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -54,7 +105,7 @@ __device__ __forceinline__ unsigned read_status(const unsigned long long* pub, l
       __hip_atomic_load(&pub[2 * c + 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   const unsigned t0 = (unsigned)(w0 >> 32), t1 = (unsigned)(w1 >> 32);
   if (t0 != t1 || (t0 >> 2) != epoch) return 0;
-  *v = __longlong_as_double((long long)((w1 << 32) | (w0 & 0xFFFFFFFFull)));
+  *v = carry_from_bits<C>((w1 << 32) | (w0 & 0xFFFFFFFFull));
   return t0 & 3u;
 }
 
@@ -165,8 +216,8 @@ struct Smem {
   A wval[NT / 64];
   int wopen[NT / 64];
   int wcnt[NT / 64];
-  double carry;
-  double slot[SLOTS ? SLOTS : 1];
+  typename CarryOf<A>::type carry;
+  typename CarryOf<A>::type slot[SLOTS ? SLOTS : 1];
 };
 
 struct Job {
@@ -359,19 +410,21 @@ __device__ __forceinline__ void tile_run(const Job& jb, unsigned long long tile,
     tval = o ? acc_add(tval, v) : v;
     topen = topen && o;
   }
-  const double tsum = acc_to_double(tval);
+  typedef typename CarryOf<A>::type C;
+  constexpr bool FOLD = __is_same(A, C);  // the carry joins each pack's sum
+  const C tsum = to_carry(tval);
 
   // look-back (wave 0); a tile whose first element is a head needs no carry
   if (wv == 0) {
     const bool need = tile > 0 && (sm.bits[0] & 1u) == 0;
     if (lane == 0) publish(jb.pub, (long long)tile, topen ? ST_AGG : ST_INCL, tsum, epoch);
-    double excl = 0.0;
+    C excl = carry_from_bits<C>(0);
     if (need) {
       long long wend = (long long)tile;  // window [wend - 64, wend)
       for (;;) {
         const long long idx = wend - 64 + lane;
         unsigned st = ST_INCL;
-        double v = 0.0;
+        C v = carry_from_bits<C>(0);
         if (idx >= 0) {
           do {
             st = read_status(jb.pub, idx, epoch, &v);
@@ -379,16 +432,16 @@ __device__ __forceinline__ void tile_run(const Job& jb, unsigned long long tile,
         }
         const unsigned long long im = __ballot(st == ST_INCL);
         const int h = im ? 63 - __builtin_clzll(im) : -1;
-        double r = lane >= h ? v : 0.0;  // lanes [max(h, 0), 63]
+        C r = lane >= h ? v : carry_from_bits<C>(0);  // lanes [max(h, 0), 63]
 #pragma unroll
-        for (int o = 32; o >= 1; o >>= 1) r += __shfl_xor(r, o, 64);
-        excl += r;
+        for (int o = 32; o >= 1; o >>= 1) r = carry_add(r, carry_shfl_xor(r, o));
+        excl = carry_add(excl, r);
         if (h >= 0) break;
         wend -= 64;
       }
     }
     if (lane == 0) {
-      if (topen) publish(jb.pub, (long long)tile, ST_INCL, excl + tsum, epoch);
+      if (topen) publish(jb.pub, (long long)tile, ST_INCL, carry_add(excl, tsum), epoch);
       sm.carry = excl;
     }
   }
@@ -396,7 +449,7 @@ __device__ __forceinline__ void tile_run(const Job& jb, unsigned long long tile,
   // an element's prefix = (the tile's carry, if no head precedes it in the
   // tile) + (its sum in A since the tile's start or its last head); in the
   // FP64 form the carry is folded into the sum at the pack's start
-  const double carry = sm.carry;
+  const C carry = sm.carry;
 
   // The outputs, and the segment totals at the tails: the first SLOTS of
   // them (in element order) through LDS slots to the segment side below;
@@ -412,10 +465,10 @@ __device__ __forceinline__ void tile_run(const Job& jb, unsigned long long tile,
     const A lp = sm.lpv[j][threadIdx.x];
     const bool open = (lpo >> j) & 1u;
     A r;
-    double c = 0.0;
-    if constexpr (__is_same(A, double)) {
-      const double wcarry = wopen ? carry + wval : wval;
-      r = open ? wcarry + lp : lp;
+    double c = 0.0;  // the carry still to add (compensated float form)
+    if constexpr (FOLD) {
+      const A wcarry = wopen ? acc_add(carry, wval) : wval;
+      r = open ? acc_add(wcarry, lp) : lp;
     } else {
       r = open ? acc_add(wval, lp) : lp;
       c = open && wopen ? carry : 0.0;
@@ -436,10 +489,15 @@ __device__ __forceinline__ void tile_run(const Job& jb, unsigned long long tile,
       }
       const A ex = r;
       r = acc_add(r, x[j][k]);
-      o[k] = (float)(c + acc_to_double(jb.inclusive ? r : ex));
+      if constexpr (FOLD)
+        o[k] = out_value(jb.inclusive ? r : ex);
+      else
+        o[k] = (float)(c + acc_to_double(jb.inclusive ? r : ex));
       if constexpr (SLOTS > 0) {
         if (tb & (1u << k)) {
-          const double tot = c + acc_to_double(r);
+          C tot;
+          if constexpr (FOLD) tot = r;
+          else tot = c + acc_to_double(r);
           if (tord < SLOTS) {
             sm.slot[tord] = tot;
           } else {
@@ -450,7 +508,7 @@ __device__ __forceinline__ void tile_run(const Job& jb, unsigned long long tile,
               if (off[m] <= e) l = m;
               else h = m;
             }
-            jb.sums[l] = (float)tot;
+            jb.sums[l] = out_value(tot);
           }
           ++tord;
         }
@@ -481,7 +539,7 @@ __device__ __forceinline__ void tile_run(const Job& jb, unsigned long long tile,
         if (!(a < b && b > (long long)tbase && b <= tend)) return;
         const int q = (int)(b - 1 - (long long)tbase);
         const int ord = sm.tpre[q >> 5] + __popc(sm.tails[q >> 5] & ((1u << (q & 31)) - 1u));
-        if (writer && ord < SLOTS) jb.sums[s] = (float)sm.slot[ord];
+        if (writer && ord < SLOTS) jb.sums[s] = out_value(sm.slot[ord]);
       };
       if (sb - sa <= 16) {
         if (wv == 0)

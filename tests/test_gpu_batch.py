@@ -15,7 +15,8 @@ import oracle
 
 pytestmark = pytest.mark.gpu
 
-TILE = 16384  # elements per segment tile of the batch launch
+TILE = 16384  # elements per segment tile of a batch launch with products
+TILE_SEG = 32768  # ... of a launch of segments alone
 ADD = "float func(float x, float y){ return x+y; }"
 
 
@@ -109,15 +110,17 @@ def test_batch_segments_and_products(skl, seg_prec, mm_prec, workers):
 @pytest.mark.parametrize("case", ["inner_range", "dense_tiny", "tile_edges", "ragged_tail",
                                   "empty_runs"])
 @pytest.mark.parametrize("inclusive", [False, True])
-def test_batch_segment_layouts(skl, case, inclusive):
-    """Segment layouts against the 16384-element tiles: uncovered ends
-    (outputs left as they are), runs of empty and one-element segments (more
-    segment ends per tile than one round of slots holds), heads on, before
-    and after tile starts, a last tile cut short, empty segments at both
-    ends of the array."""
+@pytest.mark.parametrize("alone", [True, False])
+def test_batch_segment_layouts(skl, case, inclusive, alone):
+    """Segment layouts against the tiles (32768 elements in a launch of
+    segments alone, 16384 beside products): uncovered ends (outputs left as
+    they are), runs of empty and one-element segments (more segment ends per
+    tile than the LDS slots hold: the rest are found by searching the
+    offsets), heads on, before and after tile starts, a last tile cut short,
+    empty segments at both ends of the array."""
     rng = np.random.default_rng({"inner_range": 1, "dense_tiny": 2, "tile_edges": 3,
                                  "ragged_tail": 4, "empty_runs": 5}[case])
-    T = TILE
+    T = TILE_SEG if alone else TILE
     if case == "inner_range":
         n = 5 * T + 123
         off = [1000, 1000, 20000, 20001, 60000, 4 * T + 5]
@@ -143,7 +146,11 @@ def test_batch_segment_layouts(skl, case, inclusive):
     X, O = skl.DeviceBuffer.from_array(x), skl.DeviceBuffer.from_array(np.array(off, np.int64))
     S = skl.DeviceBuffer.from_array(np.full(nseg, 7.0, np.float32))
     Y = skl.DeviceBuffer.from_array(np.full(n, 7.0, np.float32))
-    _run(skl, red, S, scan, Y, inclusive, X, n, O, nseg)
+    dd, count = None, 0
+    if not alone:
+        dd, keep, refs = _products(skl, [(32, 48, 16)], 0x5400)
+        count = 1
+    _run(skl, red, S, scan, Y, inclusive, X, n, O, nseg, dd, count, 2, 1)
     _check_sums(S.to_numpy(np.float32, nseg), x, off)
     _check_scan(Y.to_numpy(np.float32, n), x, off, inclusive, n)
 
@@ -257,3 +264,73 @@ def test_batch_graph_replays(skl):
                                       c0.view(np.uint32)), rep
     finally:
         _lib.check(L.skl_graph_destroy(g))
+
+
+@pytest.mark.parametrize("case", ["fixture", "dense_tiny", "tile_edges", "wrap"])
+@pytest.mark.parametrize("with_products", [False, True])
+def test_batch_int_segments_exact(skl, case, with_products):
+    """int Reduce(+) / Scan(+) segments (exact modulo 2^32) against the
+    oracle, bit for bit: the batched workload's lengths (first 64), runs of
+    empty and one-element segments, heads at the tile edges of both tile
+    sizes (16384 beside products, 32768 alone), and values whose sums wrap."""
+    from skelcl_amd import _lib
+    rng = np.random.default_rng({"fixture": 11, "dense_tiny": 12, "tile_edges": 13,
+                                 "wrap": 14}[case])
+    if case == "fixture":
+        lens = np.array(_fixture()["segment_lengths"][:64], np.int64)
+        off = np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)
+        n = int(off[-1])
+        x = oracle.gen_i32_below(0x5101, n, 1000)
+    elif case == "dense_tiny":
+        lens = rng.integers(0, 4, 20000)
+        off = np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)
+        n = int(off[-1]) + 9
+        x = oracle.gen_i32_below(0x5102, n, 1000)
+    elif case == "tile_edges":
+        T, U = TILE, 2 * TILE
+        off = np.array([0, T - 1, T, T + 1, U - 1, U, U + 1, 3 * T, 4 * T, 5 * T + 3], np.int64)
+        n = int(off[-1]) + 100
+        x = oracle.gen_i32_below(0x5103, n, 1000)
+    else:
+        off = np.array([0, 3, 3, 70000, 70001, 150000], np.int64)
+        n = 150000
+        x = oracle.gen_u64(0x5104, n).astype(np.uint32).view(np.int32)
+    nseg = len(off) - 1
+    red = skl.Reduce("int(int)", "int func(int x, int y){ return x+y; }")
+    scan = skl.Scan("int(int)", "int func(int x, int y){ return x+y; }", "0")
+    X, O = skl.DeviceBuffer.from_array(x), skl.DeviceBuffer.from_array(off)
+    dd, count = None, 0
+    if with_products:
+        dd, keep, refs = _products(skl, [(64, 64, 64), (130, 7, 33)], 0x5300)
+        count = 2
+    for incl in (False, True):
+        S = skl.DeviceBuffer.from_array(np.full(nseg, 0x5EED, np.int32))
+        Y = skl.DeviceBuffer.from_array(np.full(n, 0x5EED, np.int32))
+        _run(skl, red, S, scan, Y, incl, X, n, O, nseg, dd, count, 2, 1)
+        assert np.array_equal(S.to_numpy(np.uint32, nseg), oracle.segmented_sum_u32(x, off))
+        assert np.array_equal(Y.to_numpy(np.uint32, n),
+                              oracle.segmented_scan_u32(x, off, incl, fill=0x5EED))
+    bad = C.c_int(-1)
+    _lib.check(_lib.lib().skl_batch_check(scan.handle, C.byref(bad)))
+    assert bad.value == 0
+    if with_products:
+        for dC, M, N, ref in refs:
+            c = dC.to_numpy(np.float32, M * N).reshape(M, N)
+            assert np.linalg.norm(c - ref) / np.linalg.norm(ref) < 1e-6
+
+
+def test_batch_rejects_mixed_element_types(skl):
+    """A float Reduce with an int Scan, or a Reduce(*), is refused."""
+    from skelcl_amd import _lib
+    x = skl.DeviceBuffer.from_array(np.zeros(16, np.float32))
+    O = skl.DeviceBuffer.from_array(np.array([0, 16], np.int64))
+    S, Y = skl.DeviceBuffer(4), skl.DeviceBuffer(64)
+    rf = skl.Reduce("float(float)", ADD)
+    si = skl.Scan("int(int)", "int func(int x, int y){ return x+y; }", "0")
+    rm = skl.Reduce("float(float)", "float func(float x, float y){ return x*y; }")
+    L = _lib.lib()
+    assert L.skl_batch_run(rf.handle, S.ptr, si.handle, Y.ptr, 1, x.ptr, 16, O.ptr, 1, None, 0,
+                           1, 0, None) == _lib.SKL_ERR_INVALID
+    assert L.skl_batch_run(rm.handle, S.ptr, None, None, 1, x.ptr, 16, O.ptr, 1, None, 0, 1, 0,
+                           None) == _lib.SKL_ERR_UNSUPPORTED
+

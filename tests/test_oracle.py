@@ -285,6 +285,38 @@ def test_segmented():
         assert np.allclose(sc[off[k]:off[k + 1]], np.cumsum(seg), atol=1e-12)
 
 
+def test_segmented_u32():
+    """Integer segments modulo 2^32: against a Python loop (tiny case, values
+    that wrap) and against orc_scan_add_u32 / numpy int64 sums per segment."""
+    x = np.array([0xFFFFFFF0, 0x20, 7, 0x80000000, 0x80000000, 1, 2, 3], np.uint32)
+    off = np.array([0, 2, 2, 5, 8], np.int64)
+    s = oracle.segmented_sum_u32(x, off)
+    assert s.tolist() == [0x10, 0, 7, 6]  # 0xFFFFFFF0 + 0x20 wraps to 0x10
+    for incl in (False, True):
+        sc = oracle.segmented_scan_u32(x, off, incl, fill=0xDEAD)
+        exp = []
+        for k in range(4):
+            acc = 0
+            for i in range(off[k], off[k + 1]):
+                if incl:
+                    acc = (acc + int(x[i])) & 0xFFFFFFFF
+                    exp.append(acc)
+                else:
+                    exp.append(acc)
+                    acc = (acc + int(x[i])) & 0xFFFFFFFF
+        assert sc.tolist() == exp
+    y = oracle.gen_i32_below(73, 5000, 1 << 30)
+    o2 = np.array([100, 100, 101, 2500, 4000], np.int64)
+    s2 = oracle.segmented_sum_u32(y, o2)
+    sc2 = oracle.segmented_scan_u32(y, o2, True, fill=9)
+    for k in range(4):
+        seg = y[o2[k]:o2[k + 1]].astype(np.int64)
+        assert s2[k] == np.uint32(seg.sum() & 0xFFFFFFFF)
+        assert np.array_equal(sc2[o2[k]:o2[k + 1]],
+                              oracle.scan_add_u32(y[o2[k]:o2[k + 1]], True))
+    assert np.all(sc2[:100] == 9) and np.all(sc2[4000:] == 9)
+
+
 # ---- exactly rounded references (Python Fractions) ------------------------
 from fractions import Fraction  # noqa: E402
 
