@@ -588,9 +588,22 @@ __global__ __launch_bounds__(MB * NB * 128 * 128 / (WR * 16 * 64) * 64, 1) void 
   // the tile scales are powers of two: their exponent fields are the E8M0
   // block scales of the MFMA, which then accumulates directly
   const int ablock = tm * MB + (arow0 >> 7), bblock = tn * NB + (wn >> 1);
+  // The exponents of this wave's A and B tile scales, slice k in lane
+  // k % 64 of a register that holds slices [64 c, 64 c + 64), read with
+  // readlane: a slice starts without waiting for a scalar load of its
+  // scales (two per slice cost a memory round trip after every barrier).
+  int ea_v = 0, eb_v = 0;
+  auto load_scales = [&](int c) {
+    const int k = 64 * c + lane;
+    const int kk = k < nkb ? k : nkb - 1;
+    ea_v = (int)((__float_as_uint(sa[(size_t)ablock * nkb + kk]) >> 23) & 0xFF);
+    eb_v = (int)((__float_as_uint(sb[(size_t)bblock * nkb + kk]) >> 23) & 0xFF);
+  };
+  load_scales(0);
   auto slice = [&](const uint8_t* Ac, const uint8_t* Bc, uint8_t* An, uint8_t* Bn, int k) {
-    const int ea = (int)((__float_as_uint(sa[(size_t)ablock * nkb + k]) >> 23) & 0xFF);
-    const int eb = (int)((__float_as_uint(sb[(size_t)bblock * nkb + k]) >> 23) & 0xFF);
+    if (k > 0 && (k & 63) == 0) load_scales(k >> 6);
+    const int ea = __builtin_amdgcn_readlane(ea_v, k & 63);
+    const int eb = __builtin_amdgcn_readlane(eb_v, k & 63);
     // B fragments for the whole slice; A fragments one ahead of their four
     // MFMAs (scheduling barriers keep the compiler from hoisting all of them,
     // which would not fit in the registers)
