@@ -206,6 +206,74 @@ extern "C" __global__ __launch_bounds__(256) void probe_zip(const u4* __restrict
 """
 
 
+# LDS read probes: conflict-free 16- and 8-byte reads (ds_read_b128 /
+# ds_read_b64) of a 32 KiB buffer in a loop, 8 reads in flight per lane; the
+# ceiling the LDS-fed kernels (MFMA products, stencil) are compared with.
+LDS_SRC = r"""
+typedef unsigned int u4 __attribute__((ext_vector_type(4)));
+typedef unsigned int u2 __attribute__((ext_vector_type(2)));
+extern "C" __global__ __launch_bounds__(512) void lds_read_b128(unsigned* sink, int iters) {
+  __shared__ u4 buf[2048];
+  const int t = threadIdx.x;
+  for (int i = t; i < 2048; i += blockDim.x) buf[i] = u4{(unsigned)i, 1u, 2u, 3u};
+  __syncthreads();
+  u4 acc = {0u, 0u, 0u, 0u};
+  int idx = t & 2047;
+  for (int it = 0; it < iters; ++it) {
+#pragma unroll
+    for (int k = 0; k < 8; ++k) acc ^= buf[(idx + k * 64) & 2047];
+    idx = (idx + 512) & 2047;
+  }
+  if ((acc.x ^ acc.y ^ acc.z ^ acc.w) == 0x9E3779B9u) sink[0] = 1u;
+}
+extern "C" __global__ __launch_bounds__(512) void lds_read_b64(unsigned* sink, int iters) {
+  __shared__ u2 buf[4096];
+  const int t = threadIdx.x;
+  for (int i = t; i < 4096; i += blockDim.x) buf[i] = u2{(unsigned)i, 1u};
+  __syncthreads();
+  u2 acc = {0u, 0u};
+  int idx = t & 4095;
+  for (int it = 0; it < iters; ++it) {
+#pragma unroll
+    for (int k = 0; k < 8; ++k) acc ^= buf[(idx + k * 64) & 4095];
+    idx = (idx + 512) & 4095;
+  }
+  if ((acc.x ^ acc.y) == 0x9E3779B9u) sink[0] = 1u;
+}
+"""
+
+
+def run_lds_ceiling(skelcl, L, ck, stream):
+    """LDS read bandwidth per CU at a few work-group counts and sizes."""
+    import ctypes as C
+    st = stream.handle
+    prog, log = C.c_void_p(), C.c_void_p()
+    ck(L.skl_program_build(LDS_SRC.encode(), b"", C.byref(prog), C.byref(log)))
+    skelcl._lib.take_string(log)
+    cus = skelcl.device_info()["compute_units"]
+    sink = skelcl.DeviceBuffer(256)
+    sp, iters = C.c_void_p(sink.ptr.value), C.c_int(4096)
+    args = (C.c_void_p * 2)(C.cast(C.pointer(sp), C.c_void_p).value,
+                            C.cast(C.pointer(iters), C.c_void_p).value)
+    out = {}
+    for name, width in (("lds_read_b128", 16), ("lds_read_b64", 8)):
+        k = C.c_void_p()
+        ck(L.skl_kernel_get(prog, name.encode(), C.byref(k)))
+        rows = []
+        for per_cu, threads in ((1, 256), (1, 512), (2, 512), (4, 256)):
+            grid = (C.c_uint * 3)(cus * per_cu, 1, 1)
+            block = (C.c_uint * 3)(threads, 1, 1)
+            t = _time(stream, lambda: ck(L.skl_kernel_launch(k, grid, block, 0, args, st)), reps=10)
+            moved = cus * per_cu * threads * iters.value * 8 * width
+            rows.append({"work_groups_per_cu": per_cu, "threads": threads, "ms": round(t, 4),
+                         "bytes_per_ns_per_cu": round(moved / (t * 1e6) / cus, 1)})
+        out[name] = {"sweep": rows,
+                     "best_bytes_per_ns_per_cu": max(r["bytes_per_ns_per_cu"] for r in rows)}
+        progress(f"{name} done")
+    ck(L.skl_program_release(prog))
+    return out
+
+
 def run_ceiling(skelcl, L, ck, stream):
     """Read-only, write-only, copy and two-read-one-write (the saxpy pattern)
     streams over 1 GiB per array at a sweep of grid sizes (work-groups of 256
@@ -670,6 +738,7 @@ def main() -> int:
     st = stream.handle
     if args.ceiling:
         print(json.dumps({"ceiling": run_ceiling(skelcl, L, ck, stream),
+                          "lds_ceiling": run_lds_ceiling(skelcl, L, ck, stream),
                           "device": {"name": info["name"], "arch": info["arch"]}}), flush=True)
         return 0
     if args.only:
