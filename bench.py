@@ -243,6 +243,72 @@ extern "C" __global__ __launch_bounds__(512) void lds_read_b64(unsigned* sink, i
 """
 
 
+# FP32 FMA issue probes: 16 independent accumulators per lane, fused
+# multiply-adds whose operands are all vector registers, or whose multiplier
+# is wave-uniform (a scalar register), in a loop the compiler cannot fold.
+VALU_SRC = r"""
+extern "C" __global__ __launch_bounds__(256) void fma_vv(float* sink, int iters, float m) {
+  float acc[16], x = (float)threadIdx.x * 1e-3f, y = m + (float)(threadIdx.x & 7) * 1e-4f;
+#pragma unroll
+  for (int i = 0; i < 16; ++i) acc[i] = x + (float)i;
+  for (int it = 0; it < iters; ++it) {
+#pragma unroll
+    for (int r = 0; r < 8; ++r)
+#pragma unroll
+      for (int i = 0; i < 16; ++i) acc[i] = __builtin_fmaf(acc[i], y, x);
+  }
+  float t = 0.0f;
+#pragma unroll
+  for (int i = 0; i < 16; ++i) t += acc[i];
+  if (t == 1234.5f) sink[0] = t;
+}
+extern "C" __global__ __launch_bounds__(256) void fma_sv(float* sink, int iters, float m) {
+  float acc[16], x = (float)threadIdx.x * 1e-3f;
+#pragma unroll
+  for (int i = 0; i < 16; ++i) acc[i] = x + (float)i;
+  for (int it = 0; it < iters; ++it) {
+#pragma unroll
+    for (int r = 0; r < 8; ++r)
+#pragma unroll
+      for (int i = 0; i < 16; ++i) acc[i] = __builtin_fmaf(acc[i], m, x);
+  }
+  float t = 0.0f;
+#pragma unroll
+  for (int i = 0; i < 16; ++i) t += acc[i];
+  if (t == 1234.5f) sink[0] = t;
+}
+"""
+
+
+def run_valu_ceiling(skelcl, L, ck, stream):
+    """FP32 FMA rate with vector-only operands and with a scalar multiplier."""
+    import ctypes as C
+    st = stream.handle
+    prog, log = C.c_void_p(), C.c_void_p()
+    ck(L.skl_program_build(VALU_SRC.encode(), b"-ffp-contract=off", C.byref(prog), C.byref(log)))
+    skelcl._lib.take_string(log)
+    cus = skelcl.device_info()["compute_units"]
+    sink = skelcl.DeviceBuffer(256)
+    sp, iters, m = C.c_void_p(sink.ptr.value), C.c_int(2048), C.c_float(0.999)
+    args = (C.c_void_p * 3)(*[C.cast(C.pointer(v), C.c_void_p).value for v in (sp, iters, m)])
+    out = {}
+    for name in ("fma_vv", "fma_sv"):
+        k = C.c_void_p()
+        ck(L.skl_kernel_get(prog, name.encode(), C.byref(k)))
+        rows = []
+        for per_cu in (4, 8, 16):
+            grid = (C.c_uint * 3)(cus * per_cu, 1, 1)
+            block = (C.c_uint * 3)(256, 1, 1)
+            t = _time(stream, lambda: ck(L.skl_kernel_launch(k, grid, block, 0, args, st)), reps=10)
+            flop = 2.0 * cus * per_cu * 256 * iters.value * 8 * 16
+            rows.append({"work_groups_per_cu": per_cu, "ms": round(t, 4),
+                         "TFLOPs": round(flop / (t * 1e-3) / 1e12, 1)})
+        out[name] = {"sweep": rows, "best_TFLOPs": max(r["TFLOPs"] for r in rows)}
+        progress(f"{name} done")
+    ck(L.skl_program_release(prog))
+    return out
+
+
 def run_lds_ceiling(skelcl, L, ck, stream):
     """LDS read bandwidth per CU at a few work-group counts and sizes."""
     import ctypes as C
@@ -739,6 +805,7 @@ def main() -> int:
     if args.ceiling:
         print(json.dumps({"ceiling": run_ceiling(skelcl, L, ck, stream),
                           "lds_ceiling": run_lds_ceiling(skelcl, L, ck, stream),
+                          "valu_ceiling": run_valu_ceiling(skelcl, L, ck, stream),
                           "device": {"name": info["name"], "arch": info["arch"]}}), flush=True)
         return 0
     if args.only:
