@@ -11,6 +11,9 @@
 #include "skelcl_rt.h"
 #include "ld_util.h"
 
+extern "C" int skl_internal_stencil_adjoint_zero(float* din, const float* g, const float* w,
+                                                 int rows, int cols, int R, hipStream_t s);
+
 namespace {
 
 template <typename T>
@@ -99,67 +102,6 @@ __global__ __launch_bounds__(256) void stencil_transpose_kernel(
   }
 }
 
-// Compile-time R (1..5; at 6 the 169 weights no longer stay in registers): a
-// 64x4 work-group computes a 64x64 output tile, 16
-// rows of one column per thread.  The (2R+1)^2 weights sit in VGPRs (read
-// from LDS once, so they never become spilled SGPRs); each tile row is read
-// once per thread into 2R+1 registers and feeds up to 2R+1 outputs.  Tile
-// rows go in descending order, so every output takes dy ascending and,
-// within it, dx ascending: the generic kernel's order of FMAs, bit for bit.
-// Every pixel gets the interior (zero outside) formula; under NEAREST the
-// border kernel below then rewrites the pixels within R of the border.
-constexpr int FTW = 64, FRB = 16, FTH = 4 * FRB;
-template <int R>
-__global__ __launch_bounds__(256) void stencil_transpose_fast(float* __restrict__ din,
-                                                              const float* __restrict__ g,
-                                                              const float* __restrict__ w,
-                                                              int rows, int cols)
-{
-  constexpr int W = 2 * R + 1, TWP = FTW + 2 * R, THP = FTH + 2 * R;
-  __shared__ float tile[THP * TWP];  // zero outside the matrix
-  __shared__ float ws[W * W];
-  for (int i = threadIdx.x; i < W * W; i += 256) ws[i] = w[i];
-  const int c0 = blockIdx.x * FTW, r0 = blockIdx.y * FTH;
-  for (int idx = threadIdx.x; idx < THP * TWP; idx += 256) {
-    const int tr = idx / TWP, tc = idx - tr * TWP;
-    const int gr = r0 - R + tr, gc = c0 - R + tc;
-    tile[idx] = ld_or(g, (size_t)gr * cols + gc, 0, gr >= 0 && gr < rows && gc >= 0 && gc < cols,
-                      0.0f);
-  }
-  __syncthreads();
-  float wr[W * W];
-#pragma unroll
-  for (int i = 0; i < W * W; ++i) wr[i] = ws[i];
-  const int tx = threadIdx.x & 63, lr0 = (threadIdx.x >> 6) * FRB;
-  float acc[FRB];
-#pragma unroll
-  for (int i = 0; i < FRB; ++i) acc[i] = 0.0f;
-  // output i (row lr0 + i) reads tile row lr0 + i + R - dy = lr0 + t
-#pragma unroll
-  for (int t = FRB + 2 * R - 1; t >= 0; --t) {
-    float v[W];  // v[j]: column tx + R - dx, dx = j - R
-#pragma unroll
-    for (int j = 0; j < W; ++j) v[j] = tile[(lr0 + t) * TWP + tx + 2 * R - j];
-#pragma unroll
-    for (int i = 0; i < FRB; ++i) {
-      const int dy = i + R - t;
-      if (dy < -R || dy > R) continue;
-#pragma unroll
-      for (int j = 0; j < W; ++j) acc[i] = fmaf(wr[(dy + R) * W + j], v[j], acc[i]);
-    }
-    // the accumulators are materialised here: otherwise each output's FMAs
-    // sink into its guarded store at the end and every tile row stays live
-#pragma unroll
-    for (int i = 0; i < FRB; ++i) asm volatile("" : "+v"(acc[i]));
-  }
-  const int q = c0 + tx;
-#pragma unroll
-  for (int i = 0; i < FRB; ++i) {
-    const int p = r0 + lr0 + i;
-    if (p < rows && q < cols) din[(size_t)p * cols + q] = acc[i];
-  }
-}
-
 // NEAREST padding: rewrites the pixels within R of the border (one thread
 // each) after the interior formula has written every pixel.  Bands: top
 // rows, bottom rows, then the left and right columns of the rows between.
@@ -222,15 +164,11 @@ extern "C" int skl_internal_stencil_transpose(float* din, const float* dout, con
 {
   if (R < 0 || R > 32) return SKL_ERR_INVALID;
   if (R >= 1 && R <= 5) {
-    dim3 fg((cols + FTW - 1) / FTW, (rows + FTH - 1) / FTH);
-    if (fg.y > 65535) return SKL_ERR_INVALID;
-    switch (R) {
-      case 1: stencil_transpose_fast<1><<<fg, 256, 0, s>>>(din, dout, w, rows, cols); break;
-      case 2: stencil_transpose_fast<2><<<fg, 256, 0, s>>>(din, dout, w, rows, cols); break;
-      case 3: stencil_transpose_fast<3><<<fg, 256, 0, s>>>(din, dout, w, rows, cols); break;
-      case 4: stencil_transpose_fast<4><<<fg, 256, 0, s>>>(din, dout, w, rows, cols); break;
-      default: stencil_transpose_fast<5><<<fg, 256, 0, s>>>(din, dout, w, rows, cols); break;
-    }
+    // every pixel by the zero-padding formula on the register-tiled stencil
+    // kernel (native/stencil.hip, adjoint mode: the same terms in the same
+    // order as the one-output kernel below); NEAREST then rewrites the border
+    const int st = skl_internal_stencil_adjoint_zero(din, dout, w, rows, cols, R, s);
+    if (st != SKL_OK) return st;
     if (nearest) {
       const long long top = R < rows ? R : rows;
       const long long bstart = rows - R > top ? rows - R : top;

@@ -187,7 +187,7 @@ __global__ __launch_bounds__(256) void stencil_forward_kernel(const float* __res
 // one-column kernel and every weight row loaded feeds two columns.  The
 // weights enter by scalar loads and are moved to vector registers.  Every
 // output still takes its terms in the loop's order.
-template <int R, int RB, bool NEUTRAL, bool FUSED>
+template <int R, int RB, bool NEUTRAL, bool FUSED, bool ADJ = false>
 __global__ __launch_bounds__(256) void stencil_forward2_kernel(const float* __restrict__ in,
                                                                float* __restrict__ out,
                                                                const float* __restrict__ w,
@@ -264,10 +264,13 @@ __global__ __launch_bounds__(256) void stencil_forward2_kernel(const float* __re
   float wr[W][W];
   const char* win = (const char*)(tile + (wv * RB) * TWP + 2 * lane);
   constexpr int LAG = SKL_STENCIL_LAG;
+  constexpr int NSTEP = RB + 2 * R;
+  // the tile row read at step t: ascending, or descending for the adjoint
+  auto trow_of = [](int t) { return ADJ ? (t < NSTEP ? NSTEP - 1 - t : 0) : t; };
   unsigned zt[LAG + 1], zw[LAG + 1];
 #pragma unroll
   for (int q = 0; q < LAG; ++q) {
-    zt[q] = q * TWP * 4;
+    zt[q] = trow_of(q) * TWP * 4;
     zw[q] = q * W * 4;
     asm volatile("" : "+s"(zt[q]), "+s"(zw[q]));
   }
@@ -301,13 +304,15 @@ __global__ __launch_bounds__(256) void stencil_forward2_kernel(const float* __re
       if (k < 0 || k >= W) continue;
 #pragma unroll
       for (int j = 0; j < W; ++j) {
-        acc[i][0] = FUSED ? __builtin_fmaf(v[j], wr[k][j], acc[i][0]) : acc[i][0] + v[j] * wr[k][j];
-        acc[i][1] = FUSED ? __builtin_fmaf(v[j + 1], wr[k][j], acc[i][1])
-                          : acc[i][1] + v[j + 1] * wr[k][j];
+        // forward: column c + j of the window; adjoint: c + 2R - j
+        const float a0 = ADJ ? v[2 * R - j] : v[j];
+        const float a1 = ADJ ? v[2 * R + 1 - j] : v[j + 1];
+        acc[i][0] = FUSED ? __builtin_fmaf(a0, wr[k][j], acc[i][0]) : acc[i][0] + a0 * wr[k][j];
+        acc[i][1] = FUSED ? __builtin_fmaf(a1, wr[k][j], acc[i][1]) : acc[i][1] + a1 * wr[k][j];
       }
     }
     const int n = (t + LAG) % (LAG + 1);
-    zt[n] = (t + LAG) * TWP * 4;
+    zt[n] = trow_of(t + LAG) * TWP * 4;
     zw[n] = (t + LAG) * W * 4;
 #pragma unroll
     for (int i = 0; i < RB; ++i)
@@ -320,7 +325,7 @@ __global__ __launch_bounds__(256) void stencil_forward2_kernel(const float* __re
     // 8-byte aligned
 #pragma unroll
     for (int i = 0; i < RB; ++i) {
-      const int r = r0 + wv * RB + i;
+      const int r = r0 + wv * RB + (ADJ ? RB - 1 - i : i);
       if (r < rows) {
         unsigned long long rb = (unsigned long long)(out + (size_t)r * (size_t)cols + c0);
         asm volatile("" : "+s"(rb));
@@ -331,7 +336,7 @@ __global__ __launch_bounds__(256) void stencil_forward2_kernel(const float* __re
   } else if (cl < cols) {
 #pragma unroll
     for (int i = 0; i < RB; ++i) {
-      const int r = r0 + wv * RB + i;
+      const int r = r0 + wv * RB + (ADJ ? RB - 1 - i : i);
       if (r < rows) {
         float* row = out + (size_t)r * (size_t)cols + c0;
         store_row(row, o8, acc[i][0]);
@@ -409,6 +414,27 @@ hipError_t launch_r(float* out, const float* in, const float* w, int rows, int c
 }
 
 }  // namespace
+
+// The transposed stencil of g under zero padding for R in 1..5 (every
+// pixel; a NEAREST caller rewrites the border afterwards): 8 rows x 2
+// columns per thread.
+extern "C" int skl_internal_stencil_adjoint_zero(float* din, const float* g, const float* w,
+                                                 int rows, int cols, int R, hipStream_t s)
+{
+  if (rows <= 0 || cols <= 0) return rows < 0 || cols < 0 ? SKL_ERR_INVALID : SKL_OK;
+  constexpr int RB = 8;
+  dim3 grid((cols + 127) / 128, (rows + 4 * RB - 1) / (4 * RB)), block(256);
+  if (grid.y > 65535) return SKL_ERR_INVALID;
+  switch (R) {
+    case 1: hipLaunchKernelGGL((stencil_forward2_kernel<1, RB, true, true, true>), grid, block, 0, s, g, din, w, rows, cols, 0.0f); break;
+    case 2: hipLaunchKernelGGL((stencil_forward2_kernel<2, RB, true, true, true>), grid, block, 0, s, g, din, w, rows, cols, 0.0f); break;
+    case 3: hipLaunchKernelGGL((stencil_forward2_kernel<3, RB, true, true, true>), grid, block, 0, s, g, din, w, rows, cols, 0.0f); break;
+    case 4: hipLaunchKernelGGL((stencil_forward2_kernel<4, RB, true, true, true>), grid, block, 0, s, g, din, w, rows, cols, 0.0f); break;
+    case 5: hipLaunchKernelGGL((stencil_forward2_kernel<5, RB, true, true, true>), grid, block, 0, s, g, din, w, rows, cols, 0.0f); break;
+    default: return SKL_ERR_UNSUPPORTED;
+  }
+  return hipGetLastError() == hipSuccess ? SKL_OK : SKL_ERR_HIP;
+}
 
 // Largest radius the stencil kernel is instantiated for.
 extern "C" int skl_internal_stencil_max_radius() { return 5; }
