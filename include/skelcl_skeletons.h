@@ -234,14 +234,15 @@ int skl_zip_reduce_run(skl_skeleton zip, skl_skeleton reduce, void* out,
                        const void* left, const void* right, size_t n,
                        skl_stream stream);
 
-/* Batched segmented Reduce / Scan (float (+) only): nseg segments of the n
- * elements of `in`, described by device offsets[0..nseg] (int64, segment s
- * is offsets[s]..offsets[s+1]), processed by small prologue launches and one
- * main launch (no host synchronisation).  Reduce writes out[s] (nseg floats;
- * an empty segment gives 0); Scan writes out[offsets[s] + k] (n floats;
- * elements outside [offsets[0], offsets[nseg]) are left as they are).
- * Offsets are validated on the device (non-decreasing, inside [0, n]);
- * invalid offsets make Reduce write NaN to every out[s] and Scan write
+/* Batched segmented Reduce / Scan ((+) on float, or on int32 summed exactly
+ * modulo 2^32): nseg segments of the n elements of `in`, described by device
+ * offsets[0..nseg] (int64, segment s is offsets[s]..offsets[s+1]), processed
+ * by small prologue launches and one main launch (no host
+ * synchronisation).  Reduce writes out[s] (nseg elements; an empty segment
+ * gives 0); Scan writes out[offsets[s] + k] (n elements; elements outside
+ * [offsets[0], offsets[nseg]) are left as they are).  Offsets are validated
+ * on the device (non-decreasing, inside [0, n]); invalid offsets make Reduce
+ * write NaN (float) or INT_MIN (int32) to every out[s] and Scan write
  * nothing.  No counterpart in the reference. */
 int skl_reduce_segmented_run(skl_skeleton s, void* out, const void* in, size_t n,
                              const int64_t* offsets, int nseg,

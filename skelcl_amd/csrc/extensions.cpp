@@ -22,11 +22,11 @@ extern "C" int skl_internal_stencil_transpose(float* din, const float* dout, con
                                               int rows, int cols, int R, int nearest,
                                               hipStream_t s);
 extern "C" int skl_internal_segmented_sum(float* out, const float* in, unsigned long long n,
-                                          const int64_t* offsets, int nseg, int fp64, void* ws,
+                                          const int64_t* offsets, int nseg, int form, void* ws,
                                           hipStream_t s);
 extern "C" int skl_internal_segmented_scan(float* out, const float* in, unsigned long long n,
                                            const int64_t* offsets, int nseg, int inclusive,
-                                           int fp64, void* ws, hipStream_t s);
+                                           int form, void* ws, hipStream_t s);
 extern "C" size_t skl_internal_segmented_workspace(unsigned long long n, int nseg);
 extern "C" int skl_internal_broadcast(void* dst, const void* src, size_t elem, size_t count,
                                       hipStream_t s);
@@ -329,16 +329,17 @@ extern "C" int skl_reduce_segmented_run(skl_skeleton s, void* out, const void* i
 {
   SKL_TRY(ensure_initialized());
   if (!s || s->kind != SKL_REDUCE) return fail(SKL_ERR_INVALID, "not a Reduce skeleton");
-  if (!(s->op == UserOp::Add && s->extras.empty() && s->types[0].name == "float"))
-    return fail(SKL_ERR_UNSUPPORTED, "segmented Reduce is available for float Reduce(+)");
+  const int ty = segment_add_type(s, SKL_REDUCE);
+  if (!ty) return fail(SKL_ERR_UNSUPPORTED, "segmented Reduce is available for float or int Reduce(+)");
   if (nseg < 0) return fail(SKL_ERR_INVALID, "negative segment count");
   if (nseg == 0) return SKL_OK;
   if (!out || !offsets || (n && !in)) return fail(SKL_ERR_INVALID, "NULL buffer");
   SKL_TRY(seg_scratch(s, skl_internal_segmented_workspace(n, nseg)));
-  // precision FP64: FP64 chunk sums; otherwise compensated FP32 in a chunk
+  // int: exact modulo 2^32; float precision FP64: FP64 chunk sums;
+  // otherwise compensated FP32 in a chunk
   return skl_internal_segmented_sum((float*)out, (const float*)in, n, offsets, nseg,
-                                    s->precision == SKL_PREC_FP64 ? 1 : 0, s->seg_scratch,
-                                    resolve_stream(stream));
+                                    ty == 2 ? 2 : s->precision == SKL_PREC_FP64 ? 1 : 0,
+                                    s->seg_scratch, resolve_stream(stream));
 }
 
 extern "C" int skl_scan_segmented_run(skl_skeleton s, void* out, const void* in, size_t n,
@@ -347,14 +348,14 @@ extern "C" int skl_scan_segmented_run(skl_skeleton s, void* out, const void* in,
 {
   SKL_TRY(ensure_initialized());
   if (!s || s->kind != SKL_SCAN) return fail(SKL_ERR_INVALID, "not a Scan skeleton");
-  if (!(s->op == UserOp::Add && s->extras.empty() && s->types[0].name == "float"))
-    return fail(SKL_ERR_UNSUPPORTED, "segmented Scan is available for float Scan(+)");
+  const int ty = segment_add_type(s, SKL_SCAN);
+  if (!ty) return fail(SKL_ERR_UNSUPPORTED, "segmented Scan is available for float or int Scan(+)");
   if (nseg < 0) return fail(SKL_ERR_INVALID, "negative segment count");
   if (nseg == 0 || n == 0) return SKL_OK;
   if (!out || !in || !offsets) return fail(SKL_ERR_INVALID, "NULL buffer");
   SKL_TRY(seg_scratch(s, skl_internal_segmented_workspace(n, nseg)));
   return skl_internal_segmented_scan((float*)out, (const float*)in, n, offsets, nseg,
-                                     inclusive, s->precision == SKL_PREC_FP64 ? 1 : 0,
+                                     inclusive, ty == 2 ? 2 : s->precision == SKL_PREC_FP64 ? 1 : 0,
                                      s->seg_scratch, resolve_stream(stream));
 }
 

@@ -1,4 +1,4 @@
-// Batched segmented Reduce(+) / Scan(+) on float (beyond the reference):
+// Batched segmented Reduce(+) / Scan(+) on float or int32 (beyond the reference):
 // many uneven segments, given by device offsets, in one main launch after a
 // small prologue launch.
 //
@@ -14,8 +14,12 @@
 // the offsets and records each tile's first segment.  Every output is
 // float(carry + local prefix), the look-back carries in FP64.
 // Inside a chunk / tile the sums are compensated FP32 (double-float pairs,
-// acc_df), or FP64 throughout when the caller asks for FP64.
-// Invalid offsets set an error flag: no chunk or tile runs, Reduce writes NaN.
+// acc_df), or FP64 throughout when the caller asks for FP64.  int32 data
+// (the same bits passed as float) sums exactly modulo 2^32 (acc_u32): chunk
+// partials are held in FP64 as their unsigned 32-bit values, and the
+// combine adds them modulo 2^32.
+// Invalid offsets set an error flag: no chunk or tile runs, Reduce writes NaN
+// (float) or INT_MIN (int32).
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -120,6 +124,18 @@ __device__ __forceinline__ double block_sum(A v, A* red)
     for (int i = 0; i < WG / 64; ++i) t = acc_add(t, red[i]);
   return acc_to_double(t);
 }
+template <>
+__device__ __forceinline__ double block_sum<acc_u32>(acc_u32 v, acc_u32* red)
+{
+  for (int o = 32; o >= 1; o >>= 1) v.v += (unsigned)__shfl_down((int)v.v, o, 64);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (lane == 0) red[w] = v;
+  __syncthreads();
+  unsigned t = 0;
+  if (threadIdx.x == 0)
+    for (int i = 0; i < WG / 64; ++i) t += red[i].v;
+  return (double)t;  // exact: t < 2^32
+}
 
 // one work-group per chunk: sums in A (4 accumulators per thread, fixed
 // order), a full chunk read as 16 float4 packs per thread all in flight
@@ -172,7 +188,9 @@ __global__ __launch_bounds__(WG) void seg_sum_kernel(const float* __restrict__ i
 }
 
 // one wave per segment: its partials in a fixed order (lane-strided FP64
-// sums, then a fixed shuffle tree)
+// sums, then a fixed shuffle tree); INT: the partials are unsigned 32-bit
+// values, added modulo 2^32
+template <bool INT>
 __global__ __launch_bounds__(WG) void seg_combine_kernel(const long long* __restrict__ cstart,
                                                          int nseg,
                                                          const double* __restrict__ partials,
@@ -183,10 +201,17 @@ __global__ __launch_bounds__(WG) void seg_combine_kernel(const long long* __rest
   const int lane = threadIdx.x & 63;
   if (s >= nseg) return;
   if (ctl->error) {
-    if (lane == 0) out[s] = __builtin_nanf("");
+    if (lane == 0) out[s] = INT ? __uint_as_float(0x80000000u) : __builtin_nanf("");
     return;
   }
   const long long c0 = cstart[s], c1 = cstart[s + 1];
+  if (INT) {
+    unsigned u = 0;
+    for (long long c = c0 + lane; c < c1; c += 64) u += (unsigned)partials[c];
+    for (int o = 32; o >= 1; o >>= 1) u += (unsigned)__shfl_down((int)u, o, 64);
+    if (lane == 0) out[s] = __uint_as_float(u);
+    return;
+  }
   double v = 0.0;
   for (long long c = c0 + lane; c < c1; c += 64) v += partials[c];
   for (int o = 32; o >= 1; o >>= 1) v += __shfl_down(v, o, 64);
@@ -320,8 +345,9 @@ extern "C" size_t skl_internal_segmented_workspace(unsigned long long n, int nse
   return sum > scan ? sum : scan;
 }
 
+// form: 0 compensated FP32, 1 FP64, 2 int32 (exact modulo 2^32)
 extern "C" int skl_internal_segmented_sum(float* out, const float* in, unsigned long long n,
-                                          const int64_t* offsets, int nseg, int fp64, void* ws,
+                                          const int64_t* offsets, int nseg, int form, void* ws,
                                           hipStream_t s)
 {
   const size_t cap = max_chunks(n, nseg, SUM_CH);
@@ -333,18 +359,23 @@ extern "C" int skl_internal_segmented_sum(float* out, const float* in, unsigned 
   double* partials = reinterpret_cast<double*>(reinterpret_cast<char*>(cseg) + pad256(cap * 4));
   seg_sum_prologue<<<1, 1024, 0, s>>>(offsets, nseg, n, (long long)cap, cstart, ctl);
   seg_sum_fill<<<(unsigned)((nseg + 3) / 4), 256, 0, s>>>(cstart, nseg, cseg, ctl);
-  if (fp64)
+  if (form == 2)
+    seg_sum_kernel<acc_u32><<<(unsigned)cap, WG, 0, s>>>(in, offsets, cstart, cseg, partials, ctl);
+  else if (form == 1)
     seg_sum_kernel<double><<<(unsigned)cap, WG, 0, s>>>(in, offsets, cstart, cseg, partials, ctl);
   else
     seg_sum_kernel<acc_df><<<(unsigned)cap, WG, 0, s>>>(in, offsets, cstart, cseg, partials, ctl);
-  seg_combine_kernel<<<(unsigned)((nseg + WG / 64 - 1) / (WG / 64)), WG, 0, s>>>(
-      cstart, nseg, partials, out, ctl);
+  const unsigned cg = (unsigned)((nseg + WG / 64 - 1) / (WG / 64));
+  if (form == 2)
+    seg_combine_kernel<true><<<cg, WG, 0, s>>>(cstart, nseg, partials, out, ctl);
+  else
+    seg_combine_kernel<false><<<cg, WG, 0, s>>>(cstart, nseg, partials, out, ctl);
   return hipGetLastError() == hipSuccess ? SKL_OK : SKL_ERR_HIP;
 }
 
 extern "C" int skl_internal_segmented_scan(float* out, const float* in, unsigned long long n,
                                            const int64_t* offsets, int nseg, int inclusive,
-                                           int fp64, void* ws, hipStream_t s)
+                                           int form, void* ws, hipStream_t s)
 {
   const long long ntiles = flat_tiles(n);
   if (ntiles > 0x7FFFFFFE) return SKL_ERR_INVALID;
@@ -356,7 +387,10 @@ extern "C" int skl_internal_segmented_scan(float* out, const float* in, unsigned
       reinterpret_cast<unsigned long long*>(p + 512 + flat_table_bytes(ntiles));
   seg_flat_prologue<<<(unsigned)((nseg + 2 + 255) / 256), 256, 0, s>>>(offsets, nseg, n, ntiles,
                                                                       tfirst, ticket, err);
-  if (fp64)
+  if (form == 2)
+    seg_flat_scan_kernel<acc_u32><<<(unsigned)ntiles, FWG, 0, s>>>(
+        in, out, n, offsets, nseg, tfirst, ticket, pub, inclusive ? 1 : 0, err);
+  else if (form == 1)
     seg_flat_scan_kernel<double><<<(unsigned)ntiles, FWG, 0, s>>>(
         in, out, n, offsets, nseg, tfirst, ticket, pub, inclusive ? 1 : 0, err);
   else

@@ -78,7 +78,8 @@ public:
   }
 
   // extension: batched segmented Reduce(+) -- out[s] folds
-  // in[offsets[s] .. offsets[s+1]) -- all segments in one persistent launch
+  // in[offsets[s] .. offsets[s+1]) -- all segments in one main launch after
+  // small prologue launches (skelcl::batch does the sums and the scan in one)
   Vector<T> segmented(const Vector<T>& in, const Vector<long>& offsets)
   {
     Vector<T> o;

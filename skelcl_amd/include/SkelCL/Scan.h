@@ -60,7 +60,7 @@ public:
   }
 
   // extension: batched segmented Scan(+), every segment on its own, one
-  // persistent launch
+  // main launch after a small prologue launch
   Vector<T> segmented(const Vector<T>& in, const Vector<long>& offsets, bool inclusive = false)
   {
     Vector<T> o;

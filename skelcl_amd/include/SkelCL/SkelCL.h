@@ -35,6 +35,7 @@
 #include "Scan.h"
 #include "MapOverlap.h"
 #include "AllPairs.h"
+#include "Batch.h"
 
 namespace skelcl {
 

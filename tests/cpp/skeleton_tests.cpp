@@ -5,6 +5,7 @@
 #include <SkelCL/Autodiff.h>
 #include <SkelCL/SkelCL.h>
 
+#include <cmath>
 #include <cstdlib>
 #include <fstream>
 #include <vector>
@@ -587,6 +588,55 @@ TEST(AutodiffTest, FunctionPairs)
   EXPECT_EQ(6.0f, dimg[3][4]);
   EXPECT_EQ(9.0f, dimg[4][4]);
   EXPECT_EQ(0.0f, dimg[0][0]);
+}
+
+TEST(SegmentedTest, ReduceScanAndOneLaunchBatch)  // extension, no reference test
+{
+  // uneven segments with an empty one and a tail outside the last segment
+  const long offs[] = {0, 3, 3, 1000, 1001, 40000};
+  const size_t n = 40007;
+  skelcl::Vector<long> offsets(offs, offs + 6);
+  skelcl::Vector<int> in(n);
+  for (size_t i = 0; i < n; ++i) in[i] = (int)(i % 7) - 3;
+  skelcl::Reduce<int(int)> r("int func(int x, int y){ return x+y; }");
+  skelcl::Scan<int(int)> sc("int func(int x, int y){ return x+y; }", "0");
+  skelcl::Vector<int> sums = r.segmented(in, offsets);
+  skelcl::Vector<int> scan = sc.segmented(in, offsets, false);
+  auto b = skelcl::batch(r, sc, in, offsets, true);
+  for (int s = 0; s < 5; ++s) {
+    int acc = 0;
+    for (long i = offs[s]; i < offs[s + 1]; ++i) {
+      EXPECT_EQ(acc, scan[i]);
+      acc += in[i];
+      EXPECT_EQ(acc, b.scan[i]);
+    }
+    EXPECT_EQ(acc, sums[s]);
+    EXPECT_EQ(acc, b.sums[s]);
+  }
+  for (size_t i = 40000; i < n; ++i) EXPECT_EQ(0, b.scan[i]);
+
+  // float: both forms against a double-precision host scan (compensated
+  // FP32 inside a tile: far below 1e-5 of the running absolute sum)
+  skelcl::Vector<float> fin(n);
+  for (size_t i = 0; i < n; ++i) fin[i] = 0.001f * (float)((i * 37) % 1001) - 0.5f;
+  skelcl::Reduce<float(float)> fr("float func(float x, float y){ return x+y; }");
+  skelcl::Scan<float(float)> fs("float func(float x, float y){ return x+y; }", "0.0f");
+  skelcl::Vector<float> fsums = fr.segmented(fin, offsets);
+  skelcl::Vector<float> fscan = fs.segmented(fin, offsets, true);
+  auto fb = skelcl::batch(fr, fs, fin, offsets, true);
+  for (int s = 0; s < 5; ++s) {
+    double acc = 0.0, mag = 0.0;
+    bool ok = true;
+    for (long i = offs[s]; i < offs[s + 1]; ++i) {
+      acc += fin[i];
+      mag += std::fabs((double)fin[i]);
+      ok = ok && std::fabs(fscan[i] - acc) <= 1e-5 * mag + 1e-30 &&
+           std::fabs(fb.scan[i] - acc) <= 1e-5 * mag + 1e-30;
+    }
+    EXPECT_TRUE(ok);
+    EXPECT_TRUE(std::fabs(fsums[s] - acc) <= 1e-5 * mag + 1e-30);
+    EXPECT_TRUE(std::fabs(fb.sums[s] - acc) <= 1e-5 * mag + 1e-30);
+  }
 }
 
 int main(int argc, char** argv)

@@ -239,6 +239,51 @@ def test_segmented_invalid_offsets_flagged_on_device(skl):
         assert np.all(np.abs(got - ref) <= np.spacing(np.abs(ref).astype(np.float32)) + 1e-9)
 
 
+@pytest.mark.parametrize("case", ["wrap_long", "dense_tiny", "inner_range"])
+def test_segmented_int_exact(skl, case):
+    """int Reduce(+) / Scan(+) segmented (exact modulo 2^32) against the
+    oracle, bit for bit: values whose sums wrap over segments of several
+    Reduce chunks (16384) and Scan tiles (32768), runs of empty and
+    one-element segments, and an uncovered head and tail (left as they
+    are); invalid offsets make Reduce write INT_MIN."""
+    from skelcl_amd import _lib
+    rng = np.random.default_rng({"wrap_long": 21, "dense_tiny": 22, "inner_range": 23}[case])
+    if case == "wrap_long":
+        off = np.array([0, 3, 3, 300000, 300001, 1_000_000], np.int64)
+        n = 1_000_000
+        x = oracle.gen_u64(0x5201, n).astype(np.uint32).view(np.int32)
+    elif case == "dense_tiny":
+        lens = rng.integers(0, 4, 30000)
+        off = np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)
+        n = int(off[-1]) + 11
+        x = oracle.gen_i32_below(0x5202, n, 1000)
+    else:
+        off = np.array([1000, 1000, 20000, 20001, 60000, 100005], np.int64)
+        n = 100100
+        x = oracle.gen_i32_below(0x5203, n, 100000)
+    nseg = len(off) - 1
+    r = skl.Reduce("int(int)", "int func(int x, int y){ return x+y; }")
+    s = skl.Scan("int(int)", "int func(int x, int y){ return x+y; }", "0")
+    X = skl.DeviceBuffer.from_array(x)
+    O = skl.DeviceBuffer.from_array(off)
+    o = skl.DeviceBuffer.from_array(np.full(nseg, 0x5EED, np.int32))
+    _lib.check(_lib.lib().skl_reduce_segmented_run(r.handle, o.ptr, X.ptr, n, O.ptr, nseg, None))
+    assert np.array_equal(o.to_numpy(np.uint32, nseg), oracle.segmented_sum_u32(x, off))
+    for incl in (0, 1):
+        so = skl.DeviceBuffer.from_array(np.full(n, 0x5EED, np.int32))
+        _lib.check(_lib.lib().skl_scan_segmented_run(s.handle, so.ptr, X.ptr, n, O.ptr, nseg,
+                                                     incl, None))
+        assert np.array_equal(so.to_numpy(np.uint32, n),
+                              oracle.segmented_scan_u32(x, off, bool(incl), fill=0x5EED)), incl
+    # the Python mirror over the same skeletons
+    got = np.asarray(r.segmented(skl.Vector(x), off)).astype(np.int32).view(np.uint32)
+    assert np.array_equal(got, oracle.segmented_sum_u32(x, off))
+    bad = skl.DeviceBuffer.from_array(np.array([0, 40, 20, n], np.int64))
+    o = skl.DeviceBuffer.from_array(np.full(3, 7, np.int32))
+    _lib.check(_lib.lib().skl_reduce_segmented_run(r.handle, o.ptr, X.ptr, n, bad.ptr, 3, None))
+    assert np.all(o.to_numpy(np.int32, 3) == np.iinfo(np.int32).min)
+
+
 def _seg_scan_raw(skl, x, off, n, inclusive, fill=7.0):
     from skelcl_amd import _lib
     X = skl.DeviceBuffer.from_array(x)
