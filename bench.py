@@ -120,11 +120,30 @@ def measured_traffic(kernel: str):
     return None
 
 
-def _time(stream, fn, reps=20, warm=3):
-    """Median of `reps` HIP-event timings (ms) of fn() on `stream`."""
+# device time spent in back-to-back calls before each extras measurement
+SETTLE_MS = 100.0
+
+
+def _time(stream, fn, reps=20, warm=3, settle_ms=SETTLE_MS):
+    """Median of `reps` HIP-event timings (ms) of fn() on `stream`, after
+    `warm` calls and then back-to-back calls for at least `settle_ms` of
+    device time: under a sudden compute load the clocks dip for several
+    milliseconds before they settle (the 8192^2 stencil forward runs
+    0.24-0.29 ms during the first 20-40 launches and 0.21 ms afterwards), and
+    the figures here are for a device in steady use."""
     import skelcl_amd as skelcl
     for _ in range(warm):
         fn()
+    done, batch = 0.0, 1
+    while done < settle_ms:
+        a, b = skelcl.Event(), skelcl.Event()
+        a.record(stream)
+        for _ in range(batch):
+            fn()
+        b.record(stream)
+        b.synchronize()
+        done += a.elapsed_ms(b)
+        batch = min(batch * 2, 32)  # few launches queued at a time
     ts = []
     for _ in range(reps):
         a, b = skelcl.Event(), skelcl.Event()
@@ -432,7 +451,7 @@ def run_extras(skelcl, L, ck, stream, args):
         ck(L.skl_graph_begin_capture(st))
         dot_fused()
         ck(L.skl_graph_end_capture(st, C.byref(g)))
-        t_graph = _time(stream, lambda: ck(L.skl_graph_launch(g, st)), reps=20)
+        t_graph = _time(stream, lambda: ck(L.skl_graph_launch(g, st)), reps=20, settle_ms=0)
         got_graph = float(o1.to_numpy(np.float32, 1)[0])
         ck(L.skl_graph_destroy(g))
         t_int = _time(stream, lambda: ck(L.skl_reduce_run(addi.handle, o1.ptr, K1.ptr, n1, None,
@@ -694,7 +713,7 @@ def run_extras(skelcl, L, ck, stream, args):
             ck(L.skl_graph_begin_capture(st))
             pipeline(*argsp)
             ck(L.skl_graph_end_capture(st, C.byref(g)))
-            t_g = _time(stream, lambda: ck(L.skl_graph_launch(g, st)))
+            t_g = _time(stream, lambda: ck(L.skl_graph_launch(g, st)), settle_ms=0)
             same = bool(np.array_equal(sums.to_numpy(np.float32).view(np.uint32),
                                        ref_sums.view(np.uint32)))
             ck(L.skl_graph_destroy(g))
@@ -727,7 +746,7 @@ def run_extras(skelcl, L, ck, stream, args):
         ck(L.skl_graph_begin_capture(st))
         batch_one(wbest)
         ck(L.skl_graph_end_capture(st, C.byref(g)))
-        one["graph_ms"] = round(_time(stream, lambda: ck(L.skl_graph_launch(g, st))), 4)
+        one["graph_ms"] = round(_time(stream, lambda: ck(L.skl_graph_launch(g, st)), settle_ms=0), 4)
         one["graph_sums_bit_identical"] = bool(np.array_equal(
             sums_b.to_numpy(np.float32).view(np.uint32), eager_sums.view(np.uint32)))
         ck(L.skl_graph_destroy(g))

@@ -239,6 +239,17 @@ static int tune(skl_skeleton s, size_t n, const skl_arg* args, int nargs)
   // per cent of one another swap places from run to run, so the shape in
   // use stays unless another is at least 2% faster in pass 2.
   const Shape incumbent = current(s);
+  {
+    // back-to-back runs of the shape in use for at least 50 ms of device
+    // time first: under a sudden load the clocks dip for several ms before
+    // they settle, and the candidates are compared on a settled device
+    float done = 0.0f;
+    for (int i = 0, batch = 1; done < 50.0f && i < 256; ++i, batch = batch < 32 ? 2 * batch : 32) {
+      float ms = 0.0f;
+      SKL_TRY(time_shape(incumbent, batch, &ms));
+      done += ms * (float)batch;
+    }
+  }
   std::vector<std::pair<float, Shape>> scored;
   for (const Shape& c : cands) {
     float ms = 0;
