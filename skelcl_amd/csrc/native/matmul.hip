@@ -344,7 +344,8 @@ __global__ __launch_bounds__(256 * MB, 1) void gemm_bf16x3_kernel(
   }
   // no fill may still be writing LDS when the work-group ends
   __builtin_amdgcn_s_waitcnt(0);
-  // C fragment of a 16x16 tile: lane holds column lane%16, rows 4*(lane/16)+0..3
+  // C fragment of a 16x16 tile: lane holds column lane%16, rows 4*(lane/16)+0..3.
+  // Non-temporal stores, as in the FP8 product below
   const int cc = lane & 15, cr = (lane >> 4) * 4;
 #pragma unroll
   for (int i = 0; i < 4; ++i)
@@ -354,7 +355,8 @@ __global__ __launch_bounds__(256 * MB, 1) void gemm_bf16x3_kernel(
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int row = m0 + wm * 64 + i * 16 + cr + r;
-        if (row < M && col < N) C[(size_t)row * ldc + col] = acc0[i][j][r] + acc1[i][j][r];
+        if (row < M && col < N)
+          __builtin_nontemporal_store(acc0[i][j][r] + acc1[i][j][r], C + (size_t)row * ldc + col);
       }
     }
 }
@@ -651,6 +653,10 @@ __global__ __launch_bounds__(MB * NB * 128 * 128 / (WR * 16 * 64) * 64, 1) void 
   if (k < nkb) step(As0, Bs0, As1, Bs1, k);
   __builtin_amdgcn_s_waitcnt(0);  // no fill may still be writing LDS at the end
 
+  // Non-temporal stores of C: back to back, 8192^3 products took 0.418 ms
+  // against 0.458 ms with plain stores (the write-back of one product's C
+  // then no longer competes with the next one's operand reads); after the
+  // quantiser the gain is about 1%
   const int cc = lane & 15, cr = (lane >> 4) * 4;
 #pragma unroll
   for (int i = 0; i < WR; ++i)
@@ -660,7 +666,7 @@ __global__ __launch_bounds__(MB * NB * 128 * 128 / (WR * 16 * 64) * 64, 1) void 
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int row = m0 + arow0 + i * 16 + cr + r;
-        if (row < M && col < N) C[(size_t)row * ldc + col] = acc[i][j][r];
+        if (row < M && col < N) __builtin_nontemporal_store(acc[i][j][r], C + (size_t)row * ldc + col);
       }
     }
 }
