@@ -313,8 +313,8 @@ int skl_matmul_batched_f32(const skl_gemm_desc* descs_device, int count,
  * products (device descriptors, as for skl_matmul_batched_f32; at most
  * 2048; precision FP32 or BF16X3).  Any part may be left out (NULL output,
  * nseg or count 0).  One kernel launch, no prologue and no host
- * synchronisation: the segment part is one pass over tiles of 16384
- * elements (a decoupled look-back; the sums are the inclusive values at
+ * synchronisation: the segment part is one pass over tiles of 24576
+ * elements beside products, 32768 alone (a decoupled look-back; the sums are the inclusive values at
  * the segments' ends, so the array is read once for both), and
  * workers_per_cu (0 = 1, or 2) work-groups per CU walk the product tiles at
  * the same time.  Float Reduce and Scan must share a precision form

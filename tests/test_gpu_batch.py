@@ -15,7 +15,7 @@ import oracle
 
 pytestmark = pytest.mark.gpu
 
-TILE = 16384  # elements per segment tile of a batch launch with products
+TILE = 24576  # elements per segment tile of a batch launch with products
 TILE_SEG = 32768  # ... of a launch of segments alone
 ADD = "float func(float x, float y){ return x+y; }"
 
@@ -113,7 +113,7 @@ def test_batch_segments_and_products(skl, seg_prec, mm_prec, workers):
 @pytest.mark.parametrize("alone", [True, False])
 def test_batch_segment_layouts(skl, case, inclusive, alone):
     """Segment layouts against the tiles (32768 elements in a launch of
-    segments alone, 16384 beside products): uncovered ends (outputs left as
+    segments alone, 24576 beside products): uncovered ends (outputs left as
     they are), runs of empty and one-element segments (more segment ends per
     tile than the LDS slots hold: the rest are found by searching the
     offsets), heads on, before and after tile starts, a last tile cut short,
@@ -272,7 +272,7 @@ def test_batch_int_segments_exact(skl, case, with_products):
     """int Reduce(+) / Scan(+) segments (exact modulo 2^32) against the
     oracle, bit for bit: the batched workload's lengths (first 64), runs of
     empty and one-element segments, heads at the tile edges of both tile
-    sizes (16384 beside products, 32768 alone), and values whose sums wrap."""
+    sizes (24576 beside products, 32768 alone), and values whose sums wrap."""
     from skelcl_amd import _lib
     rng = np.random.default_rng({"fixture": 11, "dense_tiny": 12, "tile_edges": 13,
                                  "wrap": 14}[case])
