@@ -5,8 +5,6 @@
 // exactly modulo 2^32.
 #pragma once
 
-#include <utility>
-
 #include "Reduce.h"
 #include "Scan.h"
 #include "Vector.h"
