@@ -10,7 +10,8 @@ oracle on the same seeded inputs:
   rows against FP64 (normwise), and squared-Euclidean AllPairs on 16384
   points of dim 256 on sampled rows;
 * the batched configuration in full: 4096 segments (the committed lengths)
-  for segmented Reduce and Scan, and the 512 products.
+  for segmented Reduce and Scan, and the 512 products; and the same in one
+  skl_batch_run launch.
 """
 import ctypes as C
 import json
@@ -224,3 +225,67 @@ def test_reduce_all_sizes_8192_to_8447(skl):  # ReduceTests.cpp:155-170, every s
         _lib.check(_lib.lib().skl_reduce_run(r.handle, out.ptr, buf.ptr, n, None, 0, None))
         out.download(res)
         assert res[0] == n, n
+
+
+def test_batch_one_launch_full_config(skl):
+    """The batched configuration in one skl_batch_run launch: all 4096
+    segments (sums and inclusive scan from one read of the 5.8e8 elements)
+    and all 512 products at BF16X3, two product work-groups per CU.  Every
+    sum against FP64, every segment's last scan value against its sum, 48
+    whole segments of the scan against FP64 prefixes, no offsets flagged,
+    and the products bit for bit against skl_matmul_batched_f32 (the same
+    tile code)."""
+    cfg = _cfg5()
+    lens = np.array(cfg["segment_lengths"], np.int64)
+    off = np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)
+    n, nseg = int(off[-1]), lens.size
+    x = oracle.gen_f32_sym(0x5012, n)
+    X, O = skl.DeviceBuffer.from_array(x), skl.DeviceBuffer.from_array(off)
+    S, Y = skl.DeviceBuffer(4 * nseg), skl.DeviceBuffer(4 * n)
+    shapes = cfg["gemm_shapes"]
+    d1 = (_lib.skl_gemm_desc * len(shapes))()
+    d2 = (_lib.skl_gemm_desc * len(shapes))()
+    keep, outs = [], []
+    for i, (M, N, K) in enumerate(shapes):
+        dA = skl.DeviceBuffer.from_array(oracle.gen_f32_sym(0x5104 + 2 * i, M * K))
+        dB = skl.DeviceBuffer.from_array(oracle.gen_f32_sym(0x5105 + 2 * i, K * N))
+        c1, c2 = skl.DeviceBuffer(4 * M * N), skl.DeviceBuffer(4 * M * N)
+        for d, c in ((d1, c1), (d2, c2)):
+            d[i].A, d[i].B, d[i].C = dA.ptr, dB.ptr, c.ptr
+            d[i].M, d[i].N, d[i].K = M, N, K
+        keep += [dA, dB]
+        outs.append((c1, c2, M * N))
+    dd = []
+    for d in (d1, d2):
+        b = skl.DeviceBuffer(C.sizeof(d))
+        _lib.check(_lib.lib().skl_memcpy_h2d(b.ptr, 0, C.cast(d, C.c_void_p), C.sizeof(d),
+                                             None, 1))
+        dd.append(b)
+    r = skl.Reduce("float(float)", ADD_F)
+    s = skl.Scan("float(float)", ADD_F, "0.0f")
+    _lib.check(_lib.lib().skl_batch_run(r.handle, S.ptr, s.handle, Y.ptr, 1, X.ptr, n, O.ptr,
+                                        nseg, dd[0].ptr, len(shapes), 2, 2, None))
+    bad = C.c_int(-1)
+    _lib.check(_lib.lib().skl_batch_check(s.handle, C.byref(bad)))
+    assert bad.value == 0
+    _lib.check(_lib.lib().skl_matmul_batched_f32(dd[1].ptr, len(shapes), 2, None))
+    for c1, c2, mn in outs:
+        assert np.array_equal(c1.to_numpy(np.float32, mn).view(np.uint32),
+                              c2.to_numpy(np.float32, mn).view(np.uint32))
+    sums = oracle.segmented_sum_f64(x, off)
+    got = S.to_numpy(np.float32, nseg).astype(np.float64)
+    tol = np.spacing(np.abs(sums).astype(np.float32)).astype(np.float64) + 1e-12
+    assert np.all(np.abs(got - sums) <= tol)
+    out = Y.to_numpy(np.float32, n)
+    last = out[off[1:] - 1].astype(np.float64)
+    scale = np.array([np.abs(x[a:b]).astype(np.float64).sum() for a, b in zip(off[:-1], off[1:])])
+    assert np.all(np.abs(last - sums) <= np.spacing(np.abs(sums).astype(np.float32))
+                  + 1e-7 * scale * 2.0 ** -10)
+    order = np.argsort(lens)
+    rng = np.random.default_rng(0x5013)
+    pick = np.unique(np.concatenate([order[:8], order[-8:], rng.integers(0, nseg, 32)]))
+    for k in pick:
+        a, b = int(off[k]), int(off[k + 1])
+        ref = np.cumsum(x[a:b].astype(np.float64))
+        t = np.spacing(np.abs(ref).astype(np.float32)).astype(np.float64) + 1e-9
+        assert np.all(np.abs(out[a:b].astype(np.float64) - ref) <= t), k
