@@ -1,8 +1,9 @@
 """Seeded random shapes for the extensions whose layouts have the most
 cases: the one-launch batch (segment layouts with empty and uncovered
 parts, with and without products), the two-launch segmented Reduce/Scan
-(float and int32), and the native weighted-stencil forward and adjoint at
-random sizes, radii, paddings and rows per thread.  Integer results are
+(float and int32), the native weighted-stencil forward and adjoint at
+random sizes, radii, paddings and rows per thread, and dense products in
+the three precision forms at random ragged shapes and transposes.  Integer results are
 checked bit for bit against the oracle, the stencil forward bit for bit
 against the oracle's fmaf chain, float sums and scans within one float ulp
 of FP64, products normwise against FP64.  Every case prints its seed and
@@ -170,3 +171,39 @@ def test_random_stencil_forward_and_adjoint(skl, case):
     mult = (R + 1) ** 2 if padding == oracle.NEAREST else 1
     scale = np.abs(w).sum() * np.abs(g).max() * mult
     assert np.max(np.abs(d - dref)) <= 4e-6 * max(scale, 1e-30)
+
+
+@pytest.mark.parametrize("case", range(18))
+def test_random_products_all_forms(skl, case):
+    """Dense C = A.B at random ragged shapes and transposes in the three
+    forms, against FP64 (normwise: FP32 and BF16X3 1e-6, FP8 5e-2), and the
+    MFMA forms' work-group tiles against each other bit for bit."""
+    rng = np.random.default_rng(0xE000 + case)
+    M, N, K = (int(v) for v in rng.integers(1, 900, 3))
+    ta, tb = int(rng.integers(0, 2)), int(rng.integers(0, 2))
+    prec = (skl.PREC_FP32, skl.PREC_BF16X3, skl.PREC_FP8)[case % 3]
+    print(f"case {case}: M={M} N={N} K={K} ta={ta} tb={tb} prec={prec}", flush=True)
+    A = oracle.gen_f32_sym(0xE100 + case, M * K).reshape((K, M) if ta else (M, K))
+    B = oracle.gen_f32_sym(0xE200 + case, K * N).reshape((N, K) if tb else (K, N))
+    ref = (A.T if ta else A).astype(np.float64) @ (B.T if tb else B).astype(np.float64)
+    dA, dB = skl.DeviceBuffer.from_array(A), skl.DeviceBuffer.from_array(B)
+    dC = skl.DeviceBuffer(4 * M * N)
+    _lib.check(_lib.lib().skl_matmul_f32(dC.ptr, dA.ptr, dB.ptr, M, N, K, ta, tb, prec, None))
+    c = dC.to_numpy(np.float32, M * N).reshape(M, N)
+    err = np.linalg.norm(c - ref) / max(np.linalg.norm(ref), 1e-30)
+    assert err < (5e-2 if prec == skl.PREC_FP8 else 1e-6), err
+    if prec == skl.PREC_FP32:
+        return
+    if ta or tb:
+        return  # the tile is chosen through AllPairs, which takes untransposed operands
+    add = skl.Reduce("float(float)", ADD_F, "0")
+    mul = skl.Zip("float(float, float)", "float func(float x, float y){ return x*y; }")
+    ap = skl.AllPairs("float(float, float)", reduce=add, zip=mul, precision=prec)
+    mA, mB = skl.Matrix(A), skl.Matrix(B)
+    outs = []
+    for t in (1, 2, 3):
+        _lib.check(_lib.lib().skl_skeleton_set_mma_tile(ap.handle, t))
+        outs.append(np.asarray(ap(mA, mB)).view(np.uint32).copy())
+    for o in outs[1:]:
+        assert np.array_equal(o, outs[0])
+    assert np.array_equal(outs[0], c.view(np.uint32))
