@@ -316,7 +316,7 @@ int skl_matmul_batched_f32(const skl_gemm_desc* descs_device, int count,
  * synchronisation: the segment part is one pass over tiles of 24576
  * elements beside products, 32768 alone (a decoupled look-back; the sums are the inclusive values at
  * the segments' ends, so the array is read once for both), and
- * workers_per_cu (0 = 1, or 2) work-groups per CU walk the product tiles at
+ * workers_per_cu (1, or 2 = the default 0) work-groups per CU walk the product tiles at
  * the same time.  Float Reduce and Scan must share a precision form
  * (DEFAULT: compensated FP32 in a tile, FP64 across tiles; FP64).  The
  * offsets are checked by the tiles themselves: after invalid offsets the
