@@ -2,8 +2,9 @@
 cases: the one-launch batch (segment layouts with empty and uncovered
 parts, with and without products), the two-launch segmented Reduce/Scan
 (float and int32), the native weighted-stencil forward and adjoint at
-random sizes, radii, paddings and rows per thread, and dense products in
-the three precision forms at random ragged shapes and transposes.  Integer results are
+random sizes, radii, paddings and rows per thread, dense products in
+the three precision forms at random ragged shapes and transposes, and the
+plain Reduce and Scan at random lengths.  Integer results are
 checked bit for bit against the oracle, the stencil forward bit for bit
 against the oracle's fmaf chain, float sums and scans within one float ulp
 of FP64, products normwise against FP64.  Every case prints its seed and
@@ -207,3 +208,35 @@ def test_random_products_all_forms(skl, case):
     for o in outs[1:]:
         assert np.array_equal(o, outs[0])
     assert np.array_equal(outs[0], c.view(np.uint32))
+
+
+@pytest.mark.parametrize("case", range(16))
+def test_random_reduce_scan_sizes(skl, case):
+    """Reduce(+) and Scan(+) at random lengths (tile remainders of every
+    kind): int32 bit for bit against the oracle (modulo 2^32, values that
+    wrap), float within one float ulp of FP64 (DEFAULT and FP64 forms)."""
+    rng = np.random.default_rng(0xF000 + case)
+    n = int(rng.integers(1, 5_000_000))
+    inclusive = bool(rng.integers(0, 2))
+    print(f"case {case}: n={n} inclusive={inclusive}", flush=True)
+    xi = oracle.gen_u64(0xF100 + case, n).astype(np.uint32).view(np.int32)
+    Xi = skl.Vector(xi)
+    r = skl.Reduce("int(int)", ADD_I)
+    s = skl.Scan("int(int)", ADD_I, "0")
+    assert np.uint32(np.asarray(r(Xi))[0]) == np.uint32(xi.view(np.uint32).sum(dtype=np.uint64)
+                                                        & 0xFFFFFFFF)
+    got = np.asarray(s(Xi, inclusive=inclusive)).view(np.uint32)
+    assert np.array_equal(got, oracle.scan_add_u32(xi, inclusive))
+    xf = oracle.gen_f32_sym(0xF200 + case, n)
+    Xf = skl.Vector(xf)
+    ref_sum = oracle.sum_f64(xf)
+    ref_scan = oracle.scan_add_f64(xf, inclusive)
+    tol = np.spacing(np.abs(ref_scan).astype(np.float32)).astype(np.float64) + 1e-9
+    for form in ("DEFAULT", "FP64"):
+        P = getattr(skl, "PREC_" + form)
+        rf = skl.Reduce("float(float)", ADD_F, precision=P)
+        sf = skl.Scan("float(float)", ADD_F, "0.0f", precision=P)
+        v = float(np.asarray(rf(Xf))[0])
+        assert abs(v - ref_sum) <= np.spacing(np.float32(abs(ref_sum))) + 1e-12, form
+        y = np.asarray(sf(Xf, inclusive=inclusive)).astype(np.float64)
+        assert np.all(np.abs(y - ref_scan) <= tol), form
