@@ -3,8 +3,9 @@ cases: the one-launch batch (segment layouts with empty and uncovered
 parts, with and without products), the two-launch segmented Reduce/Scan
 (float and int32), the native weighted-stencil forward and adjoint at
 random sizes, radii, paddings and rows per thread, dense products in
-the three precision forms at random ragged shapes and transposes, and the
-plain Reduce and Scan at random lengths.  Integer results are
+the three precision forms at random ragged shapes and transposes, the
+plain Reduce and Scan at random lengths, and the generated MapOverlap
+kernel on int32 with random getData combinations.  Integer results are
 checked bit for bit against the oracle, the stencil forward bit for bit
 against the oracle's fmaf chain, float sums and scans within one float ulp
 of FP64, products normwise against FP64.  Every case prints its seed and
@@ -240,3 +241,43 @@ def test_random_reduce_scan_sizes(skl, case):
         assert abs(v - ref_sum) <= np.spacing(np.float32(abs(ref_sum))) + 1e-12, form
         y = np.asarray(sf(Xf, inclusive=inclusive)).astype(np.float64)
         assert np.all(np.abs(y - ref_scan) <= tol), form
+
+
+@pytest.mark.parametrize("case", range(12))
+def test_random_mapoverlap_generated_int(skl, case):
+    """The generated MapOverlap kernel (any user function) at random sizes,
+    radii, rows per thread and both paddings, on int32 with a random
+    combination of getData offsets: bit for bit against numpy over a padded
+    copy, whose padding is checked against the oracle's getData first."""
+    rng = np.random.default_rng(0xA000 + case)
+    rows, cols = (int(v) for v in rng.integers(1, 400, 2))
+    R = int(rng.integers(1, 7))
+    padding = int(rng.integers(0, 2))
+    neutral = int(rng.integers(-50, 50))
+    rb = int(rng.choice([1, 2, 4, 6, 8]))
+    terms = [(int(rng.integers(-R, R + 1)), int(rng.integers(-R, R + 1)),
+              int(rng.integers(-9, 10))) for _ in range(5)]
+    print(f"case {case}: {rows}x{cols} R={R} padding={padding} neutral={neutral} rb={rb} "
+          f"terms={terms}", flush=True)
+    body = " + ".join(f"({k}) * getData(img, {dx}, {dy})" for dx, dy, k in terms)
+    src = f"int func(input_matrix_t img) {{ return {body}; }}"
+    img = oracle.gen_i32_below(0xA100 + case, rows * cols, 1000).reshape(rows, cols)
+    m = skl.MapOverlap("int(int)", src, R, padding, neutral)
+    sh = (C.c_int * 4)()
+    _lib.check(_lib.lib().skl_skeleton_launch_shape(m.handle, sh))
+    sh[3] = rb
+    _lib.check(_lib.lib().skl_skeleton_set_launch_shape(m.handle, sh))
+    got = np.asarray(m(skl.Matrix(img))).astype(np.int64)
+    if padding == oracle.NEAREST:
+        pad = np.pad(img.astype(np.int64), R, mode="edge")
+    else:
+        pad = np.pad(img.astype(np.int64), R, mode="constant", constant_values=neutral)
+    for _ in range(16):
+        r, c = int(rng.integers(0, rows)), int(rng.integers(0, cols))
+        dx, dy = int(rng.integers(-R, R + 1)), int(rng.integers(-R, R + 1))
+        assert pad[R + r + dy, R + c + dx] == oracle.getdata_i32(img, r, c, dx, dy, padding,
+                                                                 neutral)
+    ref = np.zeros((rows, cols), np.int64)
+    for dx, dy, k in terms:
+        ref += k * pad[R + dy:R + dy + rows, R + dx:R + dx + cols]
+    assert np.array_equal(got, ref)
