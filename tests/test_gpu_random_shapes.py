@@ -4,8 +4,9 @@ parts, with and without products), the two-launch segmented Reduce/Scan
 (float and int32), the native weighted-stencil forward and adjoint at
 random sizes, radii, paddings and rows per thread, dense products in
 the three precision forms at random ragged shapes and transposes, the
-plain Reduce and Scan at random lengths, and the generated MapOverlap
-kernel on int32 with random getData combinations.  Integer results are
+plain Reduce and Scan at random lengths, the generated MapOverlap kernel
+on int32 with random getData combinations, and AllPairs with a user
+function at random ragged sizes.  Integer results are
 checked bit for bit against the oracle, the stencil forward bit for bit
 against the oracle's fmaf chain, float sums and scans within one float ulp
 of FP64, products normwise against FP64.  Every case prints its seed and
@@ -281,3 +282,27 @@ def test_random_mapoverlap_generated_int(skl, case):
     for dx, dy, k in terms:
         ref += k * pad[R + dy:R + dy + rows, R + dx:R + dx + cols]
     assert np.array_equal(got, ref)
+
+
+@pytest.mark.parametrize("case", range(12))
+def test_random_allpairs_general_int(skl, case):
+    """AllPairs with a user function (the general kernels: column-staged for
+    dim <= 256, the LDS and plain forms beyond) at random ragged sizes, int32
+    with a weighted L1 distance and an extra argument: bit for bit against
+    numpy."""
+    rng = np.random.default_rng(0x9000 + case)
+    M, N = (int(v) for v in rng.integers(1, 300, 2))
+    D = int(rng.integers(1, 400))
+    k = int(rng.integers(-5, 6))
+    print(f"case {case}: M={M} N={N} D={D} k={k}", flush=True)
+    src = ("int func(lmatrix_t* row, rmatrix_t* col, const unsigned int dim, int k) {"
+           " int s = 0; for (unsigned int i = 0; i < dim; ++i) {"
+           " int d = getElementFromRow(row, i) - getElementFromColumn(col, i);"
+           " s += (d < 0 ? -d : d) + k * (int)i; } return s; }")
+    A = oracle.gen_i32_below(0x9100 + case, M * D, 2000).reshape(M, D) - 1000
+    B = oracle.gen_i32_below(0x9200 + case, D * N, 2000).reshape(D, N) - 1000
+    ap = skl.AllPairs("int(int, int)", src)
+    got = np.asarray(ap(skl.Matrix(A.astype(np.int32)), skl.Matrix(B.astype(np.int32)), k))
+    ref = (np.abs(A.astype(np.int64)[:, :, None] - B.astype(np.int64)[None, :, :]).sum(axis=1)
+           + k * (D * (D - 1) // 2))
+    assert np.array_equal(got.astype(np.int64), ref)
