@@ -5,8 +5,8 @@ parts, with and without products), the two-launch segmented Reduce/Scan
 random sizes, radii, paddings and rows per thread, dense products in
 the three precision forms at random ragged shapes and transposes, the
 plain Reduce and Scan at random lengths, the generated MapOverlap kernel
-on int32 with random getData combinations, and AllPairs with a user
-function at random ragged sizes.  Integer results are
+on int32 with random getData combinations, AllPairs with a user
+function at random ragged sizes, and Map and Zip at random lengths.  Integer results are
 checked bit for bit against the oracle, the stencil forward bit for bit
 against the oracle's fmaf chain, float sums and scans within one float ulp
 of FP64, products normwise against FP64.  Every case prints its seed and
@@ -306,3 +306,25 @@ def test_random_allpairs_general_int(skl, case):
     ref = (np.abs(A.astype(np.int64)[:, :, None] - B.astype(np.int64)[None, :, :]).sum(axis=1)
            + k * (D * (D - 1) // 2))
     assert np.array_equal(got.astype(np.int64), ref)
+
+
+@pytest.mark.parametrize("case", range(10))
+def test_random_map_zip_int(skl, case):
+    """Map and Zip with an extra argument at random lengths (every tail of
+    the 16-byte packs), int32 arithmetic that wraps: bit for bit against
+    numpy modulo 2^32."""
+    rng = np.random.default_rng(0x8000 + case)
+    n = int(rng.integers(1, 3_000_000))
+    a = int(rng.integers(-100000, 100000))
+    print(f"case {case}: n={n} a={a}", flush=True)
+    x = oracle.gen_u64(0x8100 + case, n).astype(np.uint32).view(np.int32)
+    y = oracle.gen_u64(0x8200 + case, n).astype(np.uint32).view(np.int32)
+    mp = skl.Map("int(int)", "int func(int x, int a){ return a * x + 3; }")
+    zp = skl.Zip("int(int, int)", "int func(int x, int y, int a){ return x * a - (y ^ a); }")
+    got_m = np.asarray(mp(skl.Vector(x), np.int32(a))).view(np.uint32)
+    got_z = np.asarray(zp(skl.Vector(x), skl.Vector(y), np.int32(a))).view(np.uint32)
+    m32 = 0xFFFFFFFF
+    ref_m = ((a * x.astype(np.int64) + 3) & m32).astype(np.uint32)
+    ref_z = ((x.astype(np.int64) * a - (y ^ np.int32(a)).astype(np.int64)) & m32).astype(np.uint32)
+    assert np.array_equal(got_m, ref_m)
+    assert np.array_equal(got_z, ref_z)
